@@ -1,0 +1,873 @@
+// Collocation assembly on gfx950.
+//
+// The reference (atomicFuncs.cu:69-298) launches one thread per (collocation
+// row, element) pair and scatters every result with float atomicAdd into A and
+// B, so the summation order changes from run to run.  Here a wave owns a block
+// of consecutive rows (one row per lane) and a column tile (a compact patch of
+// nodes); it walks the tile's element list in a fixed order and accumulates
+// straight into its own rows of A in HBM.  No other wave writes those entries,
+// so there are no atomics and the result is bitwise repeatable.  Writes are
+// coalesced: the 64 lanes of a wave write 64 consecutive rows of one column.
+//
+// Kernels:
+//   prep_elems      one thread per element: p3, e1, e2, unit normal, J, ca/cb,
+//                   cc/cb -- the element-uniform data the quadrature reads with
+//                   scalar loads;
+//   sing_elems      one thread per (element, vertex): the vertex-collapsed rule
+//                   of the reference (mesh.cu:253-595) in FP64;
+//   asm_tile        the regular 6x6 rule (mesh.cu:597-672) for every (row,
+//                   element) pair of a tile, FP32 (or FP64), with the singular
+//                   lanes taking the FP64 values;
+//   asm_finalize    per row: 1 - sum c on the node diagonal
+//                   (atomicFuncs.cu:227-235, 99-102), and the boundary-condition
+//                   vector sum_e (cc/cb) sum_j g_j (atomicFuncs.cu:104-115);
+//   rhs_fill        B[:, s] = incident_s - bterm.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "bem_internal.hpp"
+
+namespace bem {
+
+// ------------------------------------------------------------ device math ---
+// sin(2 pi t), cos(2 pi t) for t >= 0 in revolutions, FP32: the quadrant is
+// removed exactly (u = t - j/4 with j = rint(4t) is exact in binary floating
+// point), then near-minimax polynomials in u on [-1/8, 1/8] (coefficients fitted
+// for this library; max abs error 9e-8 after FP32 rounding).
+__device__ __forceinline__ void sincos_rev(float t, float &sn, float &cs) {
+    const float j = __builtin_rintf(4.0f * t);
+    const float u = __builtin_fmaf(j, -0.25f, t);
+    const float u2 = u * u;
+    float s = __builtin_fmaf(u2, -75.24006652832031f, 81.58812713623047f);
+    s = __builtin_fmaf(u2, s, -41.34162902832031f);
+    s = __builtin_fmaf(u2, s, 6.283185005187988f);
+    s = u * s;
+    float c = __builtin_fmaf(u2, 59.24602508544922f, -85.44374084472656f);
+    c = __builtin_fmaf(u2, c, 64.93932342529297f);
+    c = __builtin_fmaf(u2, c, -19.739208221435547f);
+    c = __builtin_fmaf(u2, c, 1.0f);
+    const int q = (int)j;
+    const float ss = (q & 1) ? c : s;
+    const float cc = (q & 1) ? s : c;
+    sn = (q & 2) ? -ss : ss;
+    cs = ((q + 1) & 2) ? -cc : cc;
+}
+
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rsq(double x) { return 1.0 / sqrt(x); }
+
+// ------------------------------------------------------------- arguments ---
+template <typename R>
+struct QTab {
+    R xi1[kNQ], xi2[kNQ], w[kNQ];
+};
+
+struct SingTab {
+    double xi1[3][kNQ], xi2[3][kNQ], w[kNQ];
+};
+
+template <typename R>
+struct AsmArgs {
+    const float4 *__restrict__ pts;
+    const ElemRec<R> *__restrict__ recs;
+    const R *__restrict__ sing;  // (e*3+v)*16: p[0..5], c[6], bterm[7..8]
+    const int *__restrict__ tile_ptr;
+    const int *__restrict__ tile_entries;
+    typename std::conditional<sizeof(R) == 4, float2, double2>::type *A;
+    R *cpart;
+    typename std::conditional<sizeof(R) == 4, float2, double2>::type *bpart;
+    long ldA;
+    long Mp;
+    int M, N, nrbg, nwg;
+    R k, kf, rw;  // k, k/(2 pi), rho*omega
+    int impedance;  // any element with ca != 0
+    QTab<R> q;
+};
+
+// Workgroup id -> logical id, so that consecutive logical ids (row blocks of the
+// same tile, which share the tile's element records) land on the same XCD.
+// Dispatch places workgroup g on XCD g % 8.
+__device__ __forceinline__ int xcd_remap(int g, int n) {
+    const int per = n / 8, rem = n % 8;
+    const int xcd = g % 8, idx = g / 8;
+    return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+// --------------------------------------------------------- element prep ---
+template <typename R>
+__global__ void prep_elems(const double *__restrict__ pts64, const int *__restrict__ elems,
+                           const float2 *__restrict__ bc, int E, ElemRec<R> *__restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int n0 = elems[3 * e], n1 = elems[3 * e + 1], n2 = elems[3 * e + 2];
+    double p1[3], p2[3], p3[3], e1[3], e2[3];
+    for (int d = 0; d < 3; d++) {
+        p1[d] = pts64[3 * n0 + d];
+        p2[d] = pts64[3 * n1 + d];
+        p3[d] = pts64[3 * n2 + d];
+        e1[d] = p1[d] - p3[d];
+        e2[d] = p2[d] - p3[d];
+    }
+    // crossProd = (p1-p3) x (p2-p3), normal = crossProd / |crossProd| (mesh.cu:612-617)
+    double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    double J = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    ElemRec<R> r;
+    for (int d = 0; d < 3; d++) {
+        r.p3[d] = (R)p3[d];
+        r.e1[d] = (R)e1[d];
+        r.e2[d] = (R)e2[d];
+        r.n[d] = (R)(cr[d] / J);
+    }
+    r.J = (R)J;
+    // bc = ca/cb and cc/cb (atomicFuncs.cu:87, 105), complex division
+    const float2 ca = bc[3 * e], cb = bc[3 * e + 1], cc = bc[3 * e + 2];
+    const double den = (double)cb.x * cb.x + (double)cb.y * cb.y;
+    r.beta[0] = (R)(((double)ca.x * cb.x + (double)ca.y * cb.y) / den);
+    r.beta[1] = (R)(((double)ca.y * cb.x - (double)ca.x * cb.y) / den);
+    r.gamma[0] = (R)(((double)cc.x * cb.x + (double)cc.y * cb.y) / den);
+    r.gamma[1] = (R)(((double)cc.y * cb.x - (double)cc.x * cb.y) / den);
+    r.nodes[0] = n0;
+    r.nodes[1] = n1;
+    r.nodes[2] = n2;
+    for (int i = 0; i < (int)(sizeof(r.pad) / sizeof(int)); i++) r.pad[i] = 0;
+    out[e] = r;
+}
+
+// ------------------------------------------------------ singular, FP64 ---
+// One thread per (element, vertex v): the reference's vertex rule for x = p_v,
+// g_l_sgl{v+1}/h_l_sgl{v+1}/c_l_sgl{v+1} (mesh.cu:253-595), in FP64.  Stores
+// the A contributions h_j - (ca/cb) g_j (j = 0..2), c, and (cc/cb) sum_j g_j.
+template <typename Rout>
+__global__ void sing_elems(const double *__restrict__ pts64, const int *__restrict__ elems,
+                           const float2 *__restrict__ bc, int E, double k, double rw, SingTab tab,
+                           Rout *__restrict__ out) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= 3 * E) return;
+    const int e = id / 3, v = id % 3;
+    double p[3][3];
+    for (int j = 0; j < 3; j++)
+        for (int d = 0; d < 3; d++) p[j][d] = pts64[3 * elems[3 * e + j] + d];
+    double e1[3], e2[3];
+    for (int d = 0; d < 3; d++) {
+        e1[d] = p[0][d] - p[2][d];
+        e2[d] = p[1][d] - p[2][d];
+    }
+    double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double J = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    const double nx = cr[0] / J, ny = cr[1] / J, nz = cr[2] / J;
+    const double *x = p[v];
+    const double inv4pi = 1.0 / (4.0 * kPi);
+    double gr[3] = {0, 0, 0}, gi[3] = {0, 0, 0}, hr[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, c = 0;
+    for (int q = 0; q < kNQ; q++) {
+        const double xi1 = tab.xi1[v][q], xi2 = tab.xi2[v][q];
+        const double N[3] = {xi1, xi2, 1.0 - xi1 - xi2};
+        const double temp = tab.w[q] * J;  // 0.25 w_n w_m rho J
+        double rv[3];
+        for (int d = 0; d < 3; d++) rv[d] = N[0] * p[0][d] + N[1] * p[1][d] + N[2] * p[2][d] - x[d];
+        const double r = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+        const double prpn2 = (rv[0] * nx + rv[1] * ny + rv[2] * nz) / r;
+        double sn, cs;
+        sincos(k * r, &sn, &cs);
+        const double Gr = cs * inv4pi / r, Gi = -sn * inv4pi / r;  // exp(-ikr)/(4 pi r)
+        // dG/dn = G (-1/r - ik) dr/dn
+        const double Hr = (Gr * (-1.0 / r) + Gi * k) * prpn2;
+        const double Hi = (Gi * (-1.0 / r) - Gr * k) * prpn2;
+        const double dG0 = -inv4pi / (r * r) * prpn2;
+        for (int j = 0; j < 3; j++) {
+            gr[j] += temp * N[j] * Gr;
+            gi[j] += temp * N[j] * Gi;
+            hr[j] += temp * N[j] * Hr;
+            hi[j] += temp * N[j] * Hi;
+        }
+        c += temp * dG0;
+    }
+    // g_j *= i rho omega (mesh.cu:668-670)
+    double Gjr[3], Gji[3];
+    for (int j = 0; j < 3; j++) {
+        Gjr[j] = -rw * gi[j];
+        Gji[j] = rw * gr[j];
+    }
+    const float2 ca = bc[3 * e], cb = bc[3 * e + 1], cc = bc[3 * e + 2];
+    const double den = (double)cb.x * cb.x + (double)cb.y * cb.y;
+    const double br = ((double)ca.x * cb.x + (double)ca.y * cb.y) / den;
+    const double bi = ((double)ca.y * cb.x - (double)ca.x * cb.y) / den;
+    const double yr = ((double)cc.x * cb.x + (double)cc.y * cb.y) / den;
+    const double yi = ((double)cc.y * cb.x - (double)cc.x * cb.y) / den;
+    Rout *o = out + (size_t)id * 16;
+    double sgr = 0, sgi = 0;
+    for (int j = 0; j < 3; j++) {
+        o[2 * j] = (Rout)(hr[j] - (br * Gjr[j] - bi * Gji[j]));
+        o[2 * j + 1] = (Rout)(hi[j] - (br * Gji[j] + bi * Gjr[j]));
+        sgr += Gjr[j];
+        sgi += Gji[j];
+    }
+    o[6] = (Rout)c;
+    o[7] = (Rout)(yr * sgr - yi * sgi);
+    o[8] = (Rout)(yr * sgi + yi * sgr);
+    for (int i = 9; i < 16; i++) o[i] = 0;
+}
+
+// ------------------------------------------------- regular quadrature ---
+template <typename R>
+__device__ __forceinline__ void sincos_phase(R r, R k, R kf, R &sn, R &cs);
+template <>
+__device__ __forceinline__ void sincos_phase<float>(float r, float, float kf, float &sn, float &cs) {
+    sincos_rev(r * kf, sn, cs);
+}
+template <>
+__device__ __forceinline__ void sincos_phase<double>(double r, double k, double, double &sn, double &cs) {
+    sincos(k * r, &sn, &cs);
+}
+
+template <typename R>
+__device__ __forceinline__ R shfl_xor(R v, int m) {
+    return __shfl_xor(v, m, 64);
+}
+
+// One wave = RPW = 64/QG consecutive rows; QG lanes share a row and split the
+// 36 quadrature points (QG = 1: a lane does all of them).  A workgroup is 4
+// waves on 4 consecutive row blocks of the same tile.
+template <typename R, int QG, bool IMP>
+__global__ __launch_bounds__(256) void asm_tile(AsmArgs<R> a) {
+    using C2 = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
+    constexpr int RPW = 64 / QG;
+    constexpr int NQL = kNQ / QG;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int L = xcd_remap(blockIdx.x, a.nwg);
+    const int tile = L / a.nrbg;
+    const int rbg = L - tile * a.nrbg;
+    const int row = (rbg * 4 + wave) * RPW + lane / QG;
+    const int qg = (QG == 1) ? 0 : (lane % QG);
+    const bool active = row < a.M;
+    const float4 xp = a.pts[active ? row : 0];
+    const R x0 = xp.x, x1 = xp.y, x2 = xp.z;
+
+    R txi1[NQL], txi2[NQL], tw[NQL];
+#pragma unroll
+    for (int i = 0; i < NQL; i++) {
+        const int q = qg + QG * i;
+        txi1[i] = a.q.xi1[q];
+        txi2[i] = a.q.xi2[q];
+        tw[i] = a.q.w[q];
+    }
+    const R k = a.k, kf = a.kf;
+    const R inv4pi = (R)(1.0 / (4.0 * kPi));
+    R c_acc = 0, br_acc = 0, bi_acc = 0;
+
+    const int it0 = a.tile_ptr[tile], it1 = a.tile_ptr[tile + 1];
+    for (int it = it0; it < it1; ++it) {
+        const int ent = a.tile_entries[it];
+        const int e = ent >> 8;
+        const ElemRec<R> &rec = a.recs[e];
+        const R ax = rec.p3[0] - x0, ay = rec.p3[1] - x1, az = rec.p3[2] - x2;
+        const R e10 = rec.e1[0], e11 = rec.e1[1], e12 = rec.e1[2];
+        const R e20 = rec.e2[0], e21 = rec.e2[1], e22 = rec.e2[2];
+        // (y - x).n is constant over a flat element: n.(p3 - x)
+        const R dn = rec.n[0] * ax + rec.n[1] * ay + rec.n[2] * az;
+        R h1r = 0, h1i = 0, h2r = 0, h2i = 0, htr = 0, hti = 0, gr = 0, gi = 0, cc = 0;
+        R g1r = 0, g1i = 0, g2r = 0, g2i = 0;
+#pragma unroll
+        for (int i = 0; i < NQL; i++) {
+            const R dx = fma(txi1[i], e10, fma(txi2[i], e20, ax));
+            const R dy = fma(txi1[i], e11, fma(txi2[i], e21, ay));
+            const R dz = fma(txi1[i], e12, fma(txi2[i], e22, az));
+            const R r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            const R rinv = rsq(r2);
+            const R r = r2 * rinv;
+            R sn, cs;
+            sincos_phase<R>(r, k, kf, sn, cs);
+            // 4 pi G = (cos kr - i sin kr) / r
+            const R Gr = cs * rinv, Gi = -sn * rinv;
+            const R w2 = tw[i] * rinv;
+            // 4 pi r G (1/r + ik), weighted; dG/dn = -(dn/4pi) rinv G~ (rinv + ik)
+            const R Hr = (fma(Gr, rinv, -k * Gi)) * w2;
+            const R Hi = (fma(Gi, rinv, k * Gr)) * w2;
+            htr += Hr;
+            hti += Hi;
+            h1r = fma(txi1[i], Hr, h1r);
+            h1i = fma(txi1[i], Hi, h1i);
+            h2r = fma(txi2[i], Hr, h2r);
+            h2i = fma(txi2[i], Hi, h2i);
+            const R wGr = tw[i] * Gr, wGi = tw[i] * Gi;
+            gr += wGr;
+            gi += wGi;
+            if (IMP) {
+                g1r = fma(txi1[i], wGr, g1r);
+                g1i = fma(txi1[i], wGi, g1i);
+                g2r = fma(txi2[i], wGr, g2r);
+                g2i = fma(txi2[i], wGi, g2i);
+            }
+            cc = fma(w2 * rinv, rinv, cc);
+        }
+        if (QG > 1) {
+#pragma unroll
+            for (int m = 1; m < QG; m <<= 1) {
+                h1r += shfl_xor(h1r, m); h1i += shfl_xor(h1i, m);
+                h2r += shfl_xor(h2r, m); h2i += shfl_xor(h2i, m);
+                htr += shfl_xor(htr, m); hti += shfl_xor(hti, m);
+                gr += shfl_xor(gr, m); gi += shfl_xor(gi, m);
+                cc += shfl_xor(cc, m);
+                if (IMP) {
+                    g1r += shfl_xor(g1r, m); g1i += shfl_xor(g1i, m);
+                    g2r += shfl_xor(g2r, m); g2i += shfl_xor(g2i, m);
+                }
+            }
+        }
+        const R sJ = rec.J * inv4pi;
+        const R sh = -dn * sJ;
+        // h_j (mesh.cu:661-663), N3 = 1 - N1 - N2
+        R p0r = sh * h1r, p0i = sh * h1i;
+        R p1r = sh * h2r, p1i = sh * h2i;
+        R p2r = sh * (htr - h1r - h2r), p2i = sh * (hti - h1i - h2i);
+        // sum_j g_j = i rho omega (J/4pi) sum w G~  (mesh.cu:657-659, 668-670)
+        const R sg = a.rw * sJ;
+        const R gsr = -sg * gi, gsi = sg * gr;
+        if (IMP) {
+            // p_j = h_j - beta g_j
+            const R br = rec.beta[0], bi = rec.beta[1];
+            const R G0r = -sg * g1i, G0i = sg * g1r;
+            const R G1r = -sg * g2i, G1i = sg * g2r;
+            const R G2r = gsr - G0r - G1r, G2i = gsi - G0i - G1i;
+            p0r -= br * G0r - bi * G0i; p0i -= br * G0i + bi * G0r;
+            p1r -= br * G1r - bi * G1i; p1i -= br * G1i + bi * G1r;
+            p2r -= br * G2r - bi * G2i; p2i -= br * G2i + bi * G2r;
+        }
+        R cval = -dn * sJ * cc;  // c_l_nsgl (mesh.cu:645, 665)
+        R btr = rec.gamma[0] * gsr - rec.gamma[1] * gsi;
+        R bti = rec.gamma[0] * gsi + rec.gamma[1] * gsr;
+        // singular rows: x is a vertex of e (atomicFuncs.cu:75-76): take the FP64 rule
+        const int sv = (row == rec.nodes[0]) ? 0 : (row == rec.nodes[1]) ? 1 : (row == rec.nodes[2]) ? 2 : -1;
+        if (sv >= 0) {
+            const R *s = a.sing + ((size_t)e * 3 + sv) * 16;
+            p0r = s[0]; p0i = s[1]; p1r = s[2]; p1i = s[3]; p2r = s[4]; p2i = s[5];
+            cval = s[6];
+            btr = s[7];
+            bti = s[8];
+        }
+        // scatter into this wave's own rows of A (first touch writes)
+        const bool writer = active && (QG == 1 || qg == 0);
+        if (writer) {
+            const R pr[3] = {p0r, p1r, p2r}, pi[3] = {p0i, p1i, p2i};
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                if ((ent >> j) & 1) {
+                    C2 *dst = a.A + (size_t)rec.nodes[j] * a.ldA + row;
+                    C2 v;
+                    if ((ent >> (3 + j)) & 1) {
+                        v.x = pr[j];
+                        v.y = pi[j];
+                    } else {
+                        v = *dst;
+                        v.x += pr[j];
+                        v.y += pi[j];
+                    }
+                    *dst = v;
+                }
+            }
+        }
+        if ((ent >> 6) & 1) {
+            c_acc += cval;
+            br_acc += btr;
+            bi_acc += bti;
+        }
+    }
+    if (active && (QG == 1 || qg == 0)) {
+        a.cpart[(size_t)tile * a.Mp + row] = c_acc;
+        C2 b;
+        b.x = br_acc;
+        b.y = bi_acc;
+        a.bpart[(size_t)tile * a.Mp + row] = b;
+    }
+}
+
+// Columns of nodes no element references: only the diagonal survives.
+template <typename C2>
+__global__ void zero_columns(C2 *A, long ldA, int M, const int *__restrict__ cols, int ncols) {
+    const int c = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols || i >= M) return;
+    C2 z;
+    z.x = 0;
+    z.y = 0;
+    A[(size_t)cols[c] * ldA + i] = z;
+}
+
+// Per row: fixed-order sum of the tile partials in FP64.
+template <typename R, typename C2>
+__global__ void asm_finalize(C2 *A, long ldA, int M, int N, int P, long Mp, const R *__restrict__ cpart,
+                             const C2 *__restrict__ bpart, double2 *__restrict__ bterm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    double cs = 0, br = 0, bi = 0;
+    for (int t = 0; t < P; t++) {
+        cs += (double)cpart[(size_t)t * Mp + i];
+        const C2 b = bpart[(size_t)t * Mp + i];
+        br += (double)b.x;
+        bi += (double)b.y;
+    }
+    if (i < N) {
+        C2 *d = A + (size_t)i * ldA + i;
+        C2 v = *d;
+        v.x = (R)((double)v.x + (1.0 - cs));
+        *d = v;
+    }
+    bterm[i] = make_double2(br, bi);
+}
+
+// B[:, s] = inc_s(x) - bterm: point source q G(x, src) (atomicFuncs.cu:238-242)
+// or plane wave amp exp(-i k d.x); computed in FP64.
+template <typename C2>
+__global__ void rhs_fill(const double *__restrict__ pts64, int M, double k, int kind, double amp,
+                         const float *__restrict__ params, int nrhs, const double2 *__restrict__ bterm,
+                         C2 *B, long ldB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (i >= M || s >= nrhs) return;
+    const double x = pts64[3 * i], y = pts64[3 * i + 1], z = pts64[3 * i + 2];
+    const double px = params[3 * s], py = params[3 * s + 1], pz = params[3 * s + 2];
+    double ir, ii;
+    if (kind == BEM_RHS_PLANE) {
+        double sn, cs;
+        sincos(k * (px * x + py * y + pz * z), &sn, &cs);
+        ir = amp * cs;
+        ii = -amp * sn;
+    } else {
+        const double dx = x - px, dy = y - py, dz = z - pz;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        double sn, cs;
+        sincos(k * r, &sn, &cs);
+        const double g = amp / (4.0 * kPi * r);
+        ir = g * cs;
+        ii = -g * sn;
+        if (amp == 0.0) ir = ii = 0.0;
+    }
+    const double2 bt = bterm[i];
+    C2 v;
+    v.x = ir - bt.x;
+    v.y = ii - bt.y;
+    B[(size_t)s * ldB + i] = v;
+}
+
+// ------------------------------------------------------------- field ---
+// Exterior pressure (genExtPressure / pntElemOffset, mesh.cu:1008-1073) at P
+// points for R nodal solutions.  Lane = field point; the wave walks a chunk of
+// elements (element data uniform, read with scalar loads) with the regular
+// 6x6 rule and accumulates sum_j (h_j - beta g_j) X[node_j] + gamma sum_j g_j;
+// chunk partials are summed in a fixed order by field_finalize.
+constexpr int kFieldChunk = 64;
+constexpr int kFieldR = 16;
+
+struct FieldArgs {
+    const float4 *__restrict__ xs;
+    const ElemRec<float> *__restrict__ recs;
+    const float2 *__restrict__ X;  // N x R, ldx
+    long ldx;
+    float2 *part;                  // [chunk][r][P]
+    int P, E, R, r0;
+    float k, kf, rw;
+    QTab<float> q;
+};
+
+__global__ __launch_bounds__(64) void field_part(FieldArgs a) {
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const bool active = p < a.P;
+    const float4 xp = a.xs[active ? p : 0];
+    const int e0 = chunk * kFieldChunk, e1 = min(a.E, e0 + kFieldChunk);
+    const int nr = min(kFieldR, a.R - a.r0);
+    float accr[kFieldR], acci[kFieldR];
+#pragma unroll
+    for (int r = 0; r < kFieldR; r++) {
+        accr[r] = 0;
+        acci[r] = 0;
+    }
+    const float inv4pi = (float)(1.0 / (4.0 * kPi));
+    for (int e = e0; e < e1; e++) {
+        const ElemRec<float> &rec = a.recs[e];
+        const float ax = rec.p3[0] - xp.x, ay = rec.p3[1] - xp.y, az = rec.p3[2] - xp.z;
+        const float dn = rec.n[0] * ax + rec.n[1] * ay + rec.n[2] * az;
+        float h1r = 0, h1i = 0, h2r = 0, h2i = 0, htr = 0, hti = 0, gr = 0, gi = 0;
+        float g1r = 0, g1i = 0, g2r = 0, g2i = 0;
+#pragma unroll 6
+        for (int i = 0; i < kNQ; i++) {
+            const float xi1 = a.q.xi1[i], xi2 = a.q.xi2[i], w = a.q.w[i];
+            const float dx = fmaf(xi1, rec.e1[0], fmaf(xi2, rec.e2[0], ax));
+            const float dy = fmaf(xi1, rec.e1[1], fmaf(xi2, rec.e2[1], ay));
+            const float dz = fmaf(xi1, rec.e1[2], fmaf(xi2, rec.e2[2], az));
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float rinv = rsq(r2);
+            const float r = r2 * rinv;
+            float sn, cs;
+            sincos_rev(r * a.kf, sn, cs);
+            const float Gr = cs * rinv, Gi = -sn * rinv;
+            const float w2 = w * rinv;
+            const float Hr = fmaf(Gr, rinv, -a.k * Gi) * w2;
+            const float Hi = fmaf(Gi, rinv, a.k * Gr) * w2;
+            htr += Hr; hti += Hi;
+            h1r = fmaf(xi1, Hr, h1r); h1i = fmaf(xi1, Hi, h1i);
+            h2r = fmaf(xi2, Hr, h2r); h2i = fmaf(xi2, Hi, h2i);
+            const float wGr = w * Gr, wGi = w * Gi;
+            gr += wGr; gi += wGi;
+            g1r = fmaf(xi1, wGr, g1r); g1i = fmaf(xi1, wGi, g1i);
+            g2r = fmaf(xi2, wGr, g2r); g2i = fmaf(xi2, wGi, g2i);
+        }
+        const float sJ = rec.J * inv4pi, sh = -dn * sJ, sg = a.rw * sJ;
+        float pr[3] = {sh * h1r, sh * h2r, sh * (htr - h1r - h2r)};
+        float pi[3] = {sh * h1i, sh * h2i, sh * (hti - h1i - h2i)};
+        const float gsr = -sg * gi, gsi = sg * gr;
+        const float G0r = -sg * g1i, G0i = sg * g1r, G1r = -sg * g2i, G1i = sg * g2r;
+        const float Gjr[3] = {G0r, G1r, gsr - G0r - G1r}, Gji[3] = {G0i, G1i, gsi - G0i - G1i};
+        const float br = rec.beta[0], bi = rec.beta[1];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            pr[j] -= br * Gjr[j] - bi * Gji[j];
+            pi[j] -= br * Gji[j] + bi * Gjr[j];
+        }
+        const float btr = rec.gamma[0] * gsr - rec.gamma[1] * gsi;
+        const float bti = rec.gamma[0] * gsi + rec.gamma[1] * gsr;
+#pragma unroll
+        for (int r = 0; r < kFieldR; r++) {
+            if (r < nr) {
+                const float2 *xc = a.X + (size_t)(a.r0 + r) * a.ldx;
+                float sr = btr, si = bti;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const float2 xv = xc[rec.nodes[j]];
+                    sr += pr[j] * xv.x - pi[j] * xv.y;
+                    si += pr[j] * xv.y + pi[j] * xv.x;
+                }
+                accr[r] += sr;
+                acci[r] += si;
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < kFieldR; r++)
+            if (r < nr) a.part[((size_t)chunk * a.R + a.r0 + r) * a.P + p] = make_float2(accr[r], acci[r]);
+    }
+}
+
+// out[p, r] = inc_r(x_p) - sum_chunks part
+__global__ void field_finalize(const float4 *__restrict__ xs, int P, int R, int nchunks, const float2 *__restrict__ part,
+                               double k, int kind, double amp, const float *__restrict__ params, float2 *out,
+                               long ldo) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (p >= P || r >= R) return;
+    double sr = 0, si = 0;
+    for (int c = 0; c < nchunks; c++) {
+        const float2 v = part[((size_t)c * R + r) * P + p];
+        sr += v.x;
+        si += v.y;
+    }
+    const double x = xs[p].x, y = xs[p].y, z = xs[p].z;
+    double ir = 0, ii = 0;
+    if (params) {
+        const double px = params[3 * r], py = params[3 * r + 1], pz = params[3 * r + 2];
+        double sn, cs;
+        if (kind == BEM_RHS_PLANE) {
+            sincos(k * (px * x + py * y + pz * z), &sn, &cs);
+            ir = amp * cs;
+            ii = -amp * sn;
+        } else if (amp != 0.0) {
+            const double dx = x - px, dy = y - py, dz = z - pz;
+            const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            sincos(k * rr, &sn, &cs);
+            ir = amp * cs / (4.0 * kPi * rr);
+            ii = -amp * sn / (4.0 * kPi * rr);
+        }
+    }
+    out[(size_t)r * ldo + p] = make_float2((float)(ir - sr), (float)(ii - si));
+}
+
+int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2 *X, long ldx, int R,
+               const RhsSpec &rhs, float2 *out, long ldo, hipStream_t s) {
+    if (P <= 0 || R <= 0) return EXIT_SUCCESS;
+    const Config &cfg = config();
+    QuadTables qt;
+    quad_tables(qt);
+    const int nch = (dp.E + kFieldChunk - 1) / kFieldChunk;
+    float2 *part = nullptr;
+    float *params = nullptr;
+    HOST_CALL(dalloc(&part, (size_t)nch * R * P));
+    if (!rhs.params.empty()) {
+        HOST_CALL(dalloc(&params, rhs.params.size()));
+        HIP_CALL(hipMemcpyAsync(params, rhs.params.data(), rhs.params.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    FieldArgs a;
+    a.xs = xs;
+    a.recs = dp.rec32;
+    a.X = X;
+    a.ldx = ldx;
+    a.part = part;
+    a.P = P;
+    a.E = dp.E;
+    a.R = R;
+    a.k = k;
+    a.kf = (float)((double)k / (2.0 * kPi));
+    a.rw = (float)((double)cfg.density * (double)(k * cfg.speed));
+    for (int q = 0; q < kNQ; q++) {
+        a.q.xi1[q] = qt.xi1[q];
+        a.q.xi2[q] = qt.xi2[q];
+        a.q.w[q] = qt.w[q];
+    }
+    TimedRegion tr;
+    timing_begin(tr, "field_part", s);
+    for (int r0 = 0; r0 < R; r0 += kFieldR) {
+        a.r0 = r0;
+        dim3 grid((P + 63) / 64, nch);
+        field_part<<<grid, 64, 0, s>>>(a);
+        HIP_CALL(hipGetLastError());
+    }
+    timing_end(tr, s);
+    dim3 g2((P + 255) / 256, R);
+    field_finalize<<<g2, 256, 0, s>>>(xs, P, R, nch, part, (double)k, rhs.kind, (double)rhs.amp,
+                                      rhs.nrhs >= R ? params : nullptr, out, ldo);
+    hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(s);
+    dfree(part);
+    dfree(params);
+    HIP_CALL(e);
+    return EXIT_SUCCESS;
+}
+
+// --------------------------------------------------------------- host ---
+static int choose_tiles(int M, int N, int rows_per_wave) {
+    const char *env = getenv("BEM_TILES");
+    if (env && atoi(env) > 0) return std::min(atoi(env), N);
+    const int nrb = (M + rows_per_wave - 1) / rows_per_wave;
+    const int target_waves = 4096;
+    int P = (target_waves + nrb - 1) / nrb;
+    P = std::max(1, std::min(P, std::max(1, N / 8)));
+    return P;
+}
+
+static int choose_qg(int M, int E) {
+    const char *env = getenv("BEM_QG");
+    if (env) {
+        int q = atoi(env);
+        if (q == 1 || q == 4) return q;
+    }
+    return ((long)M * E < 4000000L) ? 4 : 1;
+}
+
+int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
+    HOST_CALL(validate_mesh(m));
+    HOST_CALL(ensure_quad());
+    device_problem_free(dp);
+    dp.N = m.N();
+    dp.C = m.C();
+    dp.M = dp.N + dp.C;
+    dp.E = m.E();
+    dp.ldA = ((long)dp.M + 63) / 64 * 64;
+    dp.Mp = dp.ldA;
+    std::vector<float4> p4(dp.M);
+    std::vector<double> p64(3 * (size_t)dp.M);
+    for (int i = 0; i < dp.M; i++) {
+        const float *src = i < dp.N ? &m.pnts[3 * i] : &m.chief[3 * (i - dp.N)];
+        p4[i] = make_float4(src[0], src[1], src[2], 0.f);
+        for (int d = 0; d < 3; d++) p64[3 * i + d] = src[d];
+    }
+    std::vector<int> el(3 * (size_t)dp.E);
+    std::vector<float2> bc(3 * (size_t)dp.E);
+    for (int e = 0; e < dp.E; e++)
+        for (int j = 0; j < 3; j++) {
+            el[3 * e + j] = m.elems[e].nodes[j];
+            bc[3 * e + j] = make_float2(m.elems[e].bc[j].x, m.elems[e].bc[j].y);
+        }
+    TilePlan plan;
+    const int qg = choose_qg(dp.M, dp.E);
+    build_tile_plan(m, choose_tiles(dp.M, dp.N, 64 / qg), plan);
+    dp.P = plan.P;
+    HOST_CALL(dalloc(&dp.pts, dp.M));
+    HOST_CALL(dalloc(&dp.pts64, 3 * (size_t)dp.M));
+    HOST_CALL(dalloc(&dp.elems, 3 * (size_t)dp.E));
+    HOST_CALL(dalloc(&dp.bc, 3 * (size_t)dp.E));
+    HOST_CALL(dalloc(&dp.rec32, dp.E));
+    HOST_CALL(dalloc(&dp.rec64, dp.E));
+    HOST_CALL(dalloc(&dp.sing32, (size_t)dp.E * 48));
+    HOST_CALL(dalloc(&dp.sing64, (size_t)dp.E * 48));
+    HOST_CALL(dalloc(&dp.tile_ptr, plan.ptr.size()));
+    HOST_CALL(dalloc(&dp.tile_entries, plan.entries.size()));
+    dp.n_isolated = (int)plan.isolated.size();
+    HOST_CALL(dalloc(&dp.isolated, std::max<size_t>(1, plan.isolated.size())));
+    HOST_CALL(dalloc(&dp.cpart, (size_t)dp.P * dp.Mp));
+    HOST_CALL(dalloc(&dp.bpart, (size_t)dp.P * dp.Mp));
+    HOST_CALL(dalloc(&dp.cpart64, (size_t)dp.P * dp.Mp));
+    HOST_CALL(dalloc(&dp.bpart64, (size_t)dp.P * dp.Mp));
+    HOST_CALL(dalloc(&dp.bterm, (size_t)dp.M));
+    HIP_CALL(hipMemcpyAsync(dp.pts, p4.data(), p4.size() * sizeof(float4), hipMemcpyHostToDevice, s));
+    HIP_CALL(hipMemcpyAsync(dp.pts64, p64.data(), p64.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    HIP_CALL(hipMemcpyAsync(dp.elems, el.data(), el.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_CALL(hipMemcpyAsync(dp.bc, bc.data(), bc.size() * sizeof(float2), hipMemcpyHostToDevice, s));
+    HIP_CALL(hipMemcpyAsync(dp.tile_ptr, plan.ptr.data(), plan.ptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    HIP_CALL(hipMemcpyAsync(dp.tile_entries, plan.entries.data(), plan.entries.size() * sizeof(int),
+                            hipMemcpyHostToDevice, s));
+    if (!plan.isolated.empty())
+        HIP_CALL(hipMemcpyAsync(dp.isolated, plan.isolated.data(), plan.isolated.size() * sizeof(int),
+                                hipMemcpyHostToDevice, s));
+    dp.qg = qg;
+    dp.impedance = 0;
+    for (const auto &e : m.elems)
+        if (e.bc[0].x != 0.f || e.bc[0].y != 0.f) dp.impedance = 1;
+    const int bs = 256;
+    prep_elems<float><<<(dp.E + bs - 1) / bs, bs, 0, s>>>(dp.pts64, dp.elems, dp.bc, dp.E, dp.rec32);
+    HIP_CALL(hipGetLastError());
+    prep_elems<double><<<(dp.E + bs - 1) / bs, bs, 0, s>>>(dp.pts64, dp.elems, dp.bc, dp.E, dp.rec64);
+    HIP_CALL(hipGetLastError());
+    HIP_CALL(hipStreamSynchronize(s));  // host vectors above go out of scope
+    dp.mesh_version = m.version;
+    return EXIT_SUCCESS;
+}
+
+void device_problem_free(DeviceProblem &dp) {
+    dfree(dp.pts);
+    dfree(dp.pts64);
+    dfree(dp.elems);
+    dfree(dp.bc);
+    dfree(dp.rec32);
+    dfree(dp.rec64);
+    dfree(dp.sing32);
+    dfree(dp.sing64);
+    dfree(dp.tile_ptr);
+    dfree(dp.tile_entries);
+    dfree(dp.isolated);
+    dfree(dp.cpart);
+    dfree(dp.bpart);
+    dfree(dp.cpart64);
+    dfree(dp.bpart64);
+    dfree(dp.bterm);
+    dp.N = dp.C = dp.M = dp.E = dp.P = 0;
+}
+
+static void sing_tab(SingTab &t) {
+    QuadTables q;
+    quad_tables(q);
+    const Config &c = config();
+    for (int n = 0; n < kQOrder; n++) {
+        const double theta = 0.5 + 0.5 * c.qp[n];
+        for (int m = 0; m < kQOrder; m++) {
+            const double rho = 0.5 + 0.5 * c.qp[m];
+            const int qi = n * kQOrder + m;
+            // vertex 1: (1-rho, rho(1-theta)); vertex 2: (rho theta, 1-rho);
+            // vertex 3: (rho(1-theta), rho theta) (mesh.cu:272-273, 312-313, 352-353)
+            t.xi1[0][qi] = 1 - rho;
+            t.xi2[0][qi] = rho * (1 - theta);
+            t.xi1[1][qi] = rho * theta;
+            t.xi2[1][qi] = 1 - rho;
+            t.xi1[2][qi] = rho * (1 - theta);
+            t.xi2[2][qi] = rho * theta;
+            t.w[qi] = q.dw[qi];
+        }
+    }
+}
+
+template <typename R, typename C2>
+static int assemble_impl(DeviceProblem &dp, R k, C2 *A, hipStream_t s) {
+    HOST_CALL(ensure_quad());
+    if (dp.M == 0) BEM_FAIL("device problem not built");
+    const Config &cfg = config();
+    const double kd = (double)k;
+    const double rw = (double)cfg.density * (double)(k * (R)cfg.speed);  // rho * omega, omega = k c
+    QuadTables qt;
+    quad_tables(qt);
+    SingTab st;
+    sing_tab(st);
+    R *sing = sizeof(R) == 4 ? (R *)dp.sing32 : (R *)dp.sing64;
+    {
+        TimedRegion tr;
+        timing_begin(tr, "sing_elems", s);
+        const int n = 3 * dp.E, bs = 128;
+        sing_elems<R><<<(n + bs - 1) / bs, bs, 0, s>>>(dp.pts64, dp.elems, dp.bc, dp.E, kd, rw, st, sing);
+        HIP_CALL(hipGetLastError());
+        timing_end(tr, s);
+    }
+    AsmArgs<R> a;
+    a.pts = dp.pts;
+    a.recs = (const ElemRec<R> *)(sizeof(R) == 4 ? (void *)dp.rec32 : (void *)dp.rec64);
+    a.sing = sing;
+    a.tile_ptr = dp.tile_ptr;
+    a.tile_entries = dp.tile_entries;
+    a.A = A;
+    a.cpart = sizeof(R) == 4 ? (R *)dp.cpart : (R *)dp.cpart64;
+    a.bpart = sizeof(R) == 4 ? (C2 *)dp.bpart : (C2 *)dp.bpart64;
+    a.ldA = dp.ldA;
+    a.Mp = dp.Mp;
+    a.M = dp.M;
+    a.N = dp.N;
+    const int rpw = 64 / dp.qg;
+    const int nrb = (dp.M + rpw - 1) / rpw;
+    a.nrbg = (nrb + 3) / 4;
+    a.nwg = a.nrbg * dp.P;
+    a.k = k;
+    a.kf = (R)(kd / (2.0 * kPi));
+    a.rw = (R)rw;
+    a.impedance = dp.impedance;
+    for (int q = 0; q < kNQ; q++) {
+        a.q.xi1[q] = sizeof(R) == 4 ? (R)qt.xi1[q] : (R)qt.dxi1[q];
+        a.q.xi2[q] = sizeof(R) == 4 ? (R)qt.xi2[q] : (R)qt.dxi2[q];
+        a.q.w[q] = sizeof(R) == 4 ? (R)qt.w[q] : (R)qt.dw[q];
+    }
+    {
+        TimedRegion tr;
+        timing_begin(tr, sizeof(R) == 4 ? "asm_tile_f32" : "asm_tile_f64", s);
+        dim3 grid(a.nwg), block(256);
+        if (dp.qg == 4) {
+            if (dp.impedance) asm_tile<R, 4, true><<<grid, block, 0, s>>>(a);
+            else asm_tile<R, 4, false><<<grid, block, 0, s>>>(a);
+        } else {
+            if (dp.impedance) asm_tile<R, 1, true><<<grid, block, 0, s>>>(a);
+            else asm_tile<R, 1, false><<<grid, block, 0, s>>>(a);
+        }
+        HIP_CALL(hipGetLastError());
+        timing_end(tr, s);
+    }
+    if (dp.n_isolated > 0) {
+        dim3 grid((dp.M + 255) / 256, dp.n_isolated);
+        zero_columns<C2><<<grid, 256, 0, s>>>(A, dp.ldA, dp.M, dp.isolated, dp.n_isolated);
+        HIP_CALL(hipGetLastError());
+    }
+    {
+        TimedRegion tr;
+        timing_begin(tr, "asm_finalize", s);
+        asm_finalize<R, C2><<<(dp.M + 255) / 256, 256, 0, s>>>(A, dp.ldA, dp.M, dp.N, dp.P, dp.Mp, a.cpart, a.bpart,
+                                                            dp.bterm);
+        HIP_CALL(hipGetLastError());
+        timing_end(tr, s);
+    }
+    return EXIT_SUCCESS;
+}
+
+int assemble_fp32(DeviceProblem &dp, float k, float2 *A, hipStream_t s) { return assemble_impl<float, float2>(dp, k, A, s); }
+int assemble_fp64(DeviceProblem &dp, double k, double2 *A, hipStream_t s) {
+    return assemble_impl<double, double2>(dp, k, A, s);
+}
+
+template <typename C2>
+static int rhs_impl(DeviceProblem &dp, double k, C2 *B, long ldB, const RhsSpec &rhs, hipStream_t s) {
+    if (rhs.nrhs <= 0) return EXIT_SUCCESS;
+    if (rhs.nrhs > kMaxRhs) BEM_FAIL("at most %d right-hand sides, got %d", kMaxRhs, rhs.nrhs);
+    float *params = nullptr;
+    HOST_CALL(dalloc(&params, 3 * (size_t)rhs.nrhs));
+    HIP_CALL(hipMemcpyAsync(params, rhs.params.data(), 3 * sizeof(float) * rhs.nrhs, hipMemcpyHostToDevice, s));
+    dim3 grid((dp.M + 255) / 256, rhs.nrhs);
+    rhs_fill<C2><<<grid, 256, 0, s>>>(dp.pts64, dp.M, k, rhs.kind, (double)rhs.amp, params, rhs.nrhs, dp.bterm, B, ldB);
+    hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(s);
+    dfree(params);
+    HIP_CALL(e);
+    return EXIT_SUCCESS;
+}
+
+int rhs_fp32(DeviceProblem &dp, double k, float2 *B, long ldB, const RhsSpec &rhs, hipStream_t s) {
+    return rhs_impl<float2>(dp, k, B, ldB, rhs, s);
+}
+int rhs_fp64(DeviceProblem &dp, double k, double2 *B, long ldB, const RhsSpec &rhs, hipStream_t s) {
+    return rhs_impl<double2>(dp, k, B, ldB, rhs, s);
+}
+
+}  // namespace bem

@@ -1,0 +1,229 @@
+// Internal declarations shared by the host and HIP sources of libbem.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "../../include/bem.h"
+
+namespace bem {
+
+// ---------------------------------------------------------------- errors ---
+void set_error(const char *file, int line, const char *fmt, ...);
+
+#define BEM_FAIL(...)                                       \
+    do {                                                    \
+        ::bem::set_error(__FILE__, __LINE__, __VA_ARGS__);  \
+        return EXIT_FAILURE;                                \
+    } while (0)
+
+#define HIP_CALL(x)                                                                        \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            ::bem::set_error(__FILE__, __LINE__, "HIP error '%s' in %s", hipGetErrorString(e_), #x); \
+            return EXIT_FAILURE;                                                           \
+        }                                                                                  \
+    } while (0)
+
+#define HOST_CALL(x)                                                     \
+    do {                                                                 \
+        if ((x) != EXIT_SUCCESS) {                                       \
+            ::bem::set_error(__FILE__, __LINE__, "call failed: %s", #x); \
+            return EXIT_FAILURE;                                         \
+        }                                                                \
+    } while (0)
+
+// ------------------------------------------------------------ constants ---
+constexpr int kQOrder = 6;            // INTORDER (numerical.h:40-42)
+constexpr int kNQ = kQOrder * kQOrder; // 36 points of the collapsed rule
+constexpr int kMaxRhs = 64;
+constexpr double kPi = 3.1415926535897932;
+
+struct Config {
+    int qorder = 0;
+    double qp[32] = {0};
+    double qw[32] = {0};
+    float density = 1.2041f;  // numerical.cu:12
+    float speed = 343.21f;    // numerical.cu:14
+    float strength = 0.0f;    // STRENGTH, numerical.h:128-130
+};
+Config &config();
+int ensure_quad();  // default order-6 tables if bem_quad_init was not called
+
+// Per-point tables of the 6x6 collapsed rule (element independent):
+// xi1, xi2 of the regular map and W = 0.25 w_n w_m rho_m (mesh.cu:623-636).
+struct QuadTables {
+    float xi1[kNQ], xi2[kNQ], w[kNQ];
+    double dxi1[kNQ], dxi2[kNQ], dw[kNQ];
+};
+void quad_tables(QuadTables &t);
+
+// ------------------------------------------------------------- context ---
+struct Context {
+    hipStream_t stream = nullptr;
+    int device = -1;
+};
+int ctx_init(Context **out);
+
+// Kernel timing with HIP events on the library stream.
+bool timing_on();
+struct TimedRegion {
+    hipEvent_t a = nullptr, b = nullptr;
+    const char *name = nullptr;
+};
+void timing_begin(TimedRegion &r, const char *name, hipStream_t s);
+void timing_end(TimedRegion &r, hipStream_t s);
+
+// ---------------------------------------------------------------- mesh ---
+struct Mesh {
+    std::vector<float> pnts;   // 3 * N
+    std::vector<float> chief;  // 3 * C
+    std::vector<bem_tri_elem> elems;
+    float bb[6] = {0, 0, 0, 0, 0, 0};
+    bool bb_set = false;
+    uint64_t version = 0;  // bumped on every change
+    int N() const { return (int)(pnts.size() / 3); }
+    int C() const { return (int)(chief.size() / 3); }
+    int E() const { return (int)elems.size(); }
+};
+int validate_mesh(const Mesh &m);
+
+// Column tiles for row-owned assembly: a partition of the nodes into P
+// spatially compact parts (recursive coordinate bisection); for each part, the
+// elements with at least one vertex in it, with per-entry flags:
+//   bits 0-2  vertex j of the element lies in this part
+//   bits 3-5  this entry is the first, in list order, to touch vertex j's column
+//   bit  6    this part owns the element (counts its c and B terms once)
+struct TilePlan {
+    int P = 0;
+    std::vector<int> ptr;      // P + 1
+    std::vector<int> entries;  // element index << 8 | flags
+    std::vector<int> isolated; // nodes referenced by no element
+};
+void build_tile_plan(const Mesh &m, int P, TilePlan &plan);
+
+// Element record read by the assembly kernels with scalar loads (128 B):
+//   p3, e1 = p1-p3, e2 = p2-p3, n (unit normal of e1 x e2), J = |e1 x e2|,
+//   beta = ca/cb, gamma = cc/cb, nodes.
+template <typename R>
+struct alignas(16) ElemRec {
+    R p3[3];
+    R e1[3];
+    R e2[3];
+    R n[3];
+    R J;
+    R beta[2];
+    R gamma[2];
+    int nodes[3];
+    int pad[(sizeof(R) == 4) ? 12 : 1];
+};
+static_assert(sizeof(ElemRec<float>) == 128, "record size");
+
+// ------------------------------------------------------------ assembly ---
+// Device-resident problem: mesh on the device and assembly scratch.
+struct DeviceProblem {
+    int N = 0, C = 0, M = 0, E = 0;
+    long ldA = 0;                 // leading dimension of the device A (M rounded up to 64)
+    float4 *pts = nullptr;        // M points (nodes then CHIEF), FP32
+    double *pts64 = nullptr;      // 3 * M, FP64 copy
+    int *elems = nullptr;         // 3 * E node indices
+    float2 *bc = nullptr;         // 3 * E (ca, cb, cc)
+    ElemRec<float> *rec32 = nullptr;
+    ElemRec<double> *rec64 = nullptr;
+    float *sing32 = nullptr;      // E * 3 * 16: singular contributions
+    double *sing64 = nullptr;
+    int P = 0;                    // column tiles
+    int qg = 1;                   // lanes per row in the quadrature kernel
+    int impedance = 0;            // some element has ca != 0
+    int *tile_ptr = nullptr;
+    int *tile_entries = nullptr;
+    int n_isolated = 0;
+    int *isolated = nullptr;
+    float *cpart = nullptr;       // P * Mp  c partial sums
+    float2 *bpart = nullptr;      // P * Mp  boundary-condition partial sums
+    double *cpart64 = nullptr;
+    double2 *bpart64 = nullptr;
+    double2 *bterm = nullptr;     // M: sum_e (cc/cb) sum_j g_j(x_i)
+    long Mp = 0;
+    uint64_t mesh_version = 0;
+};
+int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s);
+void device_problem_free(DeviceProblem &dp);
+
+struct RhsSpec {
+    int kind = BEM_RHS_POINT;
+    int nrhs = 0;
+    float amp = 0.f;
+    std::vector<float> params;  // 3 * nrhs
+};
+
+// Assemble A (ld dp.ldA, M x N) and dp.bterm on the device.
+int assemble_fp32(DeviceProblem &dp, float k, float2 *A, hipStream_t s);
+int assemble_fp64(DeviceProblem &dp, double k, double2 *A, hipStream_t s);
+// B[:, s] = incident_s - bterm (M x nrhs, ldB)
+int rhs_fp32(DeviceProblem &dp, double k, float2 *B, long ldB, const RhsSpec &rhs, hipStream_t s);
+int rhs_fp64(DeviceProblem &dp, double k, double2 *B, long ldB, const RhsSpec &rhs, hipStream_t s);
+// Exterior field: out[P x R] = inc - sum_e (...) for nodal solutions X (N x R).
+int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2 *X, long ldx, int R,
+               const RhsSpec &rhs, float2 *out, long ldo, hipStream_t s);
+
+// ---------------------------------------------------------- linear algebra ---
+size_t ws_gemv_n(long M, long N, int R);
+int gemv_n_f32(const float2 *A, long lda, long M, long N, const float2 *x, long ldx, float2 *y, long ldy, int R,
+               void *ws, hipStream_t s);
+int gemv_h_f32(const float2 *A, long lda, long M, long N, const float2 *x, long ldx, float2 *y, long ldy, int R,
+               hipStream_t s);
+int resid_n_f32a(const float2 *A, long lda, long M, long N, const double2 *x, long ldx, const double2 *b, long ldb,
+                 double2 *r, long ldr, int R, void *ws, hipStream_t s);
+int gemv_h_f32a(const float2 *A, long lda, long M, long N, const double2 *x, long ldx, double2 *y, long ldy, int R,
+                hipStream_t s);
+int gemv_n_f64(const double2 *A, long lda, long M, long N, const double2 *x, long ldx, double2 *y, long ldy, int R,
+               void *ws, hipStream_t s);
+int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, long ldx, double2 *y, long ldy, int R,
+               hipStream_t s);
+
+int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
+                 double2 *X, long ldx, int R, int restart, double tol, int maxit, double outer_tol, int max_cycles,
+                 bem_solve_stats *st, hipStream_t s);
+int qr_lsq_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, hipStream_t s);
+int qr_lsq_f64(double2 *A, long lda, int M, int N, double2 *B, long ldb, int R, hipStream_t s);
+int qr_lsq_with_q_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, float2 *Q, long ldq,
+                      hipStream_t s);
+int qr_refined_f32(const float2 *A32, long lda, int M, int N, const double2 *B64, long ldb, double2 *X64, long ldx,
+                   int R, double refine_tol, int max_refine, bem_solve_stats *st, hipStream_t s);
+
+// host-side mesh operations
+int read_obj(Mesh &m, const char *file);
+int find_bb(Mesh &m, float threshold);
+int gen_chief(Mesh &m, int num, float threshold, uint64_t seed);
+int gen_icosphere(Mesh &m, int s, float radius);
+int gen_ellipsoid(Mesh &m, int s, float ax, float ay, float az, int nb, int lmax, float amp, uint64_t seed);
+double signed_volume(const Mesh &m);
+double surf_area(const Mesh &m);
+int gauss_legendre(int n, double *x, double *w);
+int quad_init(int order);
+int timing_enable(int on);
+int timing_reset();
+int timing_get(const char *name, double *total_ms, long *count);
+const char *last_error();
+void clear_error();
+
+// Memory helpers
+template <typename T>
+int dalloc(T **p, size_t n) {
+    if (n == 0) n = 1;
+    HIP_CALL(hipMalloc((void **)p, n * sizeof(T)));
+    return EXIT_SUCCESS;
+}
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+}  // namespace bem

@@ -1,0 +1,535 @@
+// Blocked Householder least squares on the device: the job of qrSolver
+// (numerical.cu:580-644: geqrf, unmqr with Q^H, trsm) and of the reference's
+// own blocked WY QR (QR_thin, numerical.cu:246-504), without host round trips
+// inside a panel.
+//
+//   qr_panel    one workgroup factors a panel of nb columns in place: for each
+//               column a Householder reflector H = I - tau v v^H (v_0 = 1,
+//               tau real, beta = -phase(alpha) ||x||), applied to the rest of
+//               the panel; then the triangular factor T of the compact WY form
+//               H_1 ... H_nb = I - V T V^H (Schreiber & Van Loan 1989).
+//   qr_vhc      W = T^H (V^H C) for the trailing columns C (one workgroup per
+//               column, the nb dot products in one pass over the column);
+//   qr_update   C -= V W (the rank-nb trailing update, GEMM-shaped);
+//   trsv_upper  R x = (Q^H b)[0:N] by column-oriented back substitution.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bem_internal.hpp"
+
+namespace bem {
+
+template <typename T>
+struct QT;
+template <>
+struct QT<float> {
+    using c2 = float2;
+};
+template <>
+struct QT<double> {
+    using c2 = double2;
+};
+
+constexpr int kNB = 32;  // panel width
+
+template <typename T>
+__device__ __forceinline__ T wsum(T v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// V[i, l] of panel starting at column j0: unit at row j0+l, stored below it.
+template <typename C2>
+__device__ __forceinline__ C2 vget(const C2 *A, long lda, int j0, int l, int i) {
+    C2 z;
+    if (i == j0 + l) {
+        z.x = 1;
+        z.y = 0;
+    } else if (i > j0 + l) {
+        z = A[(size_t)(j0 + l) * lda + i];
+    } else {
+        z.x = 0;
+        z.y = 0;
+    }
+    return z;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void qr_panel(typename QT<T>::c2 *A, long lda, int M, int j0, int nb,
+                                                 typename QT<T>::c2 *Tout /* nb x nb, ld kNB */) {
+    using C2 = typename QT<T>::c2;
+    __shared__ T red[2][16];
+    __shared__ T s_tau[kNB];
+    __shared__ C2 s_Y[kNB][kNB];  // v_l^H v_m for l < m
+    __shared__ C2 s_T[kNB][kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+    for (int jj = 0; jj < nb; jj++) {
+        const int j = j0 + jj;
+        C2 *col = A + (size_t)j * lda;
+        // s = sum_{i>j} |A[i,j]|^2
+        T s = 0;
+        for (int i = j + 1 + tid; i < M; i += blockDim.x) s += col[i].x * col[i].x + col[i].y * col[i].y;
+        s = wsum(s);
+        if (lane == 0) red[0][wid] = s;
+        __syncthreads();
+        T tot = 0;
+        for (int q = 0; q < nw; q++) tot += red[0][q];
+        const C2 alpha = col[j];
+        const T aa = sqrt(alpha.x * alpha.x + alpha.y * alpha.y);
+        const T xnorm = sqrt(aa * aa + tot);
+        T tau = 0;
+        C2 scale;
+        scale.x = 0;
+        scale.y = 0;
+        C2 beta;
+        beta = alpha;
+        if (tot > 0 || (xnorm > 0 && alpha.y != 0)) {
+            const T phr = aa > 0 ? alpha.x / aa : (T)1, phi = aa > 0 ? alpha.y / aa : (T)0;
+            beta.x = -phr * xnorm;
+            beta.y = -phi * xnorm;
+            // scale = 1 / (alpha - beta)
+            const T dr = alpha.x - beta.x, di = alpha.y - beta.y;
+            const T dd = dr * dr + di * di;
+            scale.x = dr / dd;
+            scale.y = -di / dd;
+            tau = (T)2 / ((T)1 + tot / dd);
+        }
+        __syncthreads();  // everyone read alpha / red
+        for (int i = j + 1 + tid; i < M; i += blockDim.x) {
+            const C2 v = col[i];
+            C2 o;
+            o.x = v.x * scale.x - v.y * scale.y;
+            o.y = v.x * scale.y + v.y * scale.x;
+            col[i] = o;
+        }
+        if (tid == 0) {
+            col[j] = beta;
+            s_tau[jj] = tau;
+        }
+        __syncthreads();
+        // apply H^H = H (tau real) to the remaining panel columns: one wave per column
+        for (int c = j + 1 + wid; c < j0 + nb; c += nw) {
+            C2 *cc = A + (size_t)c * lda;
+            T dr = 0, di = 0;
+            for (int i = j + 1 + lane; i < M; i += 64) {
+                const C2 v = col[i], a = cc[i];
+                dr += v.x * a.x + v.y * a.y;
+                di += v.x * a.y - v.y * a.x;
+            }
+            dr = wsum(dr);
+            di = wsum(di);
+            const C2 aj = cc[j];
+            dr = (dr + aj.x) * tau;
+            di = (di + aj.y) * tau;
+            for (int i = j + 1 + lane; i < M; i += 64) {
+                const C2 v = col[i];
+                C2 a = cc[i];
+                a.x -= dr * v.x - di * v.y;
+                a.y -= dr * v.y + di * v.x;
+                cc[i] = a;
+            }
+            if (lane == 0) {
+                C2 a = aj;
+                a.x -= dr;
+                a.y -= di;
+                cc[j] = a;
+            }
+        }
+        __syncthreads();
+    }
+    // Y[l][m] = v_l^H v_m for l < m (dot over rows >= j0+m)
+    for (int p = wid; p < nb * nb; p += nw) {
+        const int l = p / nb, m = p % nb;
+        if (l >= m) continue;
+        T dr = 0, di = 0;
+        for (int i = j0 + m + lane; i < M; i += 64) {
+            const C2 vl = vget(A, lda, j0, l, i), vm = vget(A, lda, j0, m, i);
+            dr += vl.x * vm.x + vl.y * vm.y;
+            di += vl.x * vm.y - vl.y * vm.x;
+        }
+        dr = wsum(dr);
+        di = wsum(di);
+        if (lane == 0) {
+            s_Y[l][m].x = dr;
+            s_Y[l][m].y = di;
+        }
+    }
+    __syncthreads();
+    // T upper triangular (LAPACK larft, forward columnwise): T_mm = tau_m,
+    // T[0:m, m] = -tau_m T[0:m, 0:m] Y[0:m, m]; column m in parallel over rows
+    for (int p = tid; p < kNB * kNB; p += blockDim.x) {
+        s_T[p / kNB][p % kNB].x = 0;
+        s_T[p / kNB][p % kNB].y = 0;
+    }
+    __syncthreads();
+    for (int m = 0; m < nb; m++) {
+        if (tid < m) {
+            T zr = 0, zi = 0;
+            for (int q = tid; q < m; q++) {
+                const C2 t = s_T[tid][q], y = s_Y[q][m];
+                zr += t.x * y.x - t.y * y.y;
+                zi += t.x * y.y + t.y * y.x;
+            }
+            s_T[tid][m].x = -s_tau[m] * zr;
+            s_T[tid][m].y = -s_tau[m] * zi;
+        } else if (tid == m) {
+            s_T[m][m].x = s_tau[m];
+        }
+        __syncthreads();
+    }
+    for (int p = tid; p < kNB * kNB; p += blockDim.x) Tout[p] = s_T[p / kNB][p % kNB];
+}
+
+// W[:, c] = op(T) (V^H C[:, c]); op = conj-transpose (apply Q^H) or identity (apply Q)
+template <typename T>
+__global__ __launch_bounds__(256) void qr_vhc(const typename QT<T>::c2 *A, long lda, int M, int j0, int nb,
+                                              const typename QT<T>::c2 *Tm, const typename QT<T>::c2 *C, long ldc,
+                                              typename QT<T>::c2 *W, int conjT) {
+    using C2 = typename QT<T>::c2;
+    __shared__ T red[2][4][kNB];
+    __shared__ C2 z[kNB];
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const C2 *cc = C + (size_t)c * ldc;
+    T zr[kNB], zi[kNB];
+#pragma unroll
+    for (int l = 0; l < kNB; l++) {
+        zr[l] = 0;
+        zi[l] = 0;
+    }
+    for (int i = j0 + tid; i < M; i += 256) {
+        const C2 a = cc[i];
+#pragma unroll
+        for (int l = 0; l < kNB; l++) {
+            if (l < nb) {
+                const C2 v = vget(A, lda, j0, l, i);
+                zr[l] += v.x * a.x + v.y * a.y;
+                zi[l] += v.x * a.y - v.y * a.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < kNB; l++) {
+        const T r = wsum(zr[l]), im = wsum(zi[l]);
+        if (lane == 0) {
+            red[0][wid][l] = r;
+            red[1][wid][l] = im;
+        }
+    }
+    __syncthreads();
+    if (tid < nb) {
+        C2 s;
+        s.x = red[0][0][tid] + red[0][1][tid] + red[0][2][tid] + red[0][3][tid];
+        s.y = red[1][0][tid] + red[1][1][tid] + red[1][2][tid] + red[1][3][tid];
+        z[tid] = s;
+    }
+    __syncthreads();
+    if (tid < nb) {
+        // w = T^H z (conjT) or T z
+        T wr = 0, wi = 0;
+        for (int q = 0; q < nb; q++) {
+            C2 t = conjT ? Tm[q * kNB + tid] : Tm[tid * kNB + q];
+            if (conjT) t.y = -t.y;
+            wr += t.x * z[q].x - t.y * z[q].y;
+            wi += t.x * z[q].y + t.y * z[q].x;
+        }
+        C2 o;
+        o.x = wr;
+        o.y = wi;
+        W[(size_t)c * kNB + tid] = o;
+    }
+}
+
+// C[i, c] -= sum_l V[i, l] W[l, c]
+template <typename T>
+__global__ __launch_bounds__(256) void qr_update(const typename QT<T>::c2 *A, long lda, int M, int j0, int nb,
+                                                 const typename QT<T>::c2 *W, typename QT<T>::c2 *C, long ldc) {
+    using C2 = typename QT<T>::c2;
+    __shared__ C2 w[kNB];
+    const int c = blockIdx.y;
+    const int i = j0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < nb) w[threadIdx.x] = W[(size_t)c * kNB + threadIdx.x];
+    __syncthreads();
+    if (i >= M) return;
+    T sr = 0, si = 0;
+    for (int l = 0; l < nb; l++) {
+        const C2 v = vget(A, lda, j0, l, i);
+        sr += v.x * w[l].x - v.y * w[l].y;
+        si += v.x * w[l].y + v.y * w[l].x;
+    }
+    C2 *p = C + (size_t)c * ldc + i;
+    C2 o = *p;
+    o.x -= sr;
+    o.y -= si;
+    *p = o;
+}
+
+// R x = b (upper, non-unit), one workgroup per right-hand side, in place in B.
+template <typename T>
+__global__ __launch_bounds__(1024) void trsv_upper(const typename QT<T>::c2 *Rm, long ldr, int N,
+                                                   typename QT<T>::c2 *B, long ldb) {
+    using C2 = typename QT<T>::c2;
+    __shared__ C2 xj;
+    C2 *b = B + (size_t)blockIdx.x * ldb;
+    for (int j = N - 1; j >= 0; j--) {
+        if (threadIdx.x == 0) {
+            const C2 r = Rm[(size_t)j * ldr + j], v = b[j];
+            const T d = r.x * r.x + r.y * r.y;
+            C2 x;
+            x.x = (v.x * r.x + v.y * r.y) / d;
+            x.y = (v.y * r.x - v.x * r.y) / d;
+            b[j] = x;
+            xj = x;
+        }
+        __syncthreads();
+        const C2 x = xj;
+        const C2 *rc = Rm + (size_t)j * ldr;
+        for (int i = threadIdx.x; i < j; i += blockDim.x) {
+            const C2 r = rc[i];
+            C2 o = b[i];
+            o.x -= r.x * x.x - r.y * x.y;
+            o.y -= r.x * x.y + r.y * x.x;
+            b[i] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ host ---
+template <typename T>
+struct QRFactor {
+    using C2 = typename QT<T>::c2;
+    C2 *A = nullptr;  // factored copy (owned by caller)
+    long lda = 0;
+    int M = 0, N = 0;
+    C2 *Tb = nullptr;  // per-panel T factors
+    C2 *W = nullptr;   // workspace nb x max(ncols)
+    int wcols = 0;
+    ~QRFactor() {
+        dfree(Tb);
+        dfree(W);
+    }
+};
+
+template <typename T>
+static int qr_apply(const QRFactor<T> &f, typename QT<T>::c2 *C, long ldc, int nc, int conjT, hipStream_t s) {
+    if (nc <= 0) return EXIT_SUCCESS;
+    if (nc > f.wcols) BEM_FAIL("QR workspace too small");
+    const int np = (f.N + kNB - 1) / kNB;
+    for (int pi = 0; pi < np; pi++) {
+        const int p = conjT ? pi : np - 1 - pi;  // Q^H: H_nb..H_1 order of panels 0..; Q: reverse
+        const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
+        qr_vhc<T><<<nc, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Tb + (size_t)p * kNB * kNB, C, ldc, f.W, conjT);
+        HIP_CALL(hipGetLastError());
+        dim3 g((unsigned)((f.M - j0 + 255) / 256), (unsigned)nc);
+        qr_update<T><<<g, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.W, C, ldc);
+        HIP_CALL(hipGetLastError());
+    }
+    return EXIT_SUCCESS;
+}
+
+template <typename T>
+static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
+    using C2 = typename QT<T>::c2;
+    const int np = (f.N + kNB - 1) / kNB;
+    HOST_CALL(dalloc(&f.Tb, (size_t)np * kNB * kNB));
+    f.wcols = std::max(f.N, extra_cols);
+    HOST_CALL(dalloc(&f.W, (size_t)kNB * f.wcols));
+    for (int p = 0; p < np; p++) {
+        const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
+        TimedRegion tr;
+        timing_begin(tr, sizeof(T) == 4 ? "qr_panel_f32" : "qr_panel_f64", s);
+        qr_panel<T><<<1, 1024, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Tb + (size_t)p * kNB * kNB);
+        HIP_CALL(hipGetLastError());
+        timing_end(tr, s);
+        const int nc = f.N - (j0 + nb);
+        if (nc > 0) {
+            C2 *Ct = f.A + (size_t)(j0 + nb) * f.lda;
+            TimedRegion tu;
+            timing_begin(tu, sizeof(T) == 4 ? "qr_trailing_f32" : "qr_trailing_f64", s);
+            qr_vhc<T><<<nc, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Tb + (size_t)p * kNB * kNB, Ct, f.lda, f.W, 1);
+            HIP_CALL(hipGetLastError());
+            dim3 g((unsigned)((f.M - j0 + 255) / 256), (unsigned)nc);
+            qr_update<T><<<g, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.W, Ct, f.lda);
+            HIP_CALL(hipGetLastError());
+            timing_end(tu, s);
+        }
+    }
+    return EXIT_SUCCESS;
+}
+
+// Least squares solve with a factored A: B (M x R, ldb) <- solution in B[0:N].
+template <typename T>
+static int qr_solve_factored(const QRFactor<T> &f, typename QT<T>::c2 *B, long ldb, int R, hipStream_t s) {
+    if (R <= 0) return EXIT_SUCCESS;
+    HOST_CALL(qr_apply<T>(f, B, ldb, R, 1, s));
+    trsv_upper<T><<<R, 1024, 0, s>>>(f.A, f.lda, f.N, B, ldb);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
+// Plain least squares: A (overwritten with its factorisation), B -> solution.
+template <typename T>
+static int qr_lsq_impl(typename QT<T>::c2 *A, long lda, int M, int N, typename QT<T>::c2 *B, long ldb, int R,
+                       hipStream_t s) {
+    if (M < N || N <= 0) BEM_FAIL("least squares needs M >= N > 0 (got %d x %d)", M, N);
+    QRFactor<T> f;
+    f.A = A;
+    f.lda = lda;
+    f.M = M;
+    f.N = N;
+    HOST_CALL(qr_factor<T>(f, R, s));
+    HOST_CALL(qr_solve_factored<T>(f, B, ldb, R, s));
+    HIP_CALL(hipStreamSynchronize(s));
+    return EXIT_SUCCESS;
+}
+
+int qr_lsq_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, hipStream_t s) {
+    return qr_lsq_impl<float>(A, lda, M, N, B, ldb, R, s);
+}
+int qr_lsq_f64(double2 *A, long lda, int M, int N, double2 *B, long ldb, int R, hipStream_t s) {
+    return qr_lsq_impl<double>(A, lda, M, N, B, ldb, R, s);
+}
+
+__global__ void k_identity(float2 *Q, long ldq, int M) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < M) Q[(size_t)j * ldq + i] = make_float2(i == j ? 1.f : 0.f, 0.f);
+}
+
+// Least squares plus explicit Q: A <- R (upper triangle, zero below), Q (M x M).
+int qr_lsq_with_q_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, float2 *Q, long ldq,
+                      hipStream_t s) {
+    if (M < N || N <= 0) BEM_FAIL("least squares needs M >= N > 0 (got %d x %d)", M, N);
+    QRFactor<float> f;
+    f.A = A;
+    f.lda = lda;
+    f.M = M;
+    f.N = N;
+    HOST_CALL(qr_factor<float>(f, std::max(R, M), s));
+    if (Q) {
+        dim3 g((unsigned)((M + 255) / 256), (unsigned)M);
+        k_identity<<<g, 256, 0, s>>>(Q, ldq, M);
+        HIP_CALL(hipGetLastError());
+        HOST_CALL(qr_apply<float>(f, Q, ldq, M, 0, s));
+    }
+    HOST_CALL(qr_solve_factored<float>(f, B, ldb, R, s));
+    HIP_CALL(hipStreamSynchronize(s));
+    return EXIT_SUCCESS;
+}
+
+// ------------------------------------------------ refined LS (FP32 + FP64) ---
+int resid_n_f32a(const float2 *A, long lda, long M, long N, const double2 *x, long ldx, const double2 *b, long ldb,
+                 double2 *r, long ldr, int R, void *ws, hipStream_t s);
+
+__global__ void k_z2c(const double2 *__restrict__ in, float2 *__restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
+}
+__global__ void k_addc2z(const float2 *__restrict__ d, double2 *__restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i].x += d[i].x;
+        x[i].y += d[i].y;
+    }
+}
+__global__ void k_sumsq(const double2 *__restrict__ v, int n, double *out);
+__global__ void k_sumsq_c(const float2 *__restrict__ v, int n, double *out);
+__global__ void k_sumsq(const double2 *__restrict__ v, int n, double *out) {
+    __shared__ double red[16];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += v[i].x * v[i].x + v[i].y * v[i].y;
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q];
+        *out = t;
+    }
+}
+__global__ void k_sumsq_c(const float2 *__restrict__ v, int n, double *out) {
+    __shared__ double red[16];
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += (double)v[i].x * v[i].x + (double)v[i].y * v[i].y;
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += red[q];
+        *out = t;
+    }
+}
+
+// FP32 QR least squares wrapped in FP64 iterative refinement: the fixed-point
+// iteration x <- x + QRsolve(b - A x) with the residual accumulated in FP64
+// from the FP32 matrix.  A32 (kept), B64: M x R FP64 rhs, X64: N x R FP64
+// solution (out).  Stops when ||correction|| <= refine_tol ||x|| or when the
+// correction no longer shrinks (an inconsistent least-squares system keeps a
+// nonzero residual, so the residual itself is not the test).
+int qr_refined_f32(const float2 *A32, long lda, int M, int N, const double2 *B64, long ldb, double2 *X64, long ldx,
+                   int R, double refine_tol, int max_refine, bem_solve_stats *st, hipStream_t s) {
+    if (M < N || N <= 0) BEM_FAIL("least squares needs M >= N > 0 (got %d x %d)", M, N);
+    QRFactor<float> f;
+    HOST_CALL(dalloc(&f.A, (size_t)lda * N));
+    HIP_CALL(hipMemcpyAsync(f.A, A32, sizeof(float2) * lda * N, hipMemcpyDeviceToDevice, s));
+    f.lda = lda;
+    f.M = M;
+    f.N = N;
+    float2 *rb = nullptr;
+    double2 *r64 = nullptr;
+    double *dn = nullptr;
+    void *gws = nullptr;
+    int refinements = 0;
+    double worst = 0;
+    bool conv = true;
+    int rc = EXIT_FAILURE;
+    const int bs = 256;
+    if (qr_factor<float>(f, 1, s) != EXIT_SUCCESS) goto out;
+    if (dalloc(&rb, (size_t)lda) || dalloc(&r64, (size_t)lda) || dalloc(&dn, 2)) goto out;
+    if (hipMalloc(&gws, ws_gemv_n(M, N, 1)) != hipSuccess) goto out;
+    for (int r = 0; r < R; r++) {
+        const double2 *b = B64 + (size_t)r * ldb;
+        double2 *x = X64 + (size_t)r * ldx;
+        if (hipMemsetAsync(x, 0, sizeof(double2) * N, s) != hipSuccess) goto out;
+        double prev = 1e300, rel = 0;
+        for (int it = 0; it <= max_refine; it++) {
+            if (resid_n_f32a(A32, lda, M, N, x, 0, b, 0, r64, 0, 1, gws, s) != EXIT_SUCCESS) goto out;
+            k_z2c<<<(M + bs - 1) / bs, bs, 0, s>>>(r64, rb, M);
+            if (qr_solve_factored<float>(f, rb, lda, 1, s) != EXIT_SUCCESS) goto out;
+            k_addc2z<<<(N + bs - 1) / bs, bs, 0, s>>>(rb, x, N);
+            k_sumsq_c<<<1, 1024, 0, s>>>(rb, N, dn);
+            k_sumsq<<<1, 1024, 0, s>>>(x, N, dn + 1);
+            double h[2];
+            if (hipMemcpyAsync(h, dn, 2 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess) goto out;
+            if (hipStreamSynchronize(s) != hipSuccess) goto out;
+            rel = std::sqrt(h[0] / std::max(h[1], 1e-300));
+            refinements++;
+            if (rel <= refine_tol || rel >= 0.5 * prev) break;
+            prev = rel;
+        }
+        worst = std::max(worst, rel);
+        conv = conv && rel <= std::max(refine_tol, 1e-5);
+    }
+    rc = EXIT_SUCCESS;
+out:
+    if (st) {
+        st->refinements = refinements;
+        st->iterations = 0;
+        st->converged = conv ? 1 : 0;
+        st->residual = worst;
+    }
+    dfree(f.A);
+    dfree(rb);
+    dfree(r64);
+    dfree(dn);
+    if (gws) (void)hipFree(gws);
+    (void)hipStreamSynchronize(s);
+    if (rc != EXIT_SUCCESS) BEM_FAIL("refined QR least squares failed");
+    return rc;
+}
+
+}  // namespace bem

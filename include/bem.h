@@ -1,0 +1,223 @@
+/*
+ * bem.h -- C-ABI of the MI355X boundary-element Helmholtz solver.
+ *
+ * Drop-in boundary for the hot path of ZiqiFrancisFan/BEM_GPU_Clean_CPP:
+ * OBJ mesh load -> per-frequency collocation assembly (optional CHIEF rows)
+ * -> least-squares QR or GMRES solve -> exterior field.  Each entry point below
+ * names the reference function it replaces (file:line).  All functions use
+ * plain pointers and sizes; the POD structs are bit-compatible with the
+ * reference's cuFloatComplex (8 B), cartCoord (12 B) and triElem (40 B).
+ *
+ * Conventions kept from the reference:
+ *   - matrices are column-major, A is M x N with lda >= M, rows are the N mesh
+ *     nodes followed by the C CHIEF points, columns are the nodes
+ *     (numerical.h:44-46, mesh.cu:993-999);
+ *   - the caller owns every host buffer; nothing keeps the caller's pointers;
+ *   - functions return EXIT_SUCCESS (0) or EXIT_FAILURE (1) and print
+ *     "Error at file:line" on failure (numerical.h:60-122); bem_last_error()
+ *     returns the message of the last failure in this thread;
+ *   - time convention e^{+i omega t}, G = exp(-ikr)/(4 pi r) (numerical.cu:70-74);
+ *   - air density 1.2041 and speed of sound 343.21 by default (numerical.cu:12,14),
+ *     the source strength of the incident point source is 0 by default
+ *     (numerical.h:128-130).
+ * Extensions the reference does not have are marked "extension".
+ */
+#ifndef BEM_H
+#define BEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cuFloatComplex (cuComplex.h): interleaved float {x = real, y = imag}. */
+typedef struct __attribute__((aligned(8))) bem_complex {
+    float x, y;
+} bem_complex;
+
+/* Double-precision complex, extension (the FP64 solve path). */
+typedef struct __attribute__((aligned(16))) bem_zcomplex {
+    double x, y;
+} bem_zcomplex;
+
+/* cartCoord (mesh.h:26-99): 3 floats, 12 bytes. */
+typedef struct bem_cart_coord {
+    float coords[3];
+} bem_cart_coord;
+
+/* triElem (mesh.h:140-209): 0-based node indices and the boundary condition
+ * ca*p + cb*v = cc as bc[0..2] = (ca, cb, cc); 40 bytes. */
+typedef struct bem_tri_elem {
+    int nodes[3];
+    bem_complex bc[3];
+} bem_tri_elem;
+
+/* mesh (mesh.h:216-259): nodes, elements, CHIEF points, bounding box. */
+typedef struct bem_mesh bem_mesh;
+
+/* ---------------------------------------------------------------- mesh --- */
+
+/* mesh::mesh() (mesh.h:244). */
+bem_mesh *bem_mesh_create(void);
+/* mesh::~mesh() (mesh.cu:1352-1356). */
+void bem_mesh_destroy(bem_mesh *m);
+/* mesh::readObj (mesh.cu:1215-1265): "v x y z" vertices and "f a b c" faces
+ * (1-based), default bc (ca, cb, cc) = (0, 1, -1).  Unlike the reference, "vn"
+ * and "vt" lines are not read as vertices and "f a/b/c" faces are accepted. */
+int bem_mesh_read_obj(bem_mesh *m, const char *file);
+/* Extension: set nodes and elements from arrays (copies them). */
+int bem_mesh_set(bem_mesh *m, const bem_cart_coord *pnts, int numPnts, const bem_tri_elem *elems,
+                 int numElems);
+/* Extension: set the same boundary condition ca*p + cb*v = cc on every element. */
+int bem_mesh_set_bc(bem_mesh *m, bem_complex ca, bem_complex cb, bem_complex cc);
+/* mesh::findBB (mesh.cu:1369-1409): bounding box widened by threshold. */
+int bem_mesh_find_bb(bem_mesh *m, float threshold);
+/* Extension: the bounding box found by bem_mesh_find_bb as {xl,xu,yl,yu,zl,zu}. */
+int bem_mesh_get_bb(const bem_mesh *m, float *bb6);
+/* mesh::genCHIEF (mesh.cu:1430-1483): rejection-sample num points inside the
+ * bounding box that are inside the surface (odd ray-crossing count) and at least
+ * threshold away from every node.  The reference reseeds cuRAND per candidate
+ * and casts the ray along the same uniforms; here a SplitMix64 stream seeded by
+ * `seed` draws the position and an independent ray direction on the sphere. */
+int bem_mesh_gen_chief(bem_mesh *m, int num, float threshold, unsigned long long seed);
+/* Extension: set the CHIEF points explicitly (copies them). */
+int bem_mesh_set_chief(bem_mesh *m, const bem_cart_coord *pnts, int num);
+/* mesh::getNumPnts / getNumElems / getNumChief (mesh.h:256-258). */
+int bem_mesh_num_pnts(const bem_mesh *m);
+int bem_mesh_num_elems(const bem_mesh *m);
+int bem_mesh_num_chief(const bem_mesh *m);
+/* Extension: copy out nodes / elements / CHIEF points (arrays sized by the getters). */
+int bem_mesh_get_pnts(const bem_mesh *m, bem_cart_coord *out);
+int bem_mesh_get_elems(const bem_mesh *m, bem_tri_elem *out);
+int bem_mesh_get_chief(const bem_mesh *m, bem_cart_coord *out);
+/* Extension: icosphere of radius r at subdivision s (10*4^s+2 nodes), wound
+ * like the reference meshes (normals into the body, negative signed volume). */
+int bem_mesh_gen_icosphere(bem_mesh *m, int subdiv, float radius);
+/* Extension: perturbed ellipsoid: icosphere s mapped to semi-axes (ax,ay,az),
+ * radial factor 1 + sum_b a_b Y_b(u) over nbumps seeded real spherical
+ * harmonics up to degree lmax, sum |a_b| <= amp. */
+int bem_mesh_gen_ellipsoid(bem_mesh *m, int subdiv, float ax, float ay, float az, int nbumps,
+                           int lmax, float amp, unsigned long long seed);
+/* Extension: signed volume (negative for the reference's winding). */
+double bem_mesh_signed_volume(const bem_mesh *m);
+/* cmptSurfArea (mesh.cu:1091-1117), computed on the host. */
+float bem_mesh_surf_area(const bem_mesh *m);
+
+/* ------------------------------------------------------- global set-up --- */
+
+/* gaussQuad::gaussQuad(order) + sendToDevice (numerical.cu:152-156,227-231):
+ * Gauss-Legendre tables for the 6x6 collapsed rule.  Order 6 (INTORDER,
+ * numerical.h:40-42) is the only order the GPU kernels are built for. */
+int bem_quad_init(int order);
+/* Extension: copy the current nodes/weights (order values each). */
+int bem_quad_get(double *pnts, double *wgts);
+/* Extension: medium constants (the reference's __constant__ density/speed). */
+int bem_set_medium(float density, float speed);
+/* Extension: strength q of the incident point source (STRENGTH, numerical.h:128-130). */
+int bem_set_source_strength(float q);
+
+/* -------------------------------------------------- reference entry points --- */
+
+/* bemSystem (mesh.h:222-223, mesh.cu:985-1006) == atomicGenSystem
+ * (atomicFuncs.cu:218-298): assemble A (M x N, lda) and B (M x numSrcs, ldb)
+ * for wavenumber k; B[:, s] = q*G(x, srcs[s]) - sum_e (cc/cb) g_e(x).
+ * A and B are host buffers.  Assembly is deterministic (no atomics). */
+int bem_system(const bem_mesh *m, float k, const bem_cart_coord *srcs, int numSrcs, bem_complex *A,
+               int lda, bem_complex *B, int ldb);
+/* atomicGenSystem (atomicFuncs.h:39-42): the same on raw arrays; pnts holds
+ * the numNods nodes followed by the numCHIEF CHIEF points. */
+int bem_gen_system(float k, const bem_tri_elem *elems, int numElems, const bem_cart_coord *pnts,
+                   int numNods, int numCHIEF, const bem_cart_coord *srcs, int numSrcs, bem_complex *A,
+                   int lda, bem_complex *B, int ldb);
+/* qrSolver (numerical.h:196-197, numerical.cu:580-644): least squares
+ * min ||A X - B|| for mA >= nA by Householder QR; the solution overwrites
+ * B[0:nA, 0:nB].  A is not modified. */
+int bem_qr_solve(const bem_complex *A, int mA, int nA, int ldA, bem_complex *B, int nB, int ldB);
+/* lsqSolver (numerical.h:189-190, numerical.cu:506-554): the same least
+ * squares for m > n; A is overwritten with R (upper triangle), and if Q is not
+ * NULL it receives the explicit m x m unitary factor (ld m). */
+int bem_lsq_solve(bem_complex *A, int m, int n, int lda, bem_complex *B, int nrhs, int ldb,
+                  bem_complex *Q);
+/* GMRES (GMRES.h:104-105, GMRES.cu:145-279): square m x m (lda = m) system,
+ * unrestarted, initial guess in x, stops when the residual estimate
+ * |err|/||b|| < threshold or after maxIter steps; solution in x. */
+int bem_gmres(const bem_complex *A, const bem_complex *b, int m, int maxIter, float threshold,
+              bem_complex *x);
+/* genExtPressure (mesh.h:425-426, mesh.cu:1038-1073): pressure at one
+ * exterior point x from the nodal surface pressure:
+ * q*G(x, src) - sum_e [sum_j (h_j - (ca/cb) g_j) p_j + (cc/cb) sum_j g_j]. */
+bem_complex bem_ext_pressure(float k, const bem_mesh *m, bem_cart_coord src, bem_cart_coord x,
+                             const bem_complex *surfPressure);
+/* wrtCplxMat (numerical.h:193-194, numerical.cu:556-578): "(%f,%f) " rows. */
+int bem_wrt_cplx_mat(const bem_complex *mat, int numRows, int numCols, int ldm, const char *file);
+/* printComplexMatrix (numerical.cu:81-93) on stdout. */
+void bem_print_complex_matrix(const bem_complex *A, int row, int col, int lda);
+
+/* ------------------------------------ device-resident solver (extension) --- */
+/* A solver keeps the mesh, the assembled matrix and the Krylov state in HBM
+ * between calls; inputs/outputs cross PCIe only when asked. */
+typedef struct bem_solver bem_solver;
+
+enum { BEM_PREC_FP32 = 32, BEM_PREC_FP64 = 64 };
+enum { BEM_RHS_POINT = 0, BEM_RHS_PLANE = 1 };
+enum { BEM_SOLVE_QR = 0, BEM_SOLVE_GMRES = 1 };
+
+typedef struct bem_solve_stats {
+    int method;           /* BEM_SOLVE_QR or BEM_SOLVE_GMRES */
+    int iterations;       /* total inner Krylov iterations (all refinement steps) */
+    int refinements;      /* outer FP64 refinement steps */
+    int converged;        /* 1 when the requested tolerance was reached */
+    double residual;      /* final relative FP64 residual of the (normal) equations */
+    double assemble_ms;   /* last assembly, device time */
+    double solve_ms;      /* last solve, wall time */
+} bem_solve_stats;
+
+/* Create a solver for the mesh (nodes, elements, CHIEF points are copied to HBM). */
+bem_solver *bem_solver_create(const bem_mesh *m);
+void bem_solver_destroy(bem_solver *s);
+/* Assemble A and the boundary-condition part of B at wavenumber k in FP32
+ * (complex64 A) or FP64 (complex128 A). */
+int bem_solver_assemble(bem_solver *s, float k, int precision);
+/* Right-hand sides: kind BEM_RHS_POINT (params = x,y,z per source, incident
+ * q*G(x, src)) or BEM_RHS_PLANE (params = unit direction per wave, incident
+ * amp*exp(-i k d.x)); nrhs <= 64. */
+int bem_solver_set_rhs(bem_solver *s, int kind, const float *params, int nrhs, float amp);
+/* Solve for all right-hand sides.  method BEM_SOLVE_GMRES: restarted GMRES
+ * (restart, tol relative) on the square system, or on the normal equations
+ * A^H A x = A^H b when CHIEF rows make it overdetermined; the FP32 path is
+ * wrapped in FP64 residual refinement until refine_tol (<= 0: one pass).
+ * method BEM_SOLVE_QR: blocked Householder least squares (FP32 path also
+ * refined in FP64). */
+int bem_solver_solve(bem_solver *s, int method, int restart, float tol, int maxit, double refine_tol,
+                     int max_refine, bem_solve_stats *stats);
+/* Copy the nodal solution (N x nrhs, ldx) to the host. */
+int bem_solver_get_solution(const bem_solver *s, bem_complex *x, int ldx);
+int bem_solver_get_solution_z(const bem_solver *s, bem_zcomplex *x, int ldx);
+/* Copy the assembled A (M x N) / B (M x nrhs) to the host as complex64. */
+int bem_solver_get_system(const bem_solver *s, bem_complex *A, int lda, bem_complex *B, int ldb);
+/* Exterior field at P points for every right-hand side: out is P x nrhs (ldo). */
+int bem_solver_field(bem_solver *s, const bem_cart_coord *pts, int P, bem_complex *out, int ldo);
+/* Sizes: M rows, N columns, current nrhs. */
+int bem_solver_dims(const bem_solver *s, int *M, int *N, int *nrhs);
+/* Last solve statistics. */
+int bem_solver_stats(const bem_solver *s, bem_solve_stats *stats);
+
+/* ----------------------------------------------------- timing (extension) --- */
+/* HIP-event timing of the library's kernels on the stream they run on.
+ * bem_timing_enable(1) starts recording; bem_timing_get returns the total
+ * milliseconds and launch count recorded for a kernel name since the last
+ * bem_timing_reset. */
+int bem_timing_enable(int on);
+int bem_timing_reset(void);
+int bem_timing_get(const char *kernel, double *total_ms, long *count);
+/* Synchronise the library's stream. */
+int bem_sync(void);
+
+/* Message of the last failure in this thread ("" if none). */
+const char *bem_last_error(void);
+/* Library version string. */
+const char *bem_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEM_H */

@@ -1,0 +1,72 @@
+"""Shared problem set-ups for the tests: meshes, CHIEF points, right-hand sides,
+the analytic pulsating- and rigid-sphere solutions, and golden-file parsing."""
+import os
+import re
+
+import numpy as np
+from scipy import special
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def splitmix64(seed):
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    while True:
+        s = (s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        yield (z ^ (z >> 31)) >> 11
+
+
+def uniforms(seed, n):
+    g = splitmix64(seed)
+    return np.array([next(g) * (1.0 / 9007199254740992.0) for _ in range(n)])
+
+
+def ball_points(seed, n, radius):
+    """n points uniform in the ball |x| < radius (rejection from the cube)."""
+    out = []
+    u = iter(uniforms(seed, 300 * n + 300))
+    while len(out) < n:
+        p = np.array([2 * next(u) - 1, 2 * next(u) - 1, 2 * next(u) - 1]) * radius
+        if np.linalg.norm(p) < radius:
+            out.append(p)
+    return np.array(out, np.float32)
+
+
+def sphere_points(seed, n, radius):
+    """n points uniform on the sphere |x| = radius (normalised Gaussians)."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((n, 3))
+    return (radius * v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def read_golden(name):
+    s = open(os.path.join(GOLDEN, name)).read()
+    return np.array([complex(float(a), float(b)) for a, b in re.findall(r"\(([-\d.eE+]+),([-\d.eE+]+)\)", s)])
+
+
+def pulsating_sphere(r, a, k, rho, c):
+    """sphere.m:11 -- p = i rho omega a^2/(r (1 + i k a)) exp(-i k (r - a))."""
+    omega = k * c
+    return 1j * rho * omega * a ** 2 / (r * (1 + 1j * k * a)) * np.exp(-1j * k * (r - a))
+
+
+def rigid_sphere_surface(a, k, cos_theta, nmax=None):
+    """Total pressure on a rigid sphere of radius a for p_inc = exp(-i k z)
+    (e^{+i omega t}): p = -(i/(ka)^2) sum_n (2n+1) (-i)^n P_n(cos t) / h_n^(2)'(ka)."""
+    x = k * a
+    if nmax is None:
+        nmax = int(x + 4 * x ** (1 / 3) + 12)
+    p = np.zeros_like(np.asarray(cos_theta, float), dtype=complex)
+    for n in range(nmax + 1):
+        dh = special.spherical_jn(n, x, derivative=True) - 1j * special.spherical_yn(n, x, derivative=True)
+        p += (2 * n + 1) * (-1j) ** n * special.eval_legendre(n, cos_theta) / dh
+    return -1j / x ** 2 * p
+
+
+def mesh_arrays(mesh):
+    """(pnts float32 N x 3, elems int32 E x 3, bc complex E x 3) of a bem Mesh."""
+    e = mesh.elems()
+    return mesh.pnts(), np.ascontiguousarray(e["nodes"]), np.ascontiguousarray(e["bc"]).astype(np.complex128)

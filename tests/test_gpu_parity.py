@@ -1,0 +1,283 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle.
+
+Tolerances:
+  * FP32 assembly vs FP64 oracle: max |dA| <= 2e-5 * max |A| (complex FP32
+    quadrature; the reference computes in FP32 with fast-math);
+  * FP64 assembly vs FP64 oracle: max |dA| <= 1e-11 * max |A|;
+  * FP32 solves with FP64 refinement vs FP64 oracle solution: relative 2-norm
+    error <= 1e-5 (the north-star accuracy bar);
+  * repeat assemblies are bitwise identical (no atomics).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import bem_gpu_clean_cpp_amd as bem
+import bemcases as C
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+# the C-ABI takes float density / speed; the oracle gets the same rounded values
+RHO, SPEED = float(np.float32(1.21)), float(np.float32(343.0))
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _setup():
+    bem.quad_init(6)
+    O.quad_init(6)
+    bem.set_medium(RHO, SPEED)
+    yield
+    bem.set_medium(1.2041, 343.21)
+
+
+def _oracle_system(mesh, k, binit=None):
+    p, e, bc = C.mesh_arrays(mesh)
+    pts = np.vstack([p, mesh.chief()]).astype(np.float64)
+    return O.assemble(k, pts, p.shape[0], e, bc, binit=binit, dens=RHO, speed=SPEED)
+
+
+def _rel_max(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def _icosphere(s, r=0.1, nchief=0, seed=101):
+    m = bem.Mesh().icosphere(s, r)
+    if nchief:
+        m.set_chief(C.ball_points(seed, nchief, r / 2))
+    return m
+
+
+@pytest.mark.parametrize("qg", ["1", "4"])
+@pytest.mark.parametrize("k", [3.13, 36.6])
+def test_assembly_fp32_matches_oracle(qg, k, monkeypatch):
+    monkeypatch.setenv("BEM_QG", qg)
+    m = _icosphere(2, nchief=4)
+    print("case: icosphere s2 + 4 CHIEF, k=%g, QG=%s" % (k, qg), flush=True)
+    A, B = bem.bem_system(m, k)
+    Ao, Bo = _oracle_system(m, k)
+    assert A.shape == Ao.shape
+    err = _rel_max(A, Ao)
+    print("FP32 assembly max rel err %.3g" % err)
+    assert err < 2e-5
+
+
+def test_assembly_fp64_matches_oracle():
+    m = _icosphere(2, nchief=4)
+    s = bem.Solver(m)
+    k = 20.0
+    s.assemble(k, bem.PREC_FP64)
+    s.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+    A, B = s.system()  # complex64 copy of the FP64 matrix
+    Ao, _ = _oracle_system(m, k)
+    assert _rel_max(A, Ao) < 1e-6  # limited by the complex64 copy-out
+    # the FP64 solve checks the FP64 matrix to FP64 accuracy
+    s.solve(bem.SOLVE_QR)
+    x = s.solution()
+    p = m.pnts().astype(np.float64)
+    binit = np.exp(-1j * k * np.vstack([p, m.chief()])[:, 2:3].astype(np.float64))
+    Ao, Bo = _oracle_system(m, k, binit=binit)
+    xo = np.linalg.lstsq(Ao, Bo, rcond=None)[0]
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_assembly_reference_mesh_and_bc():
+    """sphere.obj with the reference's default bc (0,1,-1) and 1 CHIEF point."""
+    m = bem.Mesh().read_obj(os.path.join(C.GOLDEN, "sphere.obj"))
+    m.set_chief([[0.1, -0.2, 0.05]])
+    k = float(np.float32(2 * np.pi * 171.5 / 343.21))
+    bem.set_medium(1.2041, 343.21)
+    try:
+        A, B = bem.bem_system(m, k, srcs=[[1e6, 1e6, 1e6]])
+        p, e, bc = C.mesh_arrays(m)
+        pts = np.vstack([p, m.chief()]).astype(np.float64)
+        Ao, Bo = O.assemble(k, pts, p.shape[0], e, bc, binit=np.zeros((pts.shape[0], 1)),
+                            dens=float(np.float32(1.2041)), speed=float(np.float32(343.21)))
+        assert _rel_max(A, Ao) < 2e-5
+        assert _rel_max(B, Bo) < 2e-5
+    finally:
+        bem.set_medium(RHO, SPEED)
+
+
+def test_assembly_impedance_bc():
+    """ca != 0 exercises the h - (ca/cb) g path of the kernels."""
+    m = _icosphere(2, nchief=2)
+    m.set_bc(0.5 + 0.25j, 1.0 - 0.5j, 0.3 + 0.1j)
+    k = 12.0
+    A, B = bem.bem_system(m, k, srcs=[[0.5, 0.2, 0.1]])
+    p, e, bc = C.mesh_arrays(m)
+    pts = np.vstack([p, m.chief()]).astype(np.float64)
+    Ao, Bo = O.assemble(k, pts, p.shape[0], e, bc, binit=np.zeros((pts.shape[0], 1)), dens=RHO, speed=SPEED)
+    assert _rel_max(A, Ao) < 2e-5
+    assert _rel_max(B, Bo) < 2e-5
+
+
+def test_assembly_bitwise_repeatable():
+    m = _icosphere(3, nchief=8)
+    A1, _ = bem.bem_system(m, 30.0)
+    A2, _ = bem.bem_system(m, 30.0)
+    a1 = np.ascontiguousarray(A1).view(np.uint32)
+    a2 = np.ascontiguousarray(A2).view(np.uint32)
+    assert np.array_equal(a1, a2)
+
+
+def test_point_source_rhs_strength():
+    """B = q G(x, src) - BC term (atomicFuncs.cu:238-242) with q set."""
+    m = _icosphere(2)
+    k = 10.0
+    src = np.array([[0.3, -0.1, 0.2]], np.float32)
+    bem.set_source_strength(2.5)
+    try:
+        A, B = bem.bem_system(m, k, srcs=src)
+    finally:
+        bem.set_source_strength(0.0)
+    p = m.pnts().astype(np.float64)
+    r = np.linalg.norm(p - src.astype(np.float64), axis=1)
+    inc = 2.5 * np.exp(-1j * k * r) / (4 * np.pi * r)
+    _, Bo = _oracle_system(m, k, binit=inc[:, None])
+    assert _rel_max(B, Bo) < 2e-5
+
+
+def test_qr_solve_matches_lapack():
+    rng = np.random.default_rng(1)
+    m, n = 300, 260
+    A = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+    A += 20 * np.eye(m, n, dtype=np.complex64)
+    B = (rng.standard_normal((m, 2)) + 1j * rng.standard_normal((m, 2))).astype(np.complex64)
+    X = bem.qr_solve(A, B)
+    Xo = np.linalg.lstsq(A.astype(np.complex128), B.astype(np.complex128), rcond=None)[0]
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-5
+
+
+def test_lsq_solve_returns_r_and_q():
+    rng = np.random.default_rng(2)
+    m, n = 90, 70
+    A = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+    B = (rng.standard_normal((m, 1)) + 1j * rng.standard_normal((m, 1))).astype(np.complex64)
+    X, R, Q = bem.lsq_solve(A, B, want_q=True)
+    assert np.allclose(np.tril(R, -1), 0)
+    assert np.linalg.norm(Q.conj().T @ Q - np.eye(m)) < 1e-4
+    assert np.linalg.norm(Q[:, :n] @ R[:n] - A) / np.linalg.norm(A) < 1e-5
+    Xo = np.linalg.lstsq(A.astype(np.complex128), B.astype(np.complex128), rcond=None)[0]
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-5
+
+
+def test_gmres_square_matches_lapack():
+    m = _icosphere(3)
+    k = 20.0
+    A, _ = bem.bem_system(m, k)
+    p = m.pnts().astype(np.float64)
+    b = np.exp(-1j * k * p[:, 2]).astype(np.complex64)
+    x = bem.gmres(A, b, 200, 1e-6)
+    xo = np.linalg.solve(A.astype(np.complex128), b.astype(np.complex128))
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 2e-5
+
+
+@pytest.mark.parametrize("method", [bem.SOLVE_GMRES, bem.SOLVE_QR])
+def test_refined_solve_chief_1e5(method):
+    """FP32 assembly + FP32 solve + FP64 refinement vs the FP64 oracle path."""
+    m = _icosphere(3, nchief=8)
+    k = 2 * np.pi * 171 / SPEED * 10  # ka ~ 3.1
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, [[0, 0, 1], [0.6, 0.0, 0.8]], 1.0)
+    st = s.solve(method, restart=30, tol=1e-6, maxit=2000, refine_tol=1e-10, max_refine=50)
+    x = s.solution()
+    pts = np.vstack([m.pnts(), m.chief()]).astype(np.float64)
+    d = np.array([[0, 0, 1], [0.6, 0.0, 0.8]])
+    binit = np.exp(-1j * k * pts @ d.T)
+    Ao, Bo = _oracle_system(m, k, binit=binit)
+    xo = np.linalg.lstsq(Ao, Bo, rcond=None)[0]
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print("method %d stats %s err %.3g" % (method, st, err))
+    assert st["converged"] == 1
+    assert err < 1e-5
+
+
+def test_fp64_gmres_plain():
+    m = _icosphere(3, nchief=4)
+    k = 15.0
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP64)
+    s.set_rhs(bem.RHS_POINT, [[0.3, 0.0, 0.0]], 1.0)
+    st = s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=3000, refine_tol=0, max_refine=0)
+    x = s.solution()
+    pts = np.vstack([m.pnts(), m.chief()]).astype(np.float64)
+    r = np.linalg.norm(pts - np.array([0.3, 0.0, 0.0]), axis=1)
+    Ao, Bo = _oracle_system(m, k, binit=(np.exp(-1j * k * r) / (4 * np.pi * r))[:, None])
+    xo = np.linalg.lstsq(Ao, Bo, rcond=None)[0]
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9, st
+
+
+def test_field_matches_oracle():
+    m = _icosphere(2, nchief=2)
+    k = 8.0
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    dirs = [[0, 0, 1], [1, 0, 0], [0, 0.6, 0.8]]
+    s.set_rhs(bem.RHS_PLANE, dirs, 1.0)
+    s.solve(bem.SOLVE_QR)
+    x = s.solution()
+    xs = C.sphere_points(5, 64, 1.0)
+    out = s.field(xs)
+    p, e, bc = C.mesh_arrays(m)
+    inc = np.exp(-1j * k * xs.astype(np.float64) @ np.array(dirs, float).T)
+    fo = O.field(k, p.astype(np.float64), p.shape[0], e, bc, xs, x, inc=inc, dens=RHO, speed=SPEED)
+    assert np.max(np.abs(out - fo)) / np.max(np.abs(fo)) < 2e-5
+
+
+def test_ext_pressure_single_point():
+    m = _icosphere(2)
+    k = 5.0
+    rng = np.random.default_rng(3)
+    sp = (rng.standard_normal(m.num_pnts) + 1j * rng.standard_normal(m.num_pnts)).astype(np.complex64)
+    x = [0.0, 0.0, 0.7]
+    got = bem.ext_pressure(k, m, [1e6, 1e6, 1e6], x, sp)
+    p, e, bc = C.mesh_arrays(m)
+    fo = O.field(k, p.astype(np.float64), p.shape[0], e, bc, np.array([x]), sp, dens=RHO, speed=SPEED)
+    assert abs(got - fo[0, 0]) / abs(fo[0, 0]) < 2e-5
+
+
+def test_pulsating_sphere_reference_golden():
+    """The reference's CHIEF golden file (sphere.obj, f = 171.5 Hz, 1 CHIEF point)."""
+    bem.set_medium(1.2041, 343.21)
+    try:
+        m = bem.Mesh().read_obj(os.path.join(C.GOLDEN, "sphere.obj"))
+        m.find_bb(1e-4).gen_chief(1, 0.1, seed=1)
+        k = float(np.float32(2 * np.pi * 171.5 / 343.21))
+        s = bem.Solver(m)
+        s.assemble(k, bem.PREC_FP32)
+        s.set_rhs(bem.RHS_POINT, [[1e6, 1e6, 1e6]], 0.0)
+        s.solve(bem.SOLVE_QR)
+        x = s.solution()
+        r = np.arange(1.0, 3.01, 0.2)
+        xs = np.array([[-(1 + 0.2 * i), 0, 0] for i in range(1, 11)], np.float32)
+        p = np.concatenate([[x[0, 0]], s.field(xs)[:, 0]])
+        gold = C.read_golden("sphere171Hz_CHIEF")
+        ana = C.pulsating_sphere(r, 1.0, k, 1.2041, 343.21)
+        e_gold = np.linalg.norm(p - gold) / np.linalg.norm(gold)
+        e_ana = np.linalg.norm(p - ana) / np.linalg.norm(ana)
+        print("vs golden %.4f, vs analytic %.4f" % (e_gold, e_ana))
+        assert e_gold < 0.015
+        assert e_ana < 0.025
+    finally:
+        bem.set_medium(RHO, SPEED)
+
+
+def test_rigid_sphere_series_cfg1():
+    """Plain small case: icosphere s3, r = 0.1 m, 171 Hz, 8 CHIEF, plane wave +z."""
+    a = 0.1
+    m = _icosphere(3, a, nchief=8)
+    m.set_bc(0, 1, 0)  # rigid: zero normal velocity
+    k = 2 * np.pi * 171 / SPEED
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+    s.solve(bem.SOLVE_QR)
+    x = s.solution()[:, 0]
+    p = m.pnts().astype(np.float64)
+    ser = C.rigid_sphere_surface(a, k, p[:, 2] / np.linalg.norm(p, axis=1))
+    err = np.linalg.norm(x - ser) / np.linalg.norm(ser)
+    print("rigid sphere series error %.4g" % err)
+    assert err < 0.005
