@@ -1,0 +1,144 @@
+"""The C-ABI library on the CPU (no GPU compute): it loads, exports every symbol
+include/bem.h declares, and its host-side mesh front-end and quadrature tables
+behave as the reference's (OBJ reader, bounding box, CHIEF generator) and as the
+oracle."""
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+import bem_gpu_clean_cpp_amd as bem
+import bemcases as C
+import oracle_py as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    src = open(os.path.join(ROOT, "include", "bem.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bem_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = _header_functions()
+    assert len(names) >= 40
+    L = bem.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(bem.EXPORTED)
+    assert b"gfx950" in L.bem_version()
+
+
+def test_struct_layout_matches_reference():
+    import ctypes
+    assert ctypes.sizeof(bem.bem_complex) == 8      # cuFloatComplex
+    assert ctypes.sizeof(bem.bem_cart_coord) == 12  # cartCoord
+    assert ctypes.sizeof(bem.bem_tri_elem) == 40    # triElem
+    assert bem.bem_tri_elem.bc.offset == 16
+
+
+def test_quadrature_tables():
+    bem.quad_init(6)
+    x, w = bem.quad_get(6)
+    xr, wr = np.polynomial.legendre.leggauss(6)
+    assert np.allclose(x, xr, atol=1e-14) and np.allclose(w, wr, atol=1e-14)
+    with pytest.raises(bem.BemError):
+        bem.quad_init(5)  # kernels are built for the 6-point rule (INTORDER)
+
+
+@pytest.mark.parametrize("name", ["sphere.obj", "sphere1.obj", "cube.obj", "test.obj"])
+def test_read_obj_matches_reference_semantics(name):
+    path = os.path.join(C.GOLDEN, name)
+    m = bem.Mesh().read_obj(path)
+    p, e = O.read_obj(path)
+    assert np.array_equal(m.pnts(), p)
+    el = m.elems()
+    assert np.array_equal(el["nodes"], e)
+    assert np.all(el["bc"] == np.array([0, 1, -1], np.complex64))  # mesh.cu:1240-1242
+
+
+def test_read_obj_robust_lines():
+    txt = "# c\nv 0 0 0\nvn 0 0 1\nvt 0.5 0.5\nv 1 0 0\nv 0 1 0\nv 0 0 1\nf 1/1/1 3/2/1 2/3/1\nf -4 -2 -1\nf 1 2 3 4\n"
+    with tempfile.NamedTemporaryFile("w", suffix=".obj", delete=False) as f:
+        f.write(txt)
+    try:
+        m = bem.Mesh().read_obj(f.name)
+        assert m.num_pnts == 4  # vn / vt are not vertices
+        assert m.num_elems == 4  # quad fanned into two triangles
+        assert list(m.elems()["nodes"][1]) == [0, 2, 3]
+    finally:
+        os.unlink(f.name)
+
+
+def test_mesh_errors_are_reported():
+    with pytest.raises(bem.BemError, match="open"):
+        bem.Mesh().read_obj("/nonexistent/x.obj")
+    with pytest.raises(bem.BemError, match="outside"):
+        bem.Mesh().set([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 5]])
+    with pytest.raises(bem.BemError):
+        bem.Mesh().find_bb(0.1)  # empty mesh (mesh.cu:1405-1407)
+    with pytest.raises(bem.BemError):
+        bem.set_medium(-1.0, 343.0)
+    m = bem.Mesh().icosphere(1, 1.0)
+    with pytest.raises(bem.BemError):
+        m.set_bc(0, 0, 1)  # cb = 0 would divide by zero
+
+
+@pytest.mark.parametrize("s", [0, 1, 2, 3, 4])
+def test_icosphere(s):
+    a = 0.1
+    m = bem.Mesh().icosphere(s, a)
+    assert m.num_pnts == 10 * 4 ** s + 2 and m.num_elems == 20 * 4 ** s
+    p = m.pnts().astype(np.float64)
+    assert np.allclose(np.linalg.norm(p, axis=1), a, rtol=1e-6)
+    assert m.signed_volume() < 0  # normals into the body, like the reference meshes
+    if s >= 3:
+        assert abs(m.surf_area() - 4 * np.pi * a * a) / (4 * np.pi * a * a) < 0.01
+
+
+def test_perturbed_ellipsoid():
+    ax = (0.075, 0.09, 0.11)
+    m = bem.Mesh().ellipsoid(4, ax, nbumps=8, lmax=6, amp=0.05, seed=3)
+    assert m.num_pnts == 2562
+    vol = -m.signed_volume()
+    ref = 4 / 3 * np.pi * np.prod(ax)
+    assert abs(vol - ref) / ref < 0.1
+    m2 = bem.Mesh().ellipsoid(4, ax, nbumps=8, lmax=6, amp=0.05, seed=3)
+    assert np.array_equal(m.pnts(), m2.pnts())
+
+
+def test_find_bb_and_chief_generator():
+    a = 0.1
+    m = bem.Mesh().icosphere(3, a)
+    m.find_bb(1e-4)
+    bb = m.bb()
+    assert np.allclose(bb, [-a - 1e-4, a + 1e-4] * 3, atol=1e-6)
+    m.gen_chief(8, 0.03, seed=5)
+    c = m.chief()
+    assert c.shape == (8, 3)
+    p = m.pnts()
+    for x in c:
+        assert np.min(np.linalg.norm(p - x, axis=1)) >= 0.03
+        assert np.linalg.norm(x) < a  # inside the sphere
+    m2 = bem.Mesh().icosphere(3, a).find_bb(1e-4).gen_chief(8, 0.03, seed=5)
+    assert np.array_equal(m2.chief(), c)
+    with pytest.raises(bem.BemError):
+        bem.Mesh().icosphere(2, a).find_bb(1e-4).gen_chief(4, 0.5, seed=1)  # impossible threshold
+
+
+def test_wrt_cplx_mat_golden_format():
+    v = np.array([[1.5 + 2.25j, -3.125 + 0.5j]])
+    with tempfile.NamedTemporaryFile("r", suffix=".txt", delete=False) as f:
+        name = f.name
+    try:
+        bem.wrt_cplx_mat(v, name)
+        txt = open(name).read()
+        assert txt == "(1.500000,2.250000) (-3.125000,0.500000) \n"
+        assert np.allclose(C.read_golden(os.path.basename(name)) if False else
+                           np.array([complex(float(a), float(b)) for a, b in
+                                     re.findall(r"\(([-\d.]+),([-\d.]+)\)", txt)]), v[0])
+    finally:
+        os.unlink(name)
