@@ -383,6 +383,189 @@ __global__ __launch_bounds__(256) void asm_tile(AsmArgs<R> a) {
     }
 }
 
+// ------------------------------------------- FP32 fast path (one lane/row) ---
+// The 6x6 collapsed rule is separable: with theta_n, rho_m the mapped
+// Gauss-Legendre nodes, xi = rho_m ((1 - theta_n), theta_n) and
+// W = 0.25 w_n w_m rho_m, so the point offset is y - x = a + rho_m v_n with
+// v_n = e1 + theta_n (e2 - e1) (one vector per theta row), and the
+// shape-function sums factor per theta row:
+//   h_1 = sum_n (1 - theta_n) A_n,  h_2 = sum_n theta_n A_n,  h_tot = sum_n B_n,
+//   A_n = sum_m W rho_m H~,  B_n = sum_m W H~.
+// All rule constants are compile-time literals.  The phase k r is reduced to
+// half-revolutions with the round-to-nearest magic constant 1.5 * 2^23
+// (u = k r / pi - j exactly, |u| <= 1/2), and (-1)^j is folded into 1/r.
+struct GL6 {
+    static constexpr double eta[6] = {-0.9324695142031520278123016, -0.6612093864662645136613996,
+                                      -0.2386191860831969086305017, 0.2386191860831969086305017,
+                                      0.6612093864662645136613996,  0.9324695142031520278123016};
+    static constexpr double w[6] = {0.1713244923791703450402961, 0.3607615730481386075698335,
+                                    0.4679139345726910473898703, 0.4679139345726910473898703,
+                                    0.3607615730481386075698335, 0.1713244923791703450402961};
+};
+__device__ __forceinline__ constexpr float gl_theta(int n) { return (float)(0.5 + 0.5 * GL6::eta[n]); }
+__device__ __forceinline__ constexpr float gl_rho(int m) { return (float)(0.5 + 0.5 * GL6::eta[m]); }
+__device__ __forceinline__ constexpr float gl_W(int n, int m) {
+    return (float)(0.25 * GL6::w[n] * GL6::w[m] * (0.5 + 0.5 * GL6::eta[m]));
+}
+__device__ __forceinline__ constexpr float gl_Wrho(int n, int m) {
+    return (float)(0.25 * GL6::w[n] * GL6::w[m] * (0.5 + 0.5 * GL6::eta[m]) * (0.5 + 0.5 * GL6::eta[m]));
+}
+
+// sin(pi u), cos(pi u) for |u| <= 1/2: near-minimax polynomials fitted for this
+// library (max abs error 1.7e-7 after FP32 rounding).
+__device__ __forceinline__ void sincos_pi_half(float u, float &sn, float &cs) {
+    const float u2 = u * u;
+    float s = __builtin_fmaf(u2, 0.0772201344370842f, -0.5980451703071594f);
+    s = __builtin_fmaf(u2, s, 2.5500314235687256f);
+    s = __builtin_fmaf(u2, s, -5.1677069664001465f);
+    s = __builtin_fmaf(u2, s, 3.141592502593994f);
+    sn = u * s;
+    float c = __builtin_fmaf(u2, 0.22008083760738373f, -1.3320939540863037f);
+    c = __builtin_fmaf(u2, c, 4.058451175689697f);
+    c = __builtin_fmaf(u2, c, -4.934795379638672f);
+    cs = __builtin_fmaf(u2, c, 1.0f);
+}
+
+template <bool IMP>
+__global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int L = xcd_remap(blockIdx.x, a.nwg);
+    const int tile = L / a.nrbg;
+    const int rbg = L - tile * a.nrbg;
+    const int row = (rbg * 4 + wave) * 64 + lane;
+    const bool active = row < a.M;
+    const float4 xp = a.pts[active ? row : 0];
+    const float k = a.k;
+    const float kpi = a.kf * 2.0f;  // k / pi
+    const float inv4pi = (float)(1.0 / (4.0 * kPi));
+    const float MAGIC = 12582912.0f;  // 1.5 * 2^23
+    float c_acc = 0, br_acc = 0, bi_acc = 0;
+    const int it0 = a.tile_ptr[tile], it1 = a.tile_ptr[tile + 1];
+    for (int it = it0; it < it1; ++it) {
+        const int ent = a.tile_entries[it];
+        const int e = ent >> 8;
+        const ElemRec<float> &rec = a.recs[e];
+        const float ax = rec.p3[0] - xp.x, ay = rec.p3[1] - xp.y, az = rec.p3[2] - xp.z;
+        const float dn = rec.n[0] * ax + rec.n[1] * ay + rec.n[2] * az;
+        const float e1x = rec.e1[0], e1y = rec.e1[1], e1z = rec.e1[2];
+        const float fx = rec.e2[0] - e1x, fy = rec.e2[1] - e1y, fz = rec.e2[2] - e1z;
+        float h1r = 0, h1i = 0, h2r = 0, h2i = 0, htr = 0, hti = 0, gr = 0, gi = 0, cc = 0;
+        float g1r = 0, g1i = 0, g2r = 0, g2i = 0;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const float th = gl_theta(n);
+            const float vx = __builtin_fmaf(th, fx, e1x), vy = __builtin_fmaf(th, fy, e1y),
+                        vz = __builtin_fmaf(th, fz, e1z);
+            float Anr = 0, Ani = 0, Bnr = 0, Bni = 0, Cnr = 0, Cni = 0;
+#pragma unroll
+            for (int m = 0; m < 6; m++) {
+                const float rho = gl_rho(m), W = gl_W(n, m), Wr = gl_Wrho(n, m);
+                const float dx = __builtin_fmaf(rho, vx, ax);
+                const float dy = __builtin_fmaf(rho, vy, ay);
+                const float dz = __builtin_fmaf(rho, vz, az);
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                const float t2 = __builtin_fmaf(r, kpi, MAGIC);
+                const float jf = t2 - MAGIC;
+                const float u = __builtin_fmaf(r, kpi, -jf);
+                float sn, cs;
+                sincos_pi_half(u, sn, cs);
+                const unsigned sgn = __float_as_uint(t2) << 31;
+                const float rinv2 = rinv * rinv;
+                const float rs = __uint_as_float(__float_as_uint(rinv) ^ sgn);
+                const float rs2 = __uint_as_float(__float_as_uint(rinv2) ^ sgn);
+                // 4 pi G = (cos kr - i sin kr) / r ; G' = G / r
+                const float Gr = cs * rs, Gi = -sn * rs;
+                const float Pr = cs * rs2, Pi = -sn * rs2;
+                // 4 pi r dG/dn / (-dn) = G'(1/r + ik)
+                const float Hr = __builtin_fmaf(Pr, rinv, -k * Pi);
+                const float Hi = __builtin_fmaf(Pi, rinv, k * Pr);
+                Anr = __builtin_fmaf(Wr, Hr, Anr);
+                Ani = __builtin_fmaf(Wr, Hi, Ani);
+                Bnr = __builtin_fmaf(W, Hr, Bnr);
+                Bni = __builtin_fmaf(W, Hi, Bni);
+                gr = __builtin_fmaf(W, Gr, gr);
+                gi = __builtin_fmaf(W, Gi, gi);
+                if (IMP) {
+                    Cnr = __builtin_fmaf(Wr, Gr, Cnr);
+                    Cni = __builtin_fmaf(Wr, Gi, Cni);
+                }
+                cc = __builtin_fmaf(W, rinv2 * rinv, cc);
+            }
+            const float om = 1.0f - th;
+            h1r = __builtin_fmaf(om, Anr, h1r);
+            h1i = __builtin_fmaf(om, Ani, h1i);
+            h2r = __builtin_fmaf(th, Anr, h2r);
+            h2i = __builtin_fmaf(th, Ani, h2i);
+            htr += Bnr;
+            hti += Bni;
+            if (IMP) {
+                g1r = __builtin_fmaf(om, Cnr, g1r);
+                g1i = __builtin_fmaf(om, Cni, g1i);
+                g2r = __builtin_fmaf(th, Cnr, g2r);
+                g2i = __builtin_fmaf(th, Cni, g2i);
+            }
+        }
+        const float sJ = rec.J * inv4pi;
+        const float sh = -dn * sJ;
+        float p0r = sh * h1r, p0i = sh * h1i;
+        float p1r = sh * h2r, p1i = sh * h2i;
+        float p2r = sh * (htr - h1r - h2r), p2i = sh * (hti - h1i - h2i);
+        const float sg = a.rw * sJ;
+        const float gsr = -sg * gi, gsi = sg * gr;
+        if (IMP) {
+            const float br = rec.beta[0], bi = rec.beta[1];
+            const float G0r = -sg * g1i, G0i = sg * g1r;
+            const float G1r = -sg * g2i, G1i = sg * g2r;
+            const float G2r = gsr - G0r - G1r, G2i = gsi - G0i - G1i;
+            p0r -= br * G0r - bi * G0i; p0i -= br * G0i + bi * G0r;
+            p1r -= br * G1r - bi * G1i; p1i -= br * G1i + bi * G1r;
+            p2r -= br * G2r - bi * G2i; p2i -= br * G2i + bi * G2r;
+        }
+        float cval = -dn * sJ * cc;
+        float btr = rec.gamma[0] * gsr - rec.gamma[1] * gsi;
+        float bti = rec.gamma[0] * gsi + rec.gamma[1] * gsr;
+        const int sv = (row == rec.nodes[0]) ? 0 : (row == rec.nodes[1]) ? 1 : (row == rec.nodes[2]) ? 2 : -1;
+        if (sv >= 0) {
+            const float *sg2 = a.sing + ((size_t)e * 3 + sv) * 16;
+            p0r = sg2[0]; p0i = sg2[1]; p1r = sg2[2]; p1i = sg2[3]; p2r = sg2[4]; p2i = sg2[5];
+            cval = sg2[6];
+            btr = sg2[7];
+            bti = sg2[8];
+        }
+        if (active) {
+            const float pr[3] = {p0r, p1r, p2r}, pi[3] = {p0i, p1i, p2i};
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                if ((ent >> j) & 1) {
+                    float2 *dst = a.A + (size_t)rec.nodes[j] * a.ldA + row;
+                    float2 v;
+                    if ((ent >> (3 + j)) & 1) {
+                        v.x = pr[j];
+                        v.y = pi[j];
+                    } else {
+                        v = *dst;
+                        v.x += pr[j];
+                        v.y += pi[j];
+                    }
+                    *dst = v;
+                }
+            }
+        }
+        if ((ent >> 6) & 1) {
+            c_acc += cval;
+            br_acc += btr;
+            bi_acc += bti;
+        }
+    }
+    if (active) {
+        a.cpart[(size_t)tile * a.Mp + row] = c_acc;
+        a.bpart[(size_t)tile * a.Mp + row] = make_float2(br_acc, bi_acc);
+    }
+}
+
 // Columns of nodes no element references: only the diagonal survives.
 template <typename C2>
 __global__ void zero_columns(C2 *A, long ldA, int M, const int *__restrict__ cols, int ncols) {
@@ -819,6 +1002,9 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, hipStream_t s) {
         if (dp.qg == 4) {
             if (dp.impedance) asm_tile<R, 4, true><<<grid, block, 0, s>>>(a);
             else asm_tile<R, 4, false><<<grid, block, 0, s>>>(a);
+        } else if (sizeof(R) == 4 && !getenv("BEM_GENERIC_TILE")) {
+            if (dp.impedance) asm_tile_fast<true><<<grid, block, 0, s>>>(*(AsmArgs<float> *)&a);
+            else asm_tile_fast<false><<<grid, block, 0, s>>>(*(AsmArgs<float> *)&a);
         } else {
             if (dp.impedance) asm_tile<R, 1, true><<<grid, block, 0, s>>>(a);
             else asm_tile<R, 1, false><<<grid, block, 0, s>>>(a);

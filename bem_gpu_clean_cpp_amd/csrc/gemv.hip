@@ -44,15 +44,15 @@ __global__ __launch_bounds__(256) void gemv_n_part(const typename CT<TA>::c2 *__
     if (i0 + 1 < M) {
         const A2 *col = A + (size_t)j0 * lda + i0;
         int j = j0;
-        for (; j + 4 <= j1; j += 4) {
-            A2 a[4][2];
+        for (; j + 8 <= j1; j += 8) {
+            A2 a[8][2];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < 8; u++) {
                 a[u][0] = col[(size_t)u * lda];
                 a[u][1] = col[(size_t)u * lda + 1];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < 8; u++) {
                 const typename CT<TX>::c2 xv = x[j + u];
                 const TC xr = (TC)xv.x, xi = (TC)xv.y;
                 ar0 = fma((TC)a[u][0].x, xr, ar0); ar0 = fma(-(TC)a[u][0].y, xi, ar0);
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void gemv_n_part(const typename CT<TA>::c2 *__
                 ar1 = fma((TC)a[u][1].x, xr, ar1); ar1 = fma(-(TC)a[u][1].y, xi, ar1);
                 ai1 = fma((TC)a[u][1].x, xi, ai1); ai1 = fma((TC)a[u][1].y, xr, ai1);
             }
-            col += 4 * (size_t)lda;
+            col += 8 * (size_t)lda;
         }
         for (; j < j1; j++) {
             const A2 a0 = col[0], a1 = col[1];
@@ -92,29 +92,48 @@ __global__ __launch_bounds__(256) void gemv_n_part(const typename CT<TA>::c2 *__
     }
 }
 
-// y = alpha * sum_chunks part (+ beta * b): alpha/beta are +-1 here, used for
-// y = A x (alpha 1, no b) and r = b - A x (alpha -1, b).
+// y = alpha * sum_chunks part (+ b): alpha is +-1 here, used for y = A x
+// (alpha 1, no b) and r = b - A x (alpha -1, b).  A workgroup handles 64 rows;
+// its 4 waves sum interleaved quarters of the chunks, then combine in LDS in a
+// fixed order (deterministic).
 template <typename TC, typename TY>
-__global__ void gemv_n_reduce(const typename CT<TC>::c2 *__restrict__ part, long ldp, int nchunks, int M,
-                              const typename CT<TY>::c2 *__restrict__ b, typename CT<TY>::c2 *__restrict__ y,
-                              TC alpha) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
+__global__ __launch_bounds__(256) void gemv_n_reduce(const typename CT<TC>::c2 *__restrict__ part, long ldp,
+                                                     int nchunks, int M, const typename CT<TY>::c2 *__restrict__ b,
+                                                     typename CT<TY>::c2 *__restrict__ y, TC alpha) {
+    __shared__ TC red[2][4][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     TC sr = 0, si = 0;
-    for (int c = 0; c < nchunks; c++) {
-        const typename CT<TC>::c2 p = part[(size_t)c * ldp + i];
-        sr += p.x;
-        si += p.y;
+    if (i < M) {
+        int c = wid;
+        for (; c + 12 < nchunks; c += 16) {
+            const typename CT<TC>::c2 p0 = part[(size_t)c * ldp + i], p1 = part[(size_t)(c + 4) * ldp + i],
+                                      p2 = part[(size_t)(c + 8) * ldp + i], p3 = part[(size_t)(c + 12) * ldp + i];
+            sr += (p0.x + p1.x) + (p2.x + p3.x);
+            si += (p0.y + p1.y) + (p2.y + p3.y);
+        }
+        for (; c < nchunks; c += 4) {
+            const typename CT<TC>::c2 p = part[(size_t)c * ldp + i];
+            sr += p.x;
+            si += p.y;
+        }
     }
-    typename CT<TY>::c2 o;
-    if (b) {
-        o.x = (TY)((TC)b[i].x + alpha * sr);
-        o.y = (TY)((TC)b[i].y + alpha * si);
-    } else {
-        o.x = (TY)(alpha * sr);
-        o.y = (TY)(alpha * si);
+    red[0][wid][lane] = sr;
+    red[1][wid][lane] = si;
+    __syncthreads();
+    if (wid == 0 && i < M) {
+        sr = (red[0][0][lane] + red[0][1][lane]) + (red[0][2][lane] + red[0][3][lane]);
+        si = (red[1][0][lane] + red[1][1][lane]) + (red[1][2][lane] + red[1][3][lane]);
+        typename CT<TY>::c2 o;
+        if (b) {
+            o.x = (TY)((TC)b[i].x + alpha * sr);
+            o.y = (TY)((TC)b[i].y + alpha * si);
+        } else {
+            o.x = (TY)(alpha * sr);
+            o.y = (TY)(alpha * si);
+        }
+        y[i] = o;
     }
-    y[i] = o;
 }
 
 static void gemv_n_shape(long M, long N, int &nchunks, int &kc) {
@@ -147,7 +166,7 @@ static int gemv_n_impl(const typename CT<TA>::c2 *A, long lda, long M, long N, c
         gemv_n_part<TA, TX, TC><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, x + (size_t)r * ldx, kc, part, ldp);
         HIP_CALL(hipGetLastError());
         timing_end(tr, s);
-        gemv_n_reduce<TC, TY><<<(unsigned)((M + 255) / 256), 256, 0, s>>>(
+        gemv_n_reduce<TC, TY><<<(unsigned)((M + 63) / 64), 256, 0, s>>>(
             part, ldp, nch, (int)M, b ? b + (size_t)r * ldb : nullptr, y + (size_t)r * ldy, b ? (TC)-1 : (TC)1);
         HIP_CALL(hipGetLastError());
     }
@@ -198,6 +217,25 @@ __global__ __launch_bounds__(256) void gemv_h_kernel(const typename CT<TA>::c2 *
     const A2 *col = A + (size_t)j * lda;
     TC sr = 0, si = 0;
     int i = lane * 2;
+    for (; i + 384 + 1 < M; i += 512) {
+        A2 a[4][2];
+        typename CT<TX>::c2 xv[4][2];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            a[u][0] = col[i + 128 * u];
+            a[u][1] = col[i + 128 * u + 1];
+            xv[u][0] = x[i + 128 * u];
+            xv[u][1] = x[i + 128 * u + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                sr = fma((TC)a[u][q].x, (TC)xv[u][q].x, sr); sr = fma((TC)a[u][q].y, (TC)xv[u][q].y, sr);
+                si = fma((TC)a[u][q].x, (TC)xv[u][q].y, si); si = fma(-(TC)a[u][q].y, (TC)xv[u][q].x, si);
+            }
+        }
+    }
     for (; i + 1 < M; i += 128) {
         const A2 a0 = col[i], a1 = col[i + 1];
         const typename CT<TX>::c2 x0 = x[i], x1 = x[i + 1];

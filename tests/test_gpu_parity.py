@@ -49,10 +49,14 @@ def _icosphere(s, r=0.1, nchief=0, seed=101):
     return m
 
 
-@pytest.mark.parametrize("qg", ["1", "4"])
+@pytest.mark.parametrize("qg", ["1", "1g", "4"])
 @pytest.mark.parametrize("k", [3.13, 36.6])
 def test_assembly_fp32_matches_oracle(qg, k, monkeypatch):
-    monkeypatch.setenv("BEM_QG", qg)
+    """QG=1 runs the separable fast kernel, "1g" the generic one-lane-per-row
+    kernel, QG=4 the kernel that splits the 36 points over 4 lanes per row."""
+    monkeypatch.setenv("BEM_QG", qg[0])
+    if qg == "1g":
+        monkeypatch.setenv("BEM_GENERIC_TILE", "1")
     m = _icosphere(2, nchief=4)
     print("case: icosphere s2 + 4 CHIEF, k=%g, QG=%s" % (k, qg), flush=True)
     A, B = bem.bem_system(m, k)
@@ -100,8 +104,10 @@ def test_assembly_reference_mesh_and_bc():
         bem.set_medium(RHO, SPEED)
 
 
-def test_assembly_impedance_bc():
+@pytest.mark.parametrize("qg", ["1", "4"])
+def test_assembly_impedance_bc(qg, monkeypatch):
     """ca != 0 exercises the h - (ca/cb) g path of the kernels."""
+    monkeypatch.setenv("BEM_QG", qg)
     m = _icosphere(2, nchief=2)
     m.set_bc(0.5 + 0.25j, 1.0 - 0.5j, 0.3 + 0.1j)
     k = 12.0
@@ -113,7 +119,9 @@ def test_assembly_impedance_bc():
     assert _rel_max(B, Bo) < 2e-5
 
 
-def test_assembly_bitwise_repeatable():
+@pytest.mark.parametrize("qg", ["1", "4"])
+def test_assembly_bitwise_repeatable(qg, monkeypatch):
+    monkeypatch.setenv("BEM_QG", qg)
     m = _icosphere(3, nchief=8)
     A1, _ = bem.bem_system(m, 30.0)
     A2, _ = bem.bem_system(m, 30.0)
