@@ -49,11 +49,39 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (scale for FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A plain copy kernel of known size, to scale the FETCH_SIZE / WRITE_SIZE
+    hardware counters of the assembly kernel."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4}), flush=True)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel, config):
+    """HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE counters (scaled
+    by a copy kernel of known size) and recorded under profiles/; None if absent."""
+    path = os.path.join(ROOT, "profiles", "%s_s5_hbm_traffic.txt" % config)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == config:
+            return float(f[3])
+    return None
 
 
 def cpu_baseline(mesh, k, rho, c, target_s, threads):
@@ -91,6 +119,9 @@ def cpu_baseline(mesh, k, rho, c, target_s, threads):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (--gpus %d requested); the path shards by "
               "collocation rows or by request, see the design notes" % args.gpus, file=sys.stderr)
@@ -119,6 +150,7 @@ def main():
         solver.assemble(k, bem.PREC_FP32)
         return solver.solve(method, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-8, max_refine=50)
 
+    st = None
     for _ in range(args.warmup):
         st = step()
     log("warmup stats", st)
@@ -214,11 +246,12 @@ def main():
         "speedup_vs_fp64_gpu": (fp64_ms / ms_per_step) if fp64_ms else None,
         "roofline": {"bound": "valu", "kernel": "asm_tile_f32", "achieved": round(achieved_tflops, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "avg_ms": round(asm_avg_ms, 4),
+                     "traffic": measured_traffic("asm_tile_f32", args.config), "avg_ms": round(asm_avg_ms, 4),
                      "flops_per_launch": FLOP_PER_EVAL * evals},
         "roofline_gemv": {"bound": "hbm", "kernel": "gemv_n_f32", "achieved": gemv_gbs and round(gemv_gbs, 1),
                           "achieved_h": gemvh_gbs and round(gemvh_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                           "frac": gemv_gbs and round(gemv_gbs / PEAK_HBM_GBS, 4), "bytes_per_launch": gemv_bytes,
+                          "traffic": measured_traffic("gemv_n_f32", args.config),
                           "launches": gn_n},
         "cpu_baseline": cpu and {kk: cpu[kk] for kk in ("value", "unit", "cores", "kind", "sample")},
     }

@@ -43,7 +43,7 @@ def sphere_points(seed, n, radius):
 
 
 def read_golden(name):
-    s = open(os.path.join(GOLDEN, name)).read()
+    s = open(name if os.path.isabs(name) else os.path.join(GOLDEN, name)).read()
     return np.array([complex(float(a), float(b)) for a, b in re.findall(r"\(([-\d.eE+]+),([-\d.eE+]+)\)", s)])
 
 
