@@ -776,9 +776,9 @@ int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2
     const int nch = (dp.E + kFieldChunk - 1) / kFieldChunk;
     float2 *part = nullptr;
     float *params = nullptr;
-    HOST_CALL(dalloc(&part, (size_t)nch * R * P));
+    HOST_CALL(salloc(&part, (size_t)nch * R * P, s));
     if (!rhs.params.empty()) {
-        HOST_CALL(dalloc(&params, rhs.params.size()));
+        HOST_CALL(salloc(&params, rhs.params.size(), s));
         HIP_CALL(hipMemcpyAsync(params, rhs.params.data(), rhs.params.size() * sizeof(float), hipMemcpyHostToDevice, s));
     }
     FieldArgs a;
@@ -812,8 +812,8 @@ int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2
                                       rhs.nrhs >= R ? params : nullptr, out, ldo);
     hipError_t e = hipGetLastError();
     (void)hipStreamSynchronize(s);
-    dfree(part);
-    dfree(params);
+    sfree(part, s);
+    sfree(params, s);
     HIP_CALL(e);
     return EXIT_SUCCESS;
 }
@@ -1038,13 +1038,13 @@ static int rhs_impl(DeviceProblem &dp, double k, C2 *B, long ldB, const RhsSpec 
     if (rhs.nrhs <= 0) return EXIT_SUCCESS;
     if (rhs.nrhs > kMaxRhs) BEM_FAIL("at most %d right-hand sides, got %d", kMaxRhs, rhs.nrhs);
     float *params = nullptr;
-    HOST_CALL(dalloc(&params, 3 * (size_t)rhs.nrhs));
+    HOST_CALL(salloc(&params, 3 * (size_t)rhs.nrhs, s));
     HIP_CALL(hipMemcpyAsync(params, rhs.params.data(), 3 * sizeof(float) * rhs.nrhs, hipMemcpyHostToDevice, s));
     dim3 grid((dp.M + 255) / 256, rhs.nrhs);
     rhs_fill<C2><<<grid, 256, 0, s>>>(dp.pts64, dp.M, k, rhs.kind, (double)rhs.amp, params, rhs.nrhs, dp.bterm, B, ldB);
     hipError_t e = hipGetLastError();
     (void)hipStreamSynchronize(s);
-    dfree(params);
+    sfree(params, s);
     HIP_CALL(e);
     return EXIT_SUCCESS;
 }

@@ -225,5 +225,18 @@ void dfree(T *&p) {
     if (p) (void)hipFree((void *)p);
     p = nullptr;
 }
+// Stream-ordered temporaries: hipFree synchronises the whole device, which would
+// serialise solvers running on different streams.
+template <typename T>
+int salloc(T **p, size_t n, hipStream_t s) {
+    if (n == 0) n = 1;
+    HIP_CALL(hipMallocAsync((void **)p, n * sizeof(T), s));
+    return EXIT_SUCCESS;
+}
+template <typename T>
+void sfree(T *&p, hipStream_t s) {
+    if (p) (void)hipFreeAsync((void *)p, s);
+    p = nullptr;
+}
 
 }  // namespace bem

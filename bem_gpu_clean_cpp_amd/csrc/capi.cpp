@@ -27,6 +27,8 @@ struct bem_solver {
     float2 *X32 = nullptr;
     int xr = 0;  // columns held in X64
     bem_solve_stats stats{};
+    hipStream_t stream = nullptr;  // each solver runs on its own stream
+    mutable std::mutex mu;         // one call at a time per solver
 };
 
 using namespace bem;
@@ -394,12 +396,17 @@ bem_solver *bem_solver_create(const bem_mesh *m) {
         set_error(__FILE__, __LINE__, "null mesh");
         return nullptr;
     }
-    std::lock_guard<std::mutex> lk(g_mu);
-    hipStream_t s;
-    if (stream(&s) != EXIT_SUCCESS) return nullptr;
+    Context *c = nullptr;
+    if (ctx_init(&c) != EXIT_SUCCESS) return nullptr;  // device selection
     auto *sv = new bem_solver();
     sv->mesh = m->m;
-    if (device_problem_build(sv->mesh, sv->dp, s) != EXIT_SUCCESS) {
+    if (hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error(__FILE__, __LINE__, "stream creation failed");
+        delete sv;
+        return nullptr;
+    }
+    if (device_problem_build(sv->mesh, sv->dp, sv->stream) != EXIT_SUCCESS) {
+        (void)hipStreamDestroy(sv->stream);
         delete sv;
         return nullptr;
     }
@@ -416,19 +423,21 @@ static void solver_free_mats(bem_solver *s) {
 
 void bem_solver_destroy(bem_solver *s) {
     if (!s) return;
-    std::lock_guard<std::mutex> lk(g_mu);
-    (void)hipDeviceSynchronize();
-    solver_free_mats(s);
-    device_problem_free(s->dp);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        (void)hipStreamSynchronize(s->stream);
+        solver_free_mats(s);
+        device_problem_free(s->dp);
+        (void)hipStreamDestroy(s->stream);
+    }
     delete s;
 }
 
 int bem_solver_assemble(bem_solver *sv, float k, int precision) {
     if (!sv) BEM_FAIL("null solver");
     if (precision != BEM_PREC_FP32 && precision != BEM_PREC_FP64) BEM_FAIL("precision must be 32 or 64");
-    std::lock_guard<std::mutex> lk(g_mu);
-    hipStream_t s;
-    HOST_CALL(stream(&s));
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
     DeviceProblem &dp = sv->dp;
     hipEvent_t e0, e1;
     HIP_CALL(hipEventCreate(&e0));
@@ -472,9 +481,8 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
     if (!sv->prec) BEM_FAIL("solver not assembled");
     if (sv->rhs.nrhs <= 0) BEM_FAIL("no right-hand sides set");
     if (method != BEM_SOLVE_QR && method != BEM_SOLVE_GMRES) BEM_FAIL("unknown method %d", method);
-    std::lock_guard<std::mutex> lk(g_mu);
-    hipStream_t s;
-    HOST_CALL(stream(&s));
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
     DeviceProblem &dp = sv->dp;
     const int R = sv->rhs.nrhs;
     const long ld = dp.ldA;
@@ -526,9 +534,8 @@ int bem_solver_get_solution_z(const bem_solver *sv, bem_zcomplex *x, int ldx) {
     if (!sv || !x) BEM_FAIL("null argument");
     if (sv->xr <= 0) BEM_FAIL("no solution");
     if (ldx < sv->dp.N) BEM_FAIL("ldx must be >= N");
-    std::lock_guard<std::mutex> lk(g_mu);
-    hipStream_t s;
-    HOST_CALL(stream(&s));
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
     HIP_CALL(hipMemcpy2DAsync(x, sizeof(bem_zcomplex) * ldx, sv->X64, sizeof(double2) * sv->dp.ldA,
                               sizeof(double2) * sv->dp.N, sv->rhs.nrhs, hipMemcpyDeviceToHost, s));
     HIP_CALL(hipStreamSynchronize(s));
@@ -559,9 +566,8 @@ int bem_solver_get_system(const bem_solver *sv, bem_complex *A, int lda, bem_com
     const DeviceProblem &dp = sv->dp;
     if (A && lda < dp.M) BEM_FAIL("lda must be >= M");
     if (B && ldb < dp.M) BEM_FAIL("ldb must be >= M");
-    std::lock_guard<std::mutex> lk(g_mu);
-    hipStream_t s;
-    HOST_CALL(stream(&s));
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
     if (A) {
         const float2 *src = sv->A32;
         float2 *tmp = nullptr;
@@ -596,9 +602,8 @@ int bem_solver_field(bem_solver *sv, const bem_cart_coord *pts, int P, bem_compl
     if (!sv || !pts || !out || P <= 0) BEM_FAIL("bad field arguments");
     if (sv->xr <= 0) BEM_FAIL("no solution");
     if (ldo < P) BEM_FAIL("ldo must be >= P");
-    std::lock_guard<std::mutex> lk(g_mu);
-    hipStream_t s;
-    HOST_CALL(stream(&s));
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
     DeviceProblem &dp = sv->dp;
     const int R = sv->rhs.nrhs;
     const long n = dp.ldA * (long)R;
@@ -646,7 +651,7 @@ int bem_timing_get(const char *kernel, double *total_ms, long *count) {
 int bem_sync(void) {
     hipStream_t s;
     HOST_CALL(stream(&s));
-    HIP_CALL(hipStreamSynchronize(s));
+    HIP_CALL(hipDeviceSynchronize());  // every solver stream and the library stream
     return EXIT_SUCCESS;
 }
 

@@ -152,30 +152,32 @@ struct KrylovWS {
     double2 *r = nullptr;
     double2 *h = nullptr;   // 2 * (restart + 2)
     double2 *part = nullptr;
-    void *gws = nullptr;
+    char *gws = nullptr;
     size_t gws_bytes = 0;
     int nblk = 0;
+    hipStream_t st = nullptr;
     ~KrylovWS() {
-        dfree(V); dfree(w64); dfree(t64); dfree(v32); dfree(w32); dfree(t32); dfree(r); dfree(h); dfree(part);
-        if (gws) (void)hipFree(gws);
+        sfree(V, st); sfree(w64, st); sfree(t64, st); sfree(v32, st); sfree(w32, st); sfree(t32, st);
+        sfree(r, st); sfree(h, st); sfree(part, st); sfree(gws, st);
     }
-    int init(int n_, int m_, int restart_) {
+    int init(int n_, int m_, int restart_, hipStream_t s) {
+        st = s;
         n = n_;
         m = m_;
         restart = restart_;
         ldv = ((long)std::max(n, m) + 63) / 64 * 64;
         nblk = (int)((std::max(n, m) + 1023) / 1024);
-        HOST_CALL(dalloc(&V, (size_t)ldv * (restart + 1)));
-        HOST_CALL(dalloc(&w64, ldv));
-        HOST_CALL(dalloc(&t64, ldv));
-        HOST_CALL(dalloc(&v32, ldv));
-        HOST_CALL(dalloc(&w32, ldv));
-        HOST_CALL(dalloc(&t32, ldv));
-        HOST_CALL(dalloc(&r, ldv));
-        HOST_CALL(dalloc(&h, 2 * (size_t)(restart + 2)));
-        HOST_CALL(dalloc(&part, (size_t)nblk * (restart + 2)));
+        HOST_CALL(salloc(&V, (size_t)ldv * (restart + 1), s));
+        HOST_CALL(salloc(&w64, ldv, s));
+        HOST_CALL(salloc(&t64, ldv, s));
+        HOST_CALL(salloc(&v32, ldv, s));
+        HOST_CALL(salloc(&w32, ldv, s));
+        HOST_CALL(salloc(&t32, ldv, s));
+        HOST_CALL(salloc(&r, ldv, s));
+        HOST_CALL(salloc(&h, 2 * (size_t)(restart + 2), s));
+        HOST_CALL(salloc(&part, (size_t)nblk * (restart + 2), s));
         gws_bytes = ws_gemv_n(m, n, 1);
-        HIP_CALL(hipMalloc(&gws, gws_bytes));
+        HOST_CALL(salloc(&gws, gws_bytes, s));
         return EXIT_SUCCESS;
     }
 };
@@ -380,7 +382,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     if (M < N) BEM_FAIL("GMRES needs M >= N (got %d x %d)", M, N);
     restart = std::max(1, std::min(restart, N));
     KrylovWS ws;
-    HOST_CALL(ws.init(N, M, restart));
+    HOST_CALL(ws.init(N, M, restart, s));
     Operator op;
     op.A32 = A32;
     op.A64 = A64;

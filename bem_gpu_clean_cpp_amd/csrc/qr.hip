@@ -308,9 +308,10 @@ struct QRFactor {
     C2 *Tb = nullptr;  // per-panel T factors
     C2 *W = nullptr;   // workspace nb x max(ncols)
     int wcols = 0;
+    hipStream_t st = nullptr;
     ~QRFactor() {
-        dfree(Tb);
-        dfree(W);
+        sfree(Tb, st);
+        sfree(W, st);
     }
 };
 
@@ -335,9 +336,10 @@ template <typename T>
 static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
     using C2 = typename QT<T>::c2;
     const int np = (f.N + kNB - 1) / kNB;
-    HOST_CALL(dalloc(&f.Tb, (size_t)np * kNB * kNB));
+    f.st = s;
+    HOST_CALL(salloc(&f.Tb, (size_t)np * kNB * kNB, s));
     f.wcols = std::max(f.N, extra_cols);
-    HOST_CALL(dalloc(&f.W, (size_t)kNB * f.wcols));
+    HOST_CALL(salloc(&f.W, (size_t)kNB * f.wcols, s));
     for (int p = 0; p < np; p++) {
         const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
         TimedRegion tr;
@@ -474,7 +476,7 @@ int qr_refined_f32(const float2 *A32, long lda, int M, int N, const double2 *B64
                    int R, double refine_tol, int max_refine, bem_solve_stats *st, hipStream_t s) {
     if (M < N || N <= 0) BEM_FAIL("least squares needs M >= N > 0 (got %d x %d)", M, N);
     QRFactor<float> f;
-    HOST_CALL(dalloc(&f.A, (size_t)lda * N));
+    HOST_CALL(salloc(&f.A, (size_t)lda * N, s));
     HIP_CALL(hipMemcpyAsync(f.A, A32, sizeof(float2) * lda * N, hipMemcpyDeviceToDevice, s));
     f.lda = lda;
     f.M = M;
@@ -482,15 +484,15 @@ int qr_refined_f32(const float2 *A32, long lda, int M, int N, const double2 *B64
     float2 *rb = nullptr;
     double2 *r64 = nullptr;
     double *dn = nullptr;
-    void *gws = nullptr;
+    char *gws = nullptr;
     int refinements = 0;
     double worst = 0;
     bool conv = true;
     int rc = EXIT_FAILURE;
     const int bs = 256;
     if (qr_factor<float>(f, 1, s) != EXIT_SUCCESS) goto out;
-    if (dalloc(&rb, (size_t)lda) || dalloc(&r64, (size_t)lda) || dalloc(&dn, 2)) goto out;
-    if (hipMalloc(&gws, ws_gemv_n(M, N, 1)) != hipSuccess) goto out;
+    if (salloc(&rb, (size_t)lda, s) || salloc(&r64, (size_t)lda, s) || salloc(&dn, 2, s)) goto out;
+    if (salloc(&gws, ws_gemv_n(M, N, 1), s)) goto out;
     for (int r = 0; r < R; r++) {
         const double2 *b = B64 + (size_t)r * ldb;
         double2 *x = X64 + (size_t)r * ldx;
@@ -522,11 +524,11 @@ out:
         st->converged = conv ? 1 : 0;
         st->residual = worst;
     }
-    dfree(f.A);
-    dfree(rb);
-    dfree(r64);
-    dfree(dn);
-    if (gws) (void)hipFree(gws);
+    sfree(f.A, s);
+    sfree(rb, s);
+    sfree(r64, s);
+    sfree(dn, s);
+    sfree(gws, s);
     (void)hipStreamSynchronize(s);
     if (rc != EXIT_SUCCESS) BEM_FAIL("refined QR least squares failed");
     return rc;
