@@ -187,6 +187,22 @@ int gemv_n_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, long ldx, double2 *y, long ldy, int R,
                hipStream_t s);
 
+// multi-right-hand-side products (gemm.hip)
+size_t ws_gemm(long M, long N, int R);
+int gemm_n_f32(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, float2 *Y, long ldy, int R,
+               void *ws, hipStream_t s);
+int gemm_h_f32(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R,
+               void *ws, hipStream_t s);
+size_t ws_mres(long M, long N, int R);
+int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
+              double2 *out, long ldo, int R, void *ws, hipStream_t s);
+int mres_f64(const double2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
+             double2 *out, long ldo, int R, void *ws, hipStream_t s);
+int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
+            hipStream_t s);
+int mh_f64(const double2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
+           hipStream_t s);
+
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, int R, int restart, double tol, int maxit, double outer_tol, int max_cycles,
                  bem_solve_stats *st, hipStream_t s);

@@ -14,6 +14,12 @@
 // A^H A x = A^H b, i.e. BA-GMRES with B = A^H, Hayami, Yin & Ito 2010).  With an
 // FP32 matrix this is iterative refinement: the Krylov correction runs with FP32
 // operator applications, the residual and the accumulated solution are FP64.
+//
+// Many right-hand sides (plane waves, sources) run in lockstep: every system
+// keeps its own basis, Hessenberg matrix and rotations, while the operator is
+// applied to all current basis vectors at once, so A is streamed from HBM once
+// per Arnoldi step for the whole batch (the MFMA products of gemm.hip) instead
+// of once per system.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,20 +39,16 @@ int resid_n_f64(const double2 *A, long lda, long M, long N, const double2 *x, lo
 using zc = std::complex<double>;
 
 // ---------------------------------------------------------------- kernels ---
-__global__ void k_cvt_c2z(const float2 *__restrict__ in, double2 *__restrict__ out, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = make_double2(in[i].x, in[i].y);
-}
-
-__global__ void k_copy_z(const double2 *__restrict__ in, double2 *__restrict__ out, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[i];
-}
-
-// part[blk * nv + l] = sum_{i in blk} conj(V[i, l]) w[i], l < nv; 1024 rows per block
-__global__ __launch_bounds__(256) void k_mdot_part(const double2 *__restrict__ V, long ldv, int nv,
-                                                   const double2 *__restrict__ w, int n, double2 *__restrict__ part) {
-    __shared__ double red[2][8];
+// CGS2 steps for R systems at once: system r = blockIdx.y, its vectors at
+// fixed strides (basis vstride, vectors wstride, Hessenberg columns hstride).
+__global__ __launch_bounds__(256) void kb_mdot_part(const double2 *__restrict__ V, long ldv, long vstride, int nv,
+                                                    const double2 *__restrict__ w, long wstride, int n,
+                                                    double2 *__restrict__ part, long pstride) {
+    const int r = blockIdx.y;
+    V += (size_t)r * vstride;
+    w += (size_t)r * wstride;
+    part += (size_t)r * pstride;
+    __shared__ double red[2][4];
     const int base = blockIdx.x * 1024;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     double2 wv[4];
@@ -76,36 +78,35 @@ __global__ __launch_bounds__(256) void k_mdot_part(const double2 *__restrict__ V
             red[1][wid] = si;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double tr = 0, ti = 0;
-            for (int q = 0; q < 4; q++) {
-                tr += red[0][q];
-                ti += red[1][q];
-            }
-            part[(size_t)blockIdx.x * nv + l] = make_double2(tr, ti);
-        }
+        if (threadIdx.x == 0)
+            part[(size_t)blockIdx.x * nv + l] = make_double2((red[0][0] + red[0][1]) + (red[0][2] + red[0][3]),
+                                                             (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
         __syncthreads();
     }
 }
 
-// out[l] = sum_blk part[blk * nv + l] (fixed order)
-__global__ void k_mdot_final(const double2 *__restrict__ part, int nblk, int nv, double2 *__restrict__ out) {
-    const int l = threadIdx.x;
+__global__ void kb_mdot_final(const double2 *__restrict__ part, long pstride, int nblk, int nv,
+                              double2 *__restrict__ out, long ostride) {
+    const int r = blockIdx.y, l = threadIdx.x;
     if (l >= nv) return;
+    part += (size_t)r * pstride;
     double sr = 0, si = 0;
     for (int b = 0; b < nblk; b++) {
         const double2 p = part[(size_t)b * nv + l];
         sr += p.x;
         si += p.y;
     }
-    out[l] = make_double2(sr, si);
+    out[(size_t)r * ostride + l] = make_double2(sr, si);
 }
 
-// w[i] += sign * sum_l V[i, l] h[l]
-__global__ void k_maxpy(const double2 *__restrict__ V, long ldv, int nv, const double2 *__restrict__ h, double sign,
-                        double2 *__restrict__ w, int n) {
+__global__ void kb_maxpy(const double2 *__restrict__ V, long ldv, long vstride, int nv, const double2 *__restrict__ h,
+                         long hstride, double sign, double2 *__restrict__ w, long wstride, int n) {
+    const int r = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    V += (size_t)r * vstride;
+    h += (size_t)r * hstride;
+    w += (size_t)r * wstride;
     double sr = 0, si = 0;
     for (int l = 0; l < nv; l++) {
         const double2 v = V[(size_t)l * ldv + i];
@@ -119,85 +120,84 @@ __global__ void k_maxpy(const double2 *__restrict__ V, long ldv, int nv, const d
     w[i] = o;
 }
 
-// accumulate h1 += h2 (nv entries)
-__global__ void k_hadd(double2 *__restrict__ h1, const double2 *__restrict__ h2, int nv) {
-    const int l = threadIdx.x;
-    if (l < nv) {
-        h1[l].x += h2[l].x;
-        h1[l].y += h2[l].y;
-    }
+__global__ void kb_hadd(double2 *__restrict__ h1, const double2 *__restrict__ h2, long hstride, int nv) {
+    const int r = blockIdx.y, l = threadIdx.x;
+    if (l >= nv) return;
+    h1[(size_t)r * hstride + l].x += h2[(size_t)r * hstride + l].x;
+    h1[(size_t)r * hstride + l].y += h2[(size_t)r * hstride + l].y;
 }
 
-// out = w / sqrt(nrm2[0].x) in FP64 and (optionally) FP32
-__global__ void k_normalize(const double2 *__restrict__ w, const double2 *__restrict__ nrm2, double2 *__restrict__ out,
-                            float2 *__restrict__ out32, int n) {
+__global__ void kb_normalize(const double2 *__restrict__ w, long wstride, const double2 *__restrict__ nrm2,
+                             long nstride, double2 *__restrict__ out, long ostride, float2 *__restrict__ out32,
+                             long o32stride, int n) {
+    const int r = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double s = nrm2[0].x > 0 ? 1.0 / sqrt(nrm2[0].x) : 0.0;
-    const double2 v = make_double2(w[i].x * s, w[i].y * s);
-    out[i] = v;
-    if (out32) out32[i] = make_float2((float)v.x, (float)v.y);
+    const double q = nrm2[(size_t)r * nstride].x;
+    const double sc = q > 0 ? 1.0 / sqrt(q) : 0.0;
+    const double2 x = w[(size_t)r * wstride + i];
+    const double2 v = make_double2(x.x * sc, x.y * sc);
+    out[(size_t)r * ostride + i] = v;
+    if (out32) out32[(size_t)r * o32stride + i] = make_float2((float)v.x, (float)v.y);
+}
+
+__global__ void kb_cvt(const float2 *__restrict__ in, long istride, double2 *__restrict__ out, long ostride, int n) {
+    const int r = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 v = in[(size_t)r * istride + i];
+        out[(size_t)r * ostride + i] = make_double2(v.x, v.y);
+    }
 }
 
 // ---------------------------------------------------------------- driver ---
 struct KrylovWS {
-    long ldv = 0;
-    int n = 0, m = 0, restart = 0;
-    double2 *V = nullptr;
-    double2 *w64 = nullptr;
-    double2 *t64 = nullptr;
-    float2 *v32 = nullptr;
-    float2 *w32 = nullptr;
-    float2 *t32 = nullptr;
-    double2 *r = nullptr;
-    double2 *h = nullptr;   // 2 * (restart + 2)
-    double2 *part = nullptr;
+    int n = 0, m = 0, R = 0, restart = 0, nblk = 0;
+    long ldv = 0, vstride = 0, hstride = 0, pstride = 0;
+    double2 *V = nullptr, *w64 = nullptr, *t64 = nullptr, *r = nullptr, *h = nullptr, *part = nullptr;
+    float2 *v32 = nullptr, *w32 = nullptr, *t32 = nullptr;
     char *gws = nullptr;
-    size_t gws_bytes = 0;
-    int nblk = 0;
     hipStream_t st = nullptr;
     ~KrylovWS() {
-        sfree(V, st); sfree(w64, st); sfree(t64, st); sfree(v32, st); sfree(w32, st); sfree(t32, st);
-        sfree(r, st); sfree(h, st); sfree(part, st); sfree(gws, st);
+        sfree(V, st); sfree(w64, st); sfree(t64, st); sfree(r, st); sfree(h, st); sfree(part, st);
+        sfree(v32, st); sfree(w32, st); sfree(t32, st); sfree(gws, st);
     }
-    int init(int n_, int m_, int restart_, hipStream_t s) {
+    int init(int n_, int m_, int R_, int restart_, hipStream_t s) {
         st = s;
         n = n_;
         m = m_;
+        R = R_;
         restart = restart_;
         ldv = ((long)std::max(n, m) + 63) / 64 * 64;
+        vstride = ldv * (restart + 1);
+        hstride = 2L * (restart + 2);
         nblk = (int)((std::max(n, m) + 1023) / 1024);
-        HOST_CALL(salloc(&V, (size_t)ldv * (restart + 1), s));
-        HOST_CALL(salloc(&w64, ldv, s));
-        HOST_CALL(salloc(&t64, ldv, s));
-        HOST_CALL(salloc(&v32, ldv, s));
-        HOST_CALL(salloc(&w32, ldv, s));
-        HOST_CALL(salloc(&t32, ldv, s));
-        HOST_CALL(salloc(&r, ldv, s));
-        HOST_CALL(salloc(&h, 2 * (size_t)(restart + 2), s));
-        HOST_CALL(salloc(&part, (size_t)nblk * (restart + 2), s));
-        gws_bytes = ws_gemv_n(m, n, 1);
-        HOST_CALL(salloc(&gws, gws_bytes, s));
+        pstride = (long)nblk * (restart + 2);
+        HOST_CALL(salloc(&V, (size_t)vstride * R, s));
+        HOST_CALL(salloc(&w64, (size_t)ldv * R, s));
+        HOST_CALL(salloc(&t64, (size_t)ldv * R, s));
+        HOST_CALL(salloc(&r, (size_t)ldv * R, s));
+        HOST_CALL(salloc(&h, (size_t)hstride * R, s));
+        HOST_CALL(salloc(&part, (size_t)pstride * R, s));
+        HOST_CALL(salloc(&v32, (size_t)ldv * R, s));
+        HOST_CALL(salloc(&w32, (size_t)ldv * R, s));
+        HOST_CALL(salloc(&t32, (size_t)ldv * R, s));
+        const size_t g = std::max(std::max(ws_gemv_n(m, n, 1), ws_gemm(m, n, R)), ws_mres(m, n, R));
+        HOST_CALL(salloc(&gws, g, s));
         return EXIT_SUCCESS;
     }
 };
 
-// dots of V[:, 0..nv-1] against w -> out (device)
-static int mdot(KrylovWS &ws, const double2 *V, int nv, const double2 *w, int n, double2 *out, hipStream_t s) {
+// batched dots <V_r[:, 0..nv), w_r> -> out[r * ostride + l]
+static int bdot(KrylovWS &ws, const double2 *V, long vstride, int nv, const double2 *w, long wstride, int n,
+                double2 *out, long ostride, hipStream_t s) {
     const int nblk = (n + 1023) / 1024;
-    k_mdot_part<<<nblk, 256, 0, s>>>(V, ws.ldv, nv, w, n, ws.part);
+    dim3 g1(nblk, ws.R);
+    kb_mdot_part<<<g1, 256, 0, s>>>(V, ws.ldv, vstride, nv, w, wstride, n, ws.part, ws.pstride);
     HIP_CALL(hipGetLastError());
-    k_mdot_final<<<1, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, nblk, nv, out);
+    dim3 g2(1, ws.R);
+    kb_mdot_final<<<g2, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride);
     HIP_CALL(hipGetLastError());
-    return EXIT_SUCCESS;
-}
-
-static int norm2_host(KrylovWS &ws, const double2 *w, int n, double &nrm, hipStream_t s) {
-    HOST_CALL(mdot(ws, w, 1, w, n, ws.h, s));
-    double2 hv;
-    HIP_CALL(hipMemcpyAsync(&hv, ws.h, sizeof(double2), hipMemcpyDeviceToHost, s));
-    HIP_CALL(hipStreamSynchronize(s));
-    nrm = std::sqrt(std::max(0.0, hv.x));
     return EXIT_SUCCESS;
 }
 
@@ -208,201 +208,246 @@ struct Operator {
     long lda = 0;
     int M = 0, N = 0;
     bool normal() const { return M > N; }
-    // w64 <- op(V[:, j]) ; for FP32 A the input is v32
-    int apply(KrylovWS &ws, const double2 *vj, hipStream_t s) const {
+    // w64_r <- op(V_r[:, j]) for every system (FP32 A reads v32)
+    int apply(KrylovWS &ws, int j, hipStream_t s) const {
         const int bs = 256;
+        const int R = ws.R;
         if (A32) {
-            if (!normal()) {
-                HOST_CALL(gemv_n_f32(A32, lda, M, N, ws.v32, 0, ws.w32, 0, 1, ws.gws, s));
+            if (R == 1) {
+                if (!normal()) {
+                    HOST_CALL(gemv_n_f32(A32, lda, M, N, ws.v32, 0, ws.w32, 0, 1, ws.gws, s));
+                } else {
+                    HOST_CALL(gemv_n_f32(A32, lda, M, N, ws.v32, 0, ws.t32, 0, 1, ws.gws, s));
+                    HOST_CALL(gemv_h_f32(A32, lda, M, N, ws.t32, 0, ws.w32, 0, 1, s));
+                }
             } else {
-                HOST_CALL(gemv_n_f32(A32, lda, M, N, ws.v32, 0, ws.t32, 0, 1, ws.gws, s));
-                HOST_CALL(gemv_h_f32(A32, lda, M, N, ws.t32, 0, ws.w32, 0, 1, s));
+                if (!normal()) {
+                    HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.w32, ws.ldv, R, ws.gws, s));
+                } else {
+                    HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.t32, ws.ldv, R, ws.gws, s));
+                    HOST_CALL(gemm_h_f32(A32, lda, M, N, ws.t32, ws.ldv, ws.w32, ws.ldv, R, ws.gws, s));
+                }
             }
-            k_cvt_c2z<<<(N + bs - 1) / bs, bs, 0, s>>>(ws.w32, ws.w64, N);
+            dim3 g((N + bs - 1) / bs, R);
+            kb_cvt<<<g, bs, 0, s>>>(ws.w32, ws.ldv, ws.w64, ws.ldv, N);
             HIP_CALL(hipGetLastError());
         } else {
-            if (!normal()) {
-                HOST_CALL(gemv_n_f64(A64, lda, M, N, vj, 0, ws.w64, 0, 1, ws.gws, s));
-            } else {
-                HOST_CALL(gemv_n_f64(A64, lda, M, N, vj, 0, ws.t64, 0, 1, ws.gws, s));
-                HOST_CALL(gemv_h_f64(A64, lda, M, N, ws.t64, 0, ws.w64, 0, 1, s));
+            for (int q = 0; q < R; q++) {
+                const double2 *vj = ws.V + (size_t)q * ws.vstride + (size_t)j * ws.ldv;
+                double2 *w = ws.w64 + (size_t)q * ws.ldv, *t = ws.t64 + (size_t)q * ws.ldv;
+                if (!normal()) {
+                    HOST_CALL(gemv_n_f64(A64, lda, M, N, vj, 0, w, 0, 1, ws.gws, s));
+                } else {
+                    HOST_CALL(gemv_n_f64(A64, lda, M, N, vj, 0, t, 0, 1, ws.gws, s));
+                    HOST_CALL(gemv_h_f64(A64, lda, M, N, t, 0, w, 0, 1, s));
+                }
             }
         }
         return EXIT_SUCCESS;
     }
-    // r <- rhs_ne - op(x) in FP64: square: b - A x; normal: A^H (b - A x)
-    int residual(KrylovWS &ws, const double2 *b, const double2 *x, double2 *r, hipStream_t s) const {
+    // r <- rhs - op(x) in FP64 for every system: square b - A x; normal A^H (b - A x)
+    int residual(KrylovWS &ws, const double2 *B, long ldb, const double2 *X, long ldx, double2 *r,
+                 hipStream_t s) const {
+        const int R = ws.R;
         if (A32) {
-            if (!normal()) {
-                HOST_CALL(resid_n_f32a(A32, lda, M, N, x, 0, b, 0, r, 0, 1, ws.gws, s));
+            if (R == 1) {
+                if (!normal()) {
+                    HOST_CALL(resid_n_f32a(A32, lda, M, N, X, 0, B, 0, r, 0, 1, ws.gws, s));
+                } else {
+                    HOST_CALL(resid_n_f32a(A32, lda, M, N, X, 0, B, 0, ws.t64, 0, 1, ws.gws, s));
+                    HOST_CALL(gemv_h_f32a(A32, lda, M, N, ws.t64, 0, r, 0, 1, s));
+                }
+            } else if (!normal()) {
+                HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, r, ws.ldv, R, ws.gws, s));
             } else {
-                HOST_CALL(resid_n_f32a(A32, lda, M, N, x, 0, b, 0, ws.t64, 0, 1, ws.gws, s));
-                HOST_CALL(gemv_h_f32a(A32, lda, M, N, ws.t64, 0, r, 0, 1, s));
+                HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, ws.t64, ws.ldv, R, ws.gws, s));
+                HOST_CALL(mh_f32a(A32, lda, M, N, ws.t64, ws.ldv, r, ws.ldv, R, s));
             }
-        } else {
+        } else if (R == 1) {
             if (!normal()) {
-                HOST_CALL(resid_n_f64(A64, lda, M, N, x, 0, b, 0, r, 0, 1, ws.gws, s));
+                HOST_CALL(resid_n_f64(A64, lda, M, N, X, 0, B, 0, r, 0, 1, ws.gws, s));
             } else {
-                HOST_CALL(resid_n_f64(A64, lda, M, N, x, 0, b, 0, ws.t64, 0, 1, ws.gws, s));
+                HOST_CALL(resid_n_f64(A64, lda, M, N, X, 0, B, 0, ws.t64, 0, 1, ws.gws, s));
                 HOST_CALL(gemv_h_f64(A64, lda, M, N, ws.t64, 0, r, 0, 1, s));
             }
+        } else {
+            if (!normal()) {
+                HOST_CALL(mres_f64(A64, lda, M, N, X, ldx, B, ldb, r, ws.ldv, R, ws.gws, s));
+            } else {
+                HOST_CALL(mres_f64(A64, lda, M, N, X, ldx, B, ldb, ws.t64, ws.ldv, R, ws.gws, s));
+                HOST_CALL(mh_f64(A64, lda, M, N, ws.t64, ws.ldv, r, ws.ldv, R, s));
+            }
         }
         return EXIT_SUCCESS;
     }
 };
 
-struct GmresResult {
-    int iterations = 0;
-    int cycles = 0;
-    bool converged = false;
-    double relres = 0;
-};
-
-// Solve op(x) = rhs for one right-hand side.  b: M-vector (FP64, device), x:
-// N-vector (FP64, device, initial guess on entry).  Each restart cycle runs up
-// to `restart` Arnoldi steps or until the estimated residual fell by `tol`
-// relative to the cycle's start; cycles repeat until the FP64 residual is below
-// `outer_tol` relative to ||rhs|| (or maxit / max_cycles is reached).
-static int gmres_solve(const Operator &op, KrylovWS &ws, const double2 *b, double2 *x, int restart, double tol,
-                       int maxit, double outer_tol, int max_cycles, GmresResult &res, hipStream_t s) {
-    const int N = op.N;
-    const int bs = 256;
-    const int gN = (N + bs - 1) / bs;
-    // ||rhs||: rhs = b (square) or A^H b (normal equations)
-    double nb;
-    {
-        // rhs norm via the residual at x = 0
-        HIP_CALL(hipMemsetAsync(ws.w64, 0, sizeof(double2) * ws.ldv, s));
-        HOST_CALL(op.residual(ws, b, ws.w64, ws.r, s));
-        HOST_CALL(norm2_host(ws, ws.r, N, nb, s));
-    }
-    res = GmresResult();
-    if (nb == 0) {
-        HIP_CALL(hipMemsetAsync(x, 0, sizeof(double2) * N, s));
-        res.converged = true;
-        return EXIT_SUCCESS;
-    }
-    std::vector<zc> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart);
-    std::vector<double2> hcol(restart + 2);
-    int total = 0;
-    for (int cyc = 0; cyc < max_cycles; cyc++) {
-        HOST_CALL(op.residual(ws, b, x, ws.r, s));
-        double beta;
-        HOST_CALL(norm2_host(ws, ws.r, N, beta, s));
-        res.relres = beta / nb;
-        if (beta / nb <= outer_tol || total >= maxit) {
-            res.converged = beta / nb <= outer_tol;
-            break;
-        }
-        res.cycles++;
-        // V[:, 0] = r / beta
-        HOST_CALL(mdot(ws, ws.r, 1, ws.r, N, ws.h, s));
-        k_normalize<<<gN, bs, 0, s>>>(ws.r, ws.h, ws.V, op.A32 ? ws.v32 : nullptr, N);
-        HIP_CALL(hipGetLastError());
-        std::fill(g.begin(), g.end(), zc(0));
-        g[0] = beta;
-        int j = 0;
-        for (; j < restart && total < maxit; j++) {
-            total++;
-            HOST_CALL(op.apply(ws, ws.V + (size_t)j * ws.ldv, s));
-            // CGS2 in FP64
-            double2 *h1 = ws.h, *h2 = ws.h + (restart + 2);
-            HOST_CALL(mdot(ws, ws.V, j + 1, ws.w64, N, h1, s));
-            k_maxpy<<<gN, bs, 0, s>>>(ws.V, ws.ldv, j + 1, h1, -1.0, ws.w64, N);
-            HIP_CALL(hipGetLastError());
-            HOST_CALL(mdot(ws, ws.V, j + 1, ws.w64, N, h2, s));
-            k_maxpy<<<gN, bs, 0, s>>>(ws.V, ws.ldv, j + 1, h2, -1.0, ws.w64, N);
-            HIP_CALL(hipGetLastError());
-            k_hadd<<<1, 64 * ((j + 1 + 63) / 64), 0, s>>>(h1, h2, j + 1);
-            HIP_CALL(hipGetLastError());
-            HOST_CALL(mdot(ws, ws.w64, 1, ws.w64, N, h2, s));
-            k_normalize<<<gN, bs, 0, s>>>(ws.w64, h2, ws.V + (size_t)(j + 1) * ws.ldv, op.A32 ? ws.v32 : nullptr, N);
-            HIP_CALL(hipGetLastError());
-            HIP_CALL(hipMemcpyAsync(hcol.data(), h1, sizeof(double2) * (j + 1), hipMemcpyDeviceToHost, s));
-            double2 hn2;
-            HIP_CALL(hipMemcpyAsync(&hn2, h2, sizeof(double2), hipMemcpyDeviceToHost, s));
-            HIP_CALL(hipStreamSynchronize(s));
-            zc *hc = &H[(size_t)j * (restart + 1)];
-            for (int l = 0; l <= j; l++) hc[l] = zc(hcol[l].x, hcol[l].y);
-            const double hn = std::sqrt(std::max(0.0, hn2.x));
-            hc[j + 1] = hn;
-            // previous rotations, new rotation (GMRES.cu:58-88 / 44-56)
-            for (int l = 0; l < j; l++) {
-                zc t0 = cs[l] * hc[l] + sn[l] * hc[l + 1];
-                zc t1 = -std::conj(sn[l]) * hc[l] + std::conj(cs[l]) * hc[l + 1];
-                hc[l] = t0;
-                hc[l + 1] = t1;
-            }
-            const double mag = std::sqrt(std::norm(hc[j]) + std::norm(hc[j + 1]));
-            if (mag == 0) {
-                j++;
-                break;
-            }
-            cs[j] = std::conj(hc[j]) / mag;
-            sn[j] = std::conj(hc[j + 1]) / mag;
-            hc[j] = cs[j] * hc[j] + sn[j] * hc[j + 1];
-            hc[j + 1] = 0;
-            g[j + 1] = -std::conj(sn[j]) * g[j];
-            g[j] = cs[j] * g[j];
-            if (std::abs(g[j + 1]) <= tol * beta || hn == 0) {
-                j++;
-                break;
-            }
-        }
-        // back substitution (Rsolver, GMRES.cu:26-42), x += V y
-        const int t = j;
-        for (int i = t - 1; i >= 0; i--) {
-            zc acc = g[i];
-            for (int l = i + 1; l < t; l++) acc -= H[(size_t)l * (restart + 1) + i] * y[l];
-            y[i] = acc / H[(size_t)i * (restart + 1) + i];
-        }
-        std::vector<double2> yh(t);
-        for (int i = 0; i < t; i++) yh[i] = make_double2(y[i].real(), y[i].imag());
-        HIP_CALL(hipMemcpyAsync(ws.h, yh.data(), sizeof(double2) * t, hipMemcpyHostToDevice, s));
-        k_maxpy<<<gN, bs, 0, s>>>(ws.V, ws.ldv, t, ws.h, 1.0, x, N);
-        HIP_CALL(hipGetLastError());
-        HIP_CALL(hipStreamSynchronize(s));
-    }
-    res.iterations = total;
-    if (!res.converged) {
-        HOST_CALL(op.residual(ws, b, x, ws.r, s));
-        double beta;
-        HOST_CALL(norm2_host(ws, ws.r, N, beta, s));
-        res.relres = beta / nb;
-        res.converged = res.relres <= outer_tol;
-    }
-    return EXIT_SUCCESS;
-}
-
-// Public (internal) entry: solve for R right-hand sides with FP32 or FP64 A.
-// B: M x R (ldb) FP64 on the device, X: N x R (ldx) FP64 on the device (in/out).
+// Restarted GMRES for R right-hand sides in lockstep: each system has its own
+// FP64 Krylov basis, Hessenberg matrix and Givens rotations; the operator is
+// applied to all of them at once (one MFMA product per Arnoldi step when
+// R > 1).  Each cycle starts from the true FP64 residual and runs up to
+// `restart` steps; a system leaves the cycle when its estimated residual fell by
+// `tol` relative to the cycle's start, and the solve stops when every system's
+// FP64 residual is below `outer_tol` relative to ||rhs|| (or maxit / max_cycles).
+// B: M x R (ldb), X: N x R (ldx, initial guess on entry).
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, int R, int restart, double tol, int maxit, double outer_tol, int max_cycles,
                  bem_solve_stats *st, hipStream_t s) {
     if (M < N) BEM_FAIL("GMRES needs M >= N (got %d x %d)", M, N);
+    if (R <= 0) return EXIT_SUCCESS;
     restart = std::max(1, std::min(restart, N));
     KrylovWS ws;
-    HOST_CALL(ws.init(N, M, restart, s));
+    HOST_CALL(ws.init(N, M, R, restart, s));
     Operator op;
     op.A32 = A32;
     op.A64 = A64;
     op.lda = lda;
     op.M = M;
     op.N = N;
-    int iters = 0, cycles = 0;
+    const int bs = 256;
+    const long ldv = ws.ldv;
+    // ||rhs|| per system: residual at x = 0
+    HIP_CALL(hipMemsetAsync(ws.w64, 0, sizeof(double2) * ldv * R, s));
+    HOST_CALL(op.residual(ws, B, ldb, ws.w64, ldv, ws.r, s));
+    std::vector<double2> hbuf((size_t)ws.hstride * R);
+    auto norms = [&](const double2 *w, std::vector<double> &out) -> int {
+        HOST_CALL(bdot(ws, w, ldv, 1, w, ldv, N, ws.h, ws.hstride, s));
+        HIP_CALL(hipMemcpyAsync(hbuf.data(), ws.h, sizeof(double2) * ws.hstride * R, hipMemcpyDeviceToHost, s));
+        HIP_CALL(hipStreamSynchronize(s));
+        for (int q = 0; q < R; q++) out[q] = std::sqrt(std::max(0.0, hbuf[(size_t)q * ws.hstride].x));
+        return EXIT_SUCCESS;
+    };
+    std::vector<double> nb(R), beta(R), relres(R, 0.0);
+    HOST_CALL(norms(ws.r, nb));
+    std::vector<char> done(R, 0);
+    for (int q = 0; q < R; q++)
+        if (nb[q] == 0) {
+            done[q] = 1;
+            HIP_CALL(hipMemsetAsync(X + (size_t)q * ldx, 0, sizeof(double2) * N, s));
+        }
+    const int K = restart;
+    std::vector<zc> H((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K), g((size_t)R * (K + 1));
+    std::vector<int> tlen(R, 0);
+    std::vector<char> live(R, 0);
+    int total = 0, cycles = 0;
+    for (int cyc = 0; cyc < max_cycles; cyc++) {
+        HOST_CALL(op.residual(ws, B, ldb, X, ldx, ws.r, s));
+        HOST_CALL(norms(ws.r, beta));
+        bool any = false;
+        for (int q = 0; q < R; q++) {
+            if (nb[q] == 0) continue;
+            relres[q] = beta[q] / nb[q];
+            if (relres[q] <= outer_tol) done[q] = 1;
+            any = any || !done[q];
+        }
+        if (!any || total >= maxit) break;
+        cycles++;
+        // V_r[:, 0] = r_r / beta_r (norms still in ws.h[.., 0])
+        {
+            dim3 gg((N + bs - 1) / bs, R);
+            kb_normalize<<<gg, bs, 0, s>>>(ws.r, ldv, ws.h, ws.hstride, ws.V, ws.vstride, op.A32 ? ws.v32 : nullptr,
+                                           ldv, N);
+            HIP_CALL(hipGetLastError());
+        }
+        for (int q = 0; q < R; q++) {
+            live[q] = !done[q];
+            tlen[q] = 0;
+            for (int i = 0; i <= K; i++) g[(size_t)q * (K + 1) + i] = 0;
+            g[(size_t)q * (K + 1)] = beta[q];
+            if (done[q] && op.A32) HIP_CALL(hipMemsetAsync(ws.v32 + (size_t)q * ldv, 0, sizeof(float2) * ldv, s));
+        }
+        int j = 0;
+        for (; j < K && total < maxit; j++) {
+            total++;
+            HOST_CALL(op.apply(ws, j, s));
+            double2 *h1 = ws.h, *h2 = ws.h + (K + 2);
+            dim3 gg((N + bs - 1) / bs, R);
+            HOST_CALL(bdot(ws, ws.V, ws.vstride, j + 1, ws.w64, ldv, N, h1, ws.hstride, s));
+            kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, j + 1, h1, ws.hstride, -1.0, ws.w64, ldv, N);
+            HIP_CALL(hipGetLastError());
+            HOST_CALL(bdot(ws, ws.V, ws.vstride, j + 1, ws.w64, ldv, N, h2, ws.hstride, s));
+            kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, j + 1, h2, ws.hstride, -1.0, ws.w64, ldv, N);
+            HIP_CALL(hipGetLastError());
+            kb_hadd<<<dim3(1, R), 64 * ((j + 1 + 63) / 64), 0, s>>>(h1, h2, ws.hstride, j + 1);
+            HIP_CALL(hipGetLastError());
+            HOST_CALL(bdot(ws, ws.w64, ldv, 1, ws.w64, ldv, N, h2, ws.hstride, s));
+            kb_normalize<<<gg, bs, 0, s>>>(ws.w64, ldv, h2, ws.hstride, ws.V + (size_t)(j + 1) * ldv, ws.vstride,
+                                           op.A32 ? ws.v32 : nullptr, ldv, N);
+            HIP_CALL(hipGetLastError());
+            HIP_CALL(hipMemcpyAsync(hbuf.data(), ws.h, sizeof(double2) * ws.hstride * R, hipMemcpyDeviceToHost, s));
+            HIP_CALL(hipStreamSynchronize(s));
+            bool anylive = false;
+            for (int q = 0; q < R; q++) {
+                if (!live[q]) continue;
+                const double2 *hq = &hbuf[(size_t)q * ws.hstride];
+                zc *hc = &H[((size_t)q * K + j) * (K + 1)];
+                zc *csq = &cs[(size_t)q * K], *snq = &sn[(size_t)q * K], *gq = &g[(size_t)q * (K + 1)];
+                for (int l = 0; l <= j; l++) hc[l] = zc(hq[l].x, hq[l].y);
+                const double hn = std::sqrt(std::max(0.0, hq[K + 2].x));
+                hc[j + 1] = hn;
+                // previous rotations, then a new one (GMRES.cu:58-88 / 44-56)
+                for (int l = 0; l < j; l++) {
+                    const zc t0 = csq[l] * hc[l] + snq[l] * hc[l + 1];
+                    const zc t1 = -std::conj(snq[l]) * hc[l] + std::conj(csq[l]) * hc[l + 1];
+                    hc[l] = t0;
+                    hc[l + 1] = t1;
+                }
+                const double mag = std::sqrt(std::norm(hc[j]) + std::norm(hc[j + 1]));
+                tlen[q] = j + 1;
+                if (mag == 0) {
+                    live[q] = 0;
+                    continue;
+                }
+                csq[j] = std::conj(hc[j]) / mag;
+                snq[j] = std::conj(hc[j + 1]) / mag;
+                hc[j] = csq[j] * hc[j] + snq[j] * hc[j + 1];
+                hc[j + 1] = 0;
+                gq[j + 1] = -std::conj(snq[j]) * gq[j];
+                gq[j] = csq[j] * gq[j];
+                if (std::abs(gq[j + 1]) <= tol * beta[q] || hn == 0) live[q] = 0;
+                anylive = anylive || live[q];
+            }
+            if (!anylive) {
+                j++;
+                break;
+            }
+        }
+        // back substitution per system (Rsolver, GMRES.cu:26-42), x_r += V_r y_r
+        int tmax = 0;
+        for (int q = 0; q < R; q++) tmax = std::max(tmax, tlen[q]);
+        if (tmax == 0) continue;
+        std::vector<double2> yh((size_t)ws.hstride * R, make_double2(0, 0));
+        for (int q = 0; q < R; q++) {
+            const int t = tlen[q];
+            if (done[q] || t == 0) continue;
+            std::vector<zc> y(t);
+            const zc *Hq = &H[(size_t)q * K * (K + 1)];
+            const zc *gq = &g[(size_t)q * (K + 1)];
+            for (int i = t - 1; i >= 0; i--) {
+                zc acc = gq[i];
+                for (int l = i + 1; l < t; l++) acc -= Hq[(size_t)l * (K + 1) + i] * y[l];
+                y[i] = acc / Hq[(size_t)i * (K + 1) + i];
+            }
+            for (int i = 0; i < t; i++) yh[(size_t)q * ws.hstride + i] = make_double2(y[i].real(), y[i].imag());
+        }
+        HIP_CALL(hipMemcpyAsync(ws.h, yh.data(), sizeof(double2) * ws.hstride * R, hipMemcpyHostToDevice, s));
+        dim3 gg((N + bs - 1) / bs, R);
+        kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, tmax, ws.h, ws.hstride, 1.0, X, ldx, N);
+        HIP_CALL(hipGetLastError());
+        HIP_CALL(hipStreamSynchronize(s));  // yh goes out of scope
+    }
+    // final FP64 residuals
+    HOST_CALL(op.residual(ws, B, ldb, X, ldx, ws.r, s));
+    HOST_CALL(norms(ws.r, beta));
     bool conv = true;
     double worst = 0;
-    for (int r = 0; r < R; r++) {
-        GmresResult res;
-        HOST_CALL(gmres_solve(op, ws, B + (size_t)r * ldb, X + (size_t)r * ldx, restart, tol, maxit, outer_tol,
-                              max_cycles, res, s));
-        iters += res.iterations;
-        cycles += res.cycles;
-        conv = conv && res.converged;
-        worst = std::max(worst, res.relres);
+    for (int q = 0; q < R; q++) {
+        const double rr = nb[q] > 0 ? beta[q] / nb[q] : 0.0;
+        worst = std::max(worst, rr);
+        conv = conv && rr <= outer_tol;
     }
     if (st) {
-        st->iterations = iters;
+        st->iterations = total;
         st->refinements = cycles;
         st->converged = conv ? 1 : 0;
         st->residual = worst;

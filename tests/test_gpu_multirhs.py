@@ -1,0 +1,146 @@
+"""Many right-hand sides solved in lockstep (MFMA multi-RHS products).
+
+The batched GMRES applies the FP32 operator to every system's basis vector in
+one MFMA product (A X, and A^H T for CHIEF's normal equations) and computes the
+refinement residuals for all systems with FP64 accumulation.  Checks:
+  * each column against the FP64 oracle's least-squares solution, relative
+    2-norm error <= 1e-5 (the same bar as single right-hand sides);
+  * each column against the same system solved alone (the GEMV path) after
+    refinement to 1e-10: relative difference <= 1e-8;
+  * the FP64 matrix path (per-system GEMVs, FP64 multi-RHS residuals) against
+    the oracle at 1e-9.
+Right-hand-side counts cover one and two 16-wide MFMA groups, ragged groups
+(20, 40) and the maximum of 64.
+"""
+import numpy as np
+import pytest
+
+import bem_gpu_clean_cpp_amd as bem
+import bemcases as C
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+RHO, SPEED = float(np.float32(1.21)), float(np.float32(343.0))
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _setup():
+    bem.quad_init(6)
+    O.quad_init(6)
+    bem.set_medium(RHO, SPEED)
+    yield
+    bem.set_medium(1.2041, 343.21)
+
+
+def _mesh(s, nchief, r=0.1, seed=7):
+    m = bem.Mesh().icosphere(s, r)
+    if nchief:
+        m.set_chief(C.ball_points(seed, nchief, r / 2))
+    return m
+
+
+def _dirs(n, seed=11):
+    return C.sphere_points(seed, n, 1.0)
+
+
+def _oracle_lstsq(m, k, d):
+    p, e, bc = C.mesh_arrays(m)
+    pts = np.vstack([p, m.chief()]).astype(np.float64)
+    binit = np.exp(-1j * k * pts @ np.asarray(d, np.float64).T)
+    A, B = O.assemble(k, pts, p.shape[0], e, bc, binit=binit, dens=RHO, speed=SPEED)
+    return np.linalg.lstsq(A, B, rcond=None)[0]
+
+
+def _relcols(x, y):
+    return np.linalg.norm(x - y, axis=0) / np.linalg.norm(y, axis=0)
+
+
+@pytest.mark.parametrize("nchief", [0, 8])
+@pytest.mark.parametrize("R", [16, 20, 40])
+def test_multi_rhs_gmres_matches_oracle(nchief, R):
+    m = _mesh(3, nchief)
+    k = 31.3
+    d = _dirs(R)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    st = s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+    x = s.solution()
+    assert x.shape == (m.num_pnts, R)
+    xo = _oracle_lstsq(m, k, d)
+    err = _relcols(x, xo)
+    print("R %d nchief %d stats %s worst err %.3g" % (R, nchief, st, err.max()))
+    assert st["converged"] == 1
+    assert err.max() < 1e-5
+
+
+@pytest.mark.parametrize("nchief", [0, 8])
+def test_multi_rhs_equals_single_rhs(nchief):
+    m = _mesh(3, nchief)
+    k = 31.3
+    d = _dirs(20, seed=5)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+    xb = s.solution()
+    for q in (0, 7, 19):
+        s.set_rhs(bem.RHS_PLANE, d[q:q + 1], 1.0)
+        st = s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+        x1 = s.solution()[:, 0]
+        rel = np.linalg.norm(xb[:, q] - x1) / np.linalg.norm(x1)
+        assert st["converged"] == 1
+        assert rel < 1e-8, (q, rel)
+
+
+def test_multi_rhs_max_batch_point_sources():
+    """64 right-hand sides (the largest batch) of point sources inside the body."""
+    m = _mesh(2, 4)
+    k = 12.0
+    src = C.ball_points(3, 64, 0.04)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_POINT, src, 1.0)
+    st = s.solve(bem.SOLVE_GMRES, restart=40, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+    x = s.solution()
+    p, e, bc = C.mesh_arrays(m)
+    pts = np.vstack([p, m.chief()]).astype(np.float64)
+    r = np.linalg.norm(pts[:, None, :] - np.asarray(src, np.float64)[None, :, :], axis=2)
+    binit = np.exp(-1j * k * r) / (4 * np.pi * r)
+    A, B = O.assemble(k, pts, p.shape[0], e, bc, binit=binit, dens=RHO, speed=SPEED)
+    xo = np.linalg.lstsq(A, B, rcond=None)[0]
+    err = _relcols(x, xo)
+    print("stats %s worst err %.3g" % (st, err.max()))
+    assert st["converged"] == 1
+    assert err.max() < 1e-5
+
+
+def test_multi_rhs_fp64_matrix():
+    m = _mesh(2, 6)
+    k = 20.0
+    d = _dirs(9, seed=2)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP64)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    st = s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=3000, refine_tol=1e-12, max_refine=20)
+    x = s.solution()
+    xo = _oracle_lstsq(m, k, d)
+    err = _relcols(x, xo)
+    assert st["converged"] == 1, st
+    assert err.max() < 1e-9, err.max()
+
+
+def test_multi_rhs_qr_matches_gmres():
+    m = _mesh(3, 8)
+    k = 31.3
+    d = _dirs(24, seed=9)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+    xg = s.solution()
+    st = s.solve(bem.SOLVE_QR, refine_tol=1e-10, max_refine=20)
+    xq = s.solution()
+    assert st["converged"] == 1
+    assert _relcols(xq, xg).max() < 1e-7
