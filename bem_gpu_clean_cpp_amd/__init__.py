@@ -52,6 +52,32 @@ class bem_solve_stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class bem_solve_params(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("tol", ctypes.c_float),
+                ("maxit", ctypes.c_int), ("refine_tol", ctypes.c_double), ("max_refine", ctypes.c_int)]
+
+
+class bem_request_info(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_long), ("status", ctypes.c_int), ("slot", ctypes.c_int), ("N", ctypes.c_int),
+                ("M", ctypes.c_int), ("nrhs", ctypes.c_int), ("footprint_bytes", ctypes.c_double),
+                ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double), ("end_ms", ctypes.c_double),
+                ("solve", bem_solve_stats)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "solve"}
+        d["solve"] = self.solve.as_dict()
+        return d
+
+
+class bem_server_info(ctypes.Structure):
+    _fields_ = [("queued", ctypes.c_int), ("running", ctypes.c_int), ("max_running", ctypes.c_int),
+                ("slots", ctypes.c_int), ("budget_bytes", ctypes.c_double), ("in_use_bytes", ctypes.c_double),
+                ("peak_in_use_bytes", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 assert ctypes.sizeof(bem_tri_elem) == 40 and ctypes.sizeof(bem_cart_coord) == 12
 
 ELEM_DTYPE = np.dtype([("nodes", np.int32, 3), ("_pad", np.int32), ("bc", np.complex64, 3)])
@@ -106,6 +132,16 @@ _SIGS = {
     "bem_solver_field": (_I, [_P, _P, _I, _P, _I]),
     "bem_solver_dims": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "bem_solver_stats": (_I, [_P, ctypes.POINTER(bem_solve_stats)]),
+    "bem_solver_set_mesh": (_I, [_P, _P]),
+    "bem_solver_release": (_I, [_P]),
+    "bem_batch_solve": (_I, [_P, _P, _I, _I, _I, _P, _I, _I, _F, ctypes.POINTER(bem_solve_params), _I, _P,
+                             _I, _P]),
+    "bem_server_create": (_P, [_I, _D]),
+    "bem_server_submit": (ctypes.c_long, [_P, _P, _F, _I, _I, _P, _I, _F, ctypes.POINTER(bem_solve_params)]),
+    "bem_server_wait": (_I, [_P, ctypes.c_long, _P, _I, ctypes.POINTER(bem_request_info)]),
+    "bem_server_status": (_I, [_P, ctypes.POINTER(bem_server_info)]),
+    "bem_server_destroy": (None, [_P]),
+    "bem_request_footprint": (_D, [_I, _I, _I, _I, _I, _I, _I]),
     "bem_timing_enable": (_I, [_I]),
     "bem_timing_reset": (_I, []),
     "bem_timing_get": (_I, [ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long)]),
@@ -408,6 +444,84 @@ class Solver:
         st = bem_solve_stats()
         _check(lib().bem_solver_stats(self._h, ctypes.byref(st)), "stats")
         return st.as_dict()
+
+    def set_mesh(self, mesh):
+        _check(lib().bem_solver_set_mesh(self._h, mesh.handle), "set_mesh")
+        return self
+
+    def release(self):
+        _check(lib().bem_solver_release(self._h), "release")
+
+
+# ------------------------------------------------- batches and serving ---
+def solve_params(method=SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-8, max_refine=50):
+    return bem_solve_params(method, restart, tol, maxit, refine_tol, max_refine)
+
+
+def batch_solve(mesh, ks, kind, params, amp=1.0, precision=PREC_FP32, workers=4, **solve_kw):
+    """One mesh at many wavenumbers.  params: (nrhs, 3) for the same right-hand
+    sides at every frequency, or (nk, nrhs, 3) for one set per frequency.
+    Returns (X with shape (N, nrhs, nk) complex128, list of per-frequency stats)."""
+    k = np.ascontiguousarray(ks, np.float32).ravel()
+    nk = k.size
+    p = np.ascontiguousarray(params, np.float32)
+    per_freq = 1 if p.ndim == 3 else 0
+    if per_freq and p.shape[0] != nk:
+        raise BemError("params must be (nrhs, 3) or (nk, nrhs, 3)")
+    R = p.shape[-2]
+    N = mesh.num_pnts
+    X = np.zeros((nk, R, N), np.complex128)
+    st = (bem_solve_stats * nk)()
+    sp = solve_params(**solve_kw)
+    _check(lib().bem_batch_solve(mesh.handle, _ptr(k), nk, precision, kind, _ptr(p), R, per_freq, amp,
+                                 ctypes.byref(sp), workers, _ptr(X), N, st), "batch_solve")
+    return X.transpose(2, 1, 0).copy(), [x.as_dict() for x in st]
+
+
+def request_footprint(M, N, E, nrhs, precision=PREC_FP32, method=SOLVE_GMRES, restart=30):
+    return lib().bem_request_footprint(M, N, E, nrhs, precision, method, restart)
+
+
+class Server:
+    """Request server with continuous admission by HBM footprint (bem_server)."""
+
+    def __init__(self, slots=4, hbm_budget_bytes=0.0):
+        self._h = lib().bem_server_create(slots, float(hbm_budget_bytes))
+        if not self._h:
+            raise BemError("bem_server_create failed: %s" % last_error())
+        self._n = {}
+
+    def submit(self, mesh, k, kind, params, amp=1.0, precision=PREC_FP32, **solve_kw):
+        p = np.ascontiguousarray(params, np.float32).reshape(-1, 3)
+        sp = solve_params(**solve_kw)
+        rid = lib().bem_server_submit(self._h, mesh.handle, k, precision, kind, _ptr(p), p.shape[0], amp,
+                                      ctypes.byref(sp))
+        if rid < 0:
+            raise BemError("bem_server_submit failed: %s" % last_error())
+        self._n[rid] = (mesh.num_pnts, p.shape[0])
+        return rid
+
+    def wait(self, rid):
+        """(solution N x nrhs complex128, request record dict); raises on failure."""
+        N, R = self._n.pop(rid)
+        x = np.zeros((R, N), np.complex128)
+        info = bem_request_info()
+        _check(lib().bem_server_wait(self._h, rid, _ptr(x), N, ctypes.byref(info)), "server_wait")
+        return x.T.copy(), info.as_dict()
+
+    def status(self):
+        info = bem_server_info()
+        _check(lib().bem_server_status(self._h, ctypes.byref(info)), "server_status")
+        return info.as_dict()
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.bem_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 # ---------------------------------------------------------------- timing ---

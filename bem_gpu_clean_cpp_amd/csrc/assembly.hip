@@ -838,10 +838,27 @@ static int choose_qg(int M, int E) {
     return ((long)M * E < 4000000L) ? 4 : 1;
 }
 
+// Upper bound of the HBM bytes device_problem_build allocates for a mesh of
+// M rows (nodes + CHIEF), N nodes and E elements.
+size_t device_problem_bytes(int M, int N, int E) {
+    const size_t Mp = ((size_t)M + 63) / 64 * 64;
+    const int qg = choose_qg(M, E);
+    const size_t P = (size_t)choose_tiles(M, N, 64 / qg);
+    size_t b = 0;
+    b += sizeof(float4) * M + sizeof(double) * 3 * M + sizeof(double2) * M;        // points, bterm
+    b += (sizeof(int) + sizeof(float2)) * 3 * (size_t)E;                            // elements, bc
+    b += (sizeof(ElemRec<float>) + sizeof(ElemRec<double>)) * (size_t)E;            // element records
+    b += (sizeof(float) + sizeof(double)) * 48 * (size_t)E;                          // singular terms
+    b += sizeof(int) * (P + 1 + 3 * (size_t)E + N);                                  // tile CSR, isolated
+    b += (sizeof(float) + sizeof(float2) + sizeof(double) + sizeof(double2)) * P * Mp;  // tile partials
+    return b;
+}
+
 int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
     HOST_CALL(validate_mesh(m));
     HOST_CALL(ensure_quad());
     device_problem_free(dp);
+    dp.st = s;
     dp.N = m.N();
     dp.C = m.C();
     dp.M = dp.N + dp.C;
@@ -866,23 +883,23 @@ int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
     const int qg = choose_qg(dp.M, dp.E);
     build_tile_plan(m, choose_tiles(dp.M, dp.N, 64 / qg), plan);
     dp.P = plan.P;
-    HOST_CALL(dalloc(&dp.pts, dp.M));
-    HOST_CALL(dalloc(&dp.pts64, 3 * (size_t)dp.M));
-    HOST_CALL(dalloc(&dp.elems, 3 * (size_t)dp.E));
-    HOST_CALL(dalloc(&dp.bc, 3 * (size_t)dp.E));
-    HOST_CALL(dalloc(&dp.rec32, dp.E));
-    HOST_CALL(dalloc(&dp.rec64, dp.E));
-    HOST_CALL(dalloc(&dp.sing32, (size_t)dp.E * 48));
-    HOST_CALL(dalloc(&dp.sing64, (size_t)dp.E * 48));
-    HOST_CALL(dalloc(&dp.tile_ptr, plan.ptr.size()));
-    HOST_CALL(dalloc(&dp.tile_entries, plan.entries.size()));
+    HOST_CALL(salloc(&dp.pts, dp.M, s));
+    HOST_CALL(salloc(&dp.pts64, 3 * (size_t)dp.M, s));
+    HOST_CALL(salloc(&dp.elems, 3 * (size_t)dp.E, s));
+    HOST_CALL(salloc(&dp.bc, 3 * (size_t)dp.E, s));
+    HOST_CALL(salloc(&dp.rec32, dp.E, s));
+    HOST_CALL(salloc(&dp.rec64, dp.E, s));
+    HOST_CALL(salloc(&dp.sing32, (size_t)dp.E * 48, s));
+    HOST_CALL(salloc(&dp.sing64, (size_t)dp.E * 48, s));
+    HOST_CALL(salloc(&dp.tile_ptr, plan.ptr.size(), s));
+    HOST_CALL(salloc(&dp.tile_entries, plan.entries.size(), s));
     dp.n_isolated = (int)plan.isolated.size();
-    HOST_CALL(dalloc(&dp.isolated, std::max<size_t>(1, plan.isolated.size())));
-    HOST_CALL(dalloc(&dp.cpart, (size_t)dp.P * dp.Mp));
-    HOST_CALL(dalloc(&dp.bpart, (size_t)dp.P * dp.Mp));
-    HOST_CALL(dalloc(&dp.cpart64, (size_t)dp.P * dp.Mp));
-    HOST_CALL(dalloc(&dp.bpart64, (size_t)dp.P * dp.Mp));
-    HOST_CALL(dalloc(&dp.bterm, (size_t)dp.M));
+    HOST_CALL(salloc(&dp.isolated, std::max<size_t>(1, plan.isolated.size()), s));
+    HOST_CALL(salloc(&dp.cpart, (size_t)dp.P * dp.Mp, s));
+    HOST_CALL(salloc(&dp.bpart, (size_t)dp.P * dp.Mp, s));
+    HOST_CALL(salloc(&dp.cpart64, (size_t)dp.P * dp.Mp, s));
+    HOST_CALL(salloc(&dp.bpart64, (size_t)dp.P * dp.Mp, s));
+    HOST_CALL(salloc(&dp.bterm, (size_t)dp.M, s));
     HIP_CALL(hipMemcpyAsync(dp.pts, p4.data(), p4.size() * sizeof(float4), hipMemcpyHostToDevice, s));
     HIP_CALL(hipMemcpyAsync(dp.pts64, p64.data(), p64.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_CALL(hipMemcpyAsync(dp.elems, el.data(), el.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -908,22 +925,22 @@ int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
 }
 
 void device_problem_free(DeviceProblem &dp) {
-    dfree(dp.pts);
-    dfree(dp.pts64);
-    dfree(dp.elems);
-    dfree(dp.bc);
-    dfree(dp.rec32);
-    dfree(dp.rec64);
-    dfree(dp.sing32);
-    dfree(dp.sing64);
-    dfree(dp.tile_ptr);
-    dfree(dp.tile_entries);
-    dfree(dp.isolated);
-    dfree(dp.cpart);
-    dfree(dp.bpart);
-    dfree(dp.cpart64);
-    dfree(dp.bpart64);
-    dfree(dp.bterm);
+    sfree(dp.pts, dp.st);
+    sfree(dp.pts64, dp.st);
+    sfree(dp.elems, dp.st);
+    sfree(dp.bc, dp.st);
+    sfree(dp.rec32, dp.st);
+    sfree(dp.rec64, dp.st);
+    sfree(dp.sing32, dp.st);
+    sfree(dp.sing64, dp.st);
+    sfree(dp.tile_ptr, dp.st);
+    sfree(dp.tile_entries, dp.st);
+    sfree(dp.isolated, dp.st);
+    sfree(dp.cpart, dp.st);
+    sfree(dp.bpart, dp.st);
+    sfree(dp.cpart64, dp.st);
+    sfree(dp.bpart64, dp.st);
+    sfree(dp.bterm, dp.st);
     dp.N = dp.C = dp.M = dp.E = dp.P = 0;
 }
 

@@ -151,9 +151,11 @@ struct DeviceProblem {
     double2 *bterm = nullptr;     // M: sum_e (cc/cb) sum_j g_j(x_i)
     long Mp = 0;
     uint64_t mesh_version = 0;
+    hipStream_t st = nullptr;     // buffers are stream-ordered on this stream
 };
 int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s);
 void device_problem_free(DeviceProblem &dp);
+size_t device_problem_bytes(int M, int N, int E);
 
 struct RhsSpec {
     int kind = BEM_RHS_POINT;

@@ -7,29 +7,7 @@
 #include <mutex>
 
 #include "bem_internal.hpp"
-
-struct bem_mesh {
-    bem::Mesh m;
-    bem::DeviceProblem dp;
-    uint64_t dp_version = ~0ull;
-};
-
-struct bem_solver {
-    bem::Mesh mesh;
-    bem::DeviceProblem dp;
-    float k = 0.f;
-    int prec = 0;
-    float2 *A32 = nullptr;
-    double2 *A64 = nullptr;
-    bem::RhsSpec rhs;
-    double2 *B64 = nullptr;
-    double2 *X64 = nullptr;
-    float2 *X32 = nullptr;
-    int xr = 0;  // columns held in X64
-    bem_solve_stats stats{};
-    hipStream_t stream = nullptr;  // each solver runs on its own stream
-    mutable std::mutex mu;         // one call at a time per solver
-};
+#include "handles.hpp"
 
 using namespace bem;
 
@@ -413,12 +391,17 @@ bem_solver *bem_solver_create(const bem_mesh *m) {
     return sv;
 }
 
+// buffers are stream-ordered on the solver's stream: freeing them never waits
+// for other solvers' work
 static void solver_free_mats(bem_solver *s) {
-    dfree(s->A32);
-    dfree(s->A64);
-    dfree(s->B64);
-    dfree(s->X64);
-    dfree(s->X32);
+    sfree(s->A32, s->stream);
+    sfree(s->A64, s->stream);
+    sfree(s->B64, s->stream);
+    sfree(s->X64, s->stream);
+    sfree(s->X32, s->stream);
+    s->a_cap = s->x_cap = 0;
+    s->xr = 0;
+    s->prec = 0;
 }
 
 void bem_solver_destroy(bem_solver *s) {
@@ -433,23 +416,56 @@ void bem_solver_destroy(bem_solver *s) {
     delete s;
 }
 
+int bem_solver_set_mesh(bem_solver *sv, const bem_mesh *m) {
+    if (!sv || !m) BEM_FAIL("null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    solver_free_mats(sv);
+    sv->mesh = m->m;
+    return device_problem_build(sv->mesh, sv->dp, sv->stream);
+}
+
+int bem_solver_release(bem_solver *sv) {
+    if (!sv) BEM_FAIL("null solver");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    solver_free_mats(sv);
+    device_problem_free(sv->dp);
+    HIP_CALL(hipStreamSynchronize(sv->stream));
+    return EXIT_SUCCESS;
+}
+
 int bem_solver_assemble(bem_solver *sv, float k, int precision) {
     if (!sv) BEM_FAIL("null solver");
     if (precision != BEM_PREC_FP32 && precision != BEM_PREC_FP64) BEM_FAIL("precision must be 32 or 64");
     std::lock_guard<std::mutex> lk(sv->mu);
     hipStream_t s = sv->stream;
     DeviceProblem &dp = sv->dp;
+    if (dp.M == 0) BEM_FAIL("solver has no mesh (released)");
     hipEvent_t e0, e1;
     HIP_CALL(hipEventCreate(&e0));
     HIP_CALL(hipEventCreate(&e1));
     HIP_CALL(hipEventRecord(e0, s));
+    const size_t need = (size_t)dp.ldA * dp.N;
     if (precision == BEM_PREC_FP32) {
-        dfree(sv->A64);
-        if (!sv->A32) HOST_CALL(dalloc(&sv->A32, (size_t)dp.ldA * dp.N));
+        if (sv->A64) {
+            sfree(sv->A64, s);
+            sv->a_cap = 0;
+        }
+        if (sv->a_cap < need) {
+            sfree(sv->A32, s);
+            HOST_CALL(salloc(&sv->A32, need, s));
+            sv->a_cap = need;
+        }
         HOST_CALL(assemble_fp32(dp, k, sv->A32, s));
     } else {
-        dfree(sv->A32);
-        if (!sv->A64) HOST_CALL(dalloc(&sv->A64, (size_t)dp.ldA * dp.N));
+        if (sv->A32) {
+            sfree(sv->A32, s);
+            sv->a_cap = 0;
+        }
+        if (sv->a_cap < need) {
+            sfree(sv->A64, s);
+            HOST_CALL(salloc(&sv->A64, need, s));
+            sv->a_cap = need;
+        }
         HOST_CALL(assemble_fp64(dp, (double)k, sv->A64, s));
     }
     HIP_CALL(hipEventRecord(e1, s));
@@ -461,6 +477,7 @@ int bem_solver_assemble(bem_solver *sv, float k, int precision) {
     sv->stats.assemble_ms = ms;
     sv->k = k;
     sv->prec = precision;
+    sv->xr = 0;
     return EXIT_SUCCESS;
 }
 
@@ -468,10 +485,12 @@ int bem_solver_set_rhs(bem_solver *sv, int kind, const float *params, int nrhs, 
     if (!sv) BEM_FAIL("null solver");
     if (kind != BEM_RHS_POINT && kind != BEM_RHS_PLANE) BEM_FAIL("unknown right-hand-side kind %d", kind);
     if (nrhs <= 0 || nrhs > kMaxRhs || !params) BEM_FAIL("nrhs must be in [1, %d]", kMaxRhs);
+    std::lock_guard<std::mutex> lk(sv->mu);
     sv->rhs.kind = kind;
     sv->rhs.nrhs = nrhs;
     sv->rhs.amp = amp;
     sv->rhs.params.assign(params, params + 3 * (size_t)nrhs);
+    sv->xr = 0;  // the previous solution belongs to other right-hand sides
     return EXIT_SUCCESS;
 }
 
@@ -486,15 +505,16 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
     DeviceProblem &dp = sv->dp;
     const int R = sv->rhs.nrhs;
     const long ld = dp.ldA;
-    if (sv->xr < R) {
-        dfree(sv->B64);
-        dfree(sv->X64);
-        dfree(sv->X32);
-        HOST_CALL(dalloc(&sv->B64, (size_t)ld * R));
-        HOST_CALL(dalloc(&sv->X64, (size_t)ld * R));
-        HOST_CALL(dalloc(&sv->X32, (size_t)ld * R));
-        sv->xr = R;
+    if (sv->x_cap < (size_t)ld * R) {
+        sfree(sv->B64, s);
+        sfree(sv->X64, s);
+        sfree(sv->X32, s);
+        HOST_CALL(salloc(&sv->B64, (size_t)ld * R, s));
+        HOST_CALL(salloc(&sv->X64, (size_t)ld * R, s));
+        HOST_CALL(salloc(&sv->X32, (size_t)ld * R, s));
+        sv->x_cap = (size_t)ld * R;
     }
+    sv->xr = 0;
     const auto t0 = std::chrono::steady_clock::now();
     HOST_CALL(rhs_fp64(dp, (double)sv->k, sv->B64, ld, sv->rhs, s));
     bem_solve_stats st{};
@@ -512,16 +532,17 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
                                      refine_tol > 0 ? refine_tol : 1e-12, max_refine > 0 ? max_refine : 5, &st, s));
         } else {
             double2 *Af = nullptr;
-            HOST_CALL(dalloc(&Af, (size_t)ld * dp.N));
+            HOST_CALL(salloc(&Af, (size_t)ld * dp.N, s));
             HIP_CALL(hipMemcpyAsync(Af, sv->A64, sizeof(double2) * ld * dp.N, hipMemcpyDeviceToDevice, s));
             HIP_CALL(hipMemcpyAsync(sv->X64, sv->B64, sizeof(double2) * ld * R, hipMemcpyDeviceToDevice, s));
             int rc = qr_lsq_f64(Af, ld, dp.M, dp.N, sv->X64, ld, R, s);
-            dfree(Af);
+            sfree(Af, s);
             HOST_CALL(rc);
             st.converged = 1;
         }
     }
     HIP_CALL(hipStreamSynchronize(s));
+    sv->xr = R;
     const auto t1 = std::chrono::steady_clock::now();
     st.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     st.assemble_ms = sv->stats.assemble_ms;
@@ -573,7 +594,7 @@ int bem_solver_get_system(const bem_solver *sv, bem_complex *A, int lda, bem_com
         float2 *tmp = nullptr;
         if (!src) {
             const long n = dp.ldA * (long)dp.N;
-            HOST_CALL(dalloc(&tmp, (size_t)n));
+            HOST_CALL(salloc(&tmp, (size_t)n, s));
             k_z2c_copy<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(sv->A64, tmp, n);
             HIP_CALL(hipGetLastError());
             src = tmp;
@@ -581,17 +602,17 @@ int bem_solver_get_system(const bem_solver *sv, bem_complex *A, int lda, bem_com
         hipError_t e = hipMemcpy2DAsync(A, sizeof(bem_complex) * lda, src, sizeof(float2) * dp.ldA,
                                         sizeof(float2) * dp.M, dp.N, hipMemcpyDeviceToHost, s);
         (void)hipStreamSynchronize(s);
-        dfree(tmp);
+        sfree(tmp, s);
         HIP_CALL(e);
     }
     if (B && sv->rhs.nrhs > 0) {
         float2 *Bd = nullptr;
-        HOST_CALL(dalloc(&Bd, (size_t)dp.ldA * sv->rhs.nrhs));
+        HOST_CALL(salloc(&Bd, (size_t)dp.ldA * sv->rhs.nrhs, s));
         int rc = rhs_fp32(const_cast<DeviceProblem &>(dp), (double)sv->k, Bd, dp.ldA, sv->rhs, s);
         hipError_t e = hipMemcpy2DAsync(B, sizeof(bem_complex) * ldb, Bd, sizeof(float2) * dp.ldA,
                                         sizeof(float2) * dp.M, sv->rhs.nrhs, hipMemcpyDeviceToHost, s);
         (void)hipStreamSynchronize(s);
-        dfree(Bd);
+        sfree(Bd, s);
         HOST_CALL(rc);
         HIP_CALL(e);
     }
@@ -613,15 +634,15 @@ int bem_solver_field(bem_solver *sv, const bem_cart_coord *pts, int P, bem_compl
     for (int p = 0; p < P; p++) xh[p] = make_float4(pts[p].coords[0], pts[p].coords[1], pts[p].coords[2], 0.f);
     float4 *xd = nullptr;
     float2 *od = nullptr;
-    HOST_CALL(dalloc(&xd, P));
-    HOST_CALL(dalloc(&od, (size_t)P * R));
+    HOST_CALL(salloc(&xd, P, s));
+    HOST_CALL(salloc(&od, (size_t)P * R, s));
     HIP_CALL(hipMemcpyAsync(xd, xh.data(), sizeof(float4) * P, hipMemcpyHostToDevice, s));
     int rc = field_fp32(dp, sv->k, xd, P, sv->X32, dp.ldA, R, sv->rhs, od, P, s);
     hipError_t e = hipMemcpy2DAsync(out, sizeof(bem_complex) * ldo, od, sizeof(float2) * P, sizeof(float2) * P, R,
                                     hipMemcpyDeviceToHost, s);
     (void)hipStreamSynchronize(s);
-    dfree(xd);
-    dfree(od);
+    sfree(xd, s);
+    sfree(od, s);
     HOST_CALL(rc);
     HIP_CALL(e);
     return EXIT_SUCCESS;

@@ -1,0 +1,32 @@
+// The opaque handles of bem.h, shared by the C-ABI (capi.cpp) and the
+// batch / request-server layer (serve.cpp).
+#pragma once
+
+#include <mutex>
+
+#include "bem_internal.hpp"
+
+struct bem_mesh {
+    bem::Mesh m;
+    bem::DeviceProblem dp;
+    uint64_t dp_version = ~0ull;
+};
+
+struct bem_solver {
+    bem::Mesh mesh;
+    bem::DeviceProblem dp;
+    float k = 0.f;
+    int prec = 0;
+    float2 *A32 = nullptr;
+    double2 *A64 = nullptr;
+    bem::RhsSpec rhs;
+    double2 *B64 = nullptr;
+    double2 *X64 = nullptr;
+    float2 *X32 = nullptr;
+    size_t a_cap = 0;  // elements allocated for A32 / A64
+    size_t x_cap = 0;  // elements allocated for each of B64, X64, X32
+    int xr = 0;        // columns of the last solution in X64
+    bem_solve_stats stats{};
+    hipStream_t stream = nullptr;  // each solver runs on its own stream
+    mutable std::mutex mu;         // one call at a time per solver
+};

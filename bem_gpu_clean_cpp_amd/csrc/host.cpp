@@ -116,6 +116,8 @@ int quad_init(int order) {
 }
 
 int ensure_quad() {
+    static std::mutex mu;  // solvers on several host threads may get here first together
+    std::lock_guard<std::mutex> lk(mu);
     if (config().qorder == 0) return quad_init(kQOrder);
     return EXIT_SUCCESS;
 }
@@ -151,6 +153,14 @@ int ctx_init(Context **out) {
         // default-priority stream
         HIP_CALL(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking));
         ctx.device = dev;
+        // Stream-ordered buffers (matrices, Krylov state, temporaries) come from the
+        // device's default pool; keep freed memory in the pool so the next
+        // solve's allocations do not map fresh pages.
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
     }
     *out = &ctx;
     return EXIT_SUCCESS;

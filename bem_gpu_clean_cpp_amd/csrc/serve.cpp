@@ -1,0 +1,353 @@
+// Multi-frequency batches and the request server.
+//
+// Both run several resident solvers at once, each driven by its own host
+// thread on its own HIP stream.  The assembly of one problem (VALU-bound) then
+// overlaps the solve of another (HBM-bound matrix products plus host-side
+// Givens rotations), and the host round trips of one solver's Krylov loop hide
+// behind the device work of the others.
+//
+// bem_batch_solve: one mesh, many wavenumbers, right-hand sides shared by all
+// frequencies or given per frequency; the workers keep their mesh and matrix
+// buffers for all frequencies they take.
+//
+// bem_server: a request loop with continuous admission.  Requests (own mesh,
+// wavenumber, right-hand sides) queue in arrival order; a free slot takes the
+// head of the queue as soon as its HBM footprint fits in what the running
+// requests leave of the budget, so requests that arrive while others run start
+// without waiting for a batch boundary.  Every slot frees its device buffers
+// (stream-ordered, into the memory pool) when its request ends, so the budget
+// accounts for exactly the running requests.  Results do not depend on what
+// else runs: every kernel's launch shape and reduction order follow from the
+// request's own sizes.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "bem_internal.hpp"
+#include "handles.hpp"
+
+using namespace bem;
+
+namespace {
+
+double now_ms(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+bem_solve_params default_params() {
+    bem_solve_params p;
+    p.method = BEM_SOLVE_GMRES;
+    p.restart = 30;
+    p.tol = 1e-6f;
+    p.maxit = 3000;
+    p.refine_tol = 1e-8;
+    p.max_refine = 50;
+    return p;
+}
+
+// one solve on a resident solver: assemble, right-hand sides, solve, copy out
+int run_one(bem_solver *sv, float k, int precision, int kind, const float *params, int nrhs, float amp,
+            const bem_solve_params &sp, bem_zcomplex *x, int ldx, bem_solve_stats *st) {
+    HOST_CALL(bem_solver_assemble(sv, k, precision));
+    HOST_CALL(bem_solver_set_rhs(sv, kind, params, nrhs, amp));
+    HOST_CALL(bem_solver_solve(sv, sp.method, sp.restart, sp.tol, sp.maxit, sp.refine_tol, sp.max_refine, st));
+    if (x) HOST_CALL(bem_solver_get_solution_z(sv, x, ldx));
+    return EXIT_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int method, int restart) {
+    if (M < N || N <= 0 || E <= 0 || nrhs <= 0) return -1.0;
+    const size_t ld = ((size_t)M + 63) / 64 * 64;
+    const size_t ldv = ((size_t)std::max(M, N) + 63) / 64 * 64;
+    const size_t R = (size_t)nrhs;
+    const size_t ea = precision == BEM_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+    size_t b = device_problem_bytes(M, N, E);
+    b += ea * ld * N;                                               // A
+    b += (2 * sizeof(double2) + sizeof(float2)) * ld * R;           // B, X (FP64), X (FP32)
+    if (method == BEM_SOLVE_QR) {
+        b += ea * ld * N;                                           // factored copy of A
+        b += sizeof(float2) * 32 * (ld + 32 * ((N + 31) / 32));     // block reflectors, T factors
+        b += 2 * sizeof(double2) * ld + ws_gemv_n(M, N, 1);
+    } else {
+        const size_t K = (size_t)std::max(1, std::min(restart, N));
+        b += sizeof(double2) * ldv * (K + 1) * R;                   // Krylov basis
+        b += (3 * sizeof(double2) + 3 * sizeof(float2)) * ldv * R;  // work vectors
+        b += std::max(std::max(ws_gemv_n(M, N, 1), ws_gemm(M, N, nrhs)), ws_mres(M, N, nrhs));
+        b += sizeof(double2) * (K + 2) * (2 + ((size_t)std::max(M, N) + 1023) / 1024) * R;
+    }
+    b += sizeof(float2) * R * ld;  // right-hand-side partials
+    return (double)b * 1.05 + (1 << 20);
+}
+
+// ---------------------------------------------------------- batch solve ---
+int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, int rhs_kind, const float *params,
+                    int nrhs, int params_per_freq, float amp, const bem_solve_params *sp, int workers,
+                    bem_zcomplex *X, int ldx, bem_solve_stats *stats) {
+    if (!m || !k || nk <= 0 || !params || nrhs <= 0) BEM_FAIL("bad batch arguments");
+    if (X && ldx < (int)m->m.N()) BEM_FAIL("ldx must be >= N");
+    const bem_solve_params p = sp ? *sp : default_params();
+    workers = std::max(1, std::min(std::min(workers, nk), 8));
+    std::vector<bem_solver *> sv(workers, nullptr);
+    for (int w = 0; w < workers; w++) {
+        sv[w] = bem_solver_create(m);
+        if (!sv[w]) {
+            const std::string e = last_error();
+            for (auto *s : sv) bem_solver_destroy(s);
+            BEM_FAIL("batch: solver creation failed: %s", e.c_str());
+        }
+    }
+    std::atomic<int> next{0};
+    std::atomic<int> failed{0};
+    std::string first_error;
+    std::mutex emu;
+    auto work = [&](int w) {
+        for (;;) {
+            const int f = next.fetch_add(1);
+            if (f >= nk || failed.load()) break;
+            bem_zcomplex *xf = X ? X + (size_t)f * nrhs * ldx : nullptr;
+            bem_solve_stats st{};
+            const float *pf = params_per_freq ? params + (size_t)f * 3 * nrhs : params;
+            if (run_one(sv[w], k[f], precision, rhs_kind, pf, nrhs, amp, p, xf, ldx, &st) != EXIT_SUCCESS) {
+                std::lock_guard<std::mutex> lk(emu);
+                if (!failed.exchange(1)) first_error = last_error();
+                break;
+            }
+            if (stats) stats[f] = st;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < workers; w++) th.emplace_back(work, w);
+    work(0);
+    for (auto &t : th) t.join();
+    for (auto *s : sv) bem_solver_destroy(s);
+    if (failed.load()) BEM_FAIL("batch: %s", first_error.c_str());
+    return EXIT_SUCCESS;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------- request server ---
+namespace {
+
+struct Request {
+    long id = 0;
+    bem_mesh mesh;
+    float k = 0.f;
+    int precision = BEM_PREC_FP32;
+    int kind = BEM_RHS_PLANE;
+    std::vector<float> params;
+    int nrhs = 0;
+    float amp = 1.f;
+    bem_solve_params sp{};
+    double footprint = 0;
+    int state = 0;  // 0 queued, 1 running, 2 done, 3 failed, 4 cancelled
+    int slot = -1;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    std::vector<bem_zcomplex> x;
+    bem_solve_stats st{};
+    std::string error;
+};
+
+}  // namespace
+
+struct bem_server {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::unordered_map<long, std::shared_ptr<Request>> reqs;  // until collected by bem_server_wait
+    double budget = 0;
+    double in_use = 0;
+    double peak_in_use = 0;
+    long next_id = 1;
+    int running = 0;
+    int max_running = 0;
+    bool stop = false;
+    std::vector<std::thread> threads;
+    std::vector<bem_solver *> slots;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    void worker(int i) {
+        bem_solver *&sv = slots[i];
+        for (;;) {
+            std::shared_ptr<Request> r;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] {
+                    return stop || (!queue.empty() && in_use + queue.front()->footprint <= budget);
+                });
+                if (stop) return;
+                r = queue.front();
+                queue.pop_front();
+                in_use += r->footprint;
+                peak_in_use = std::max(peak_in_use, in_use);
+                running++;
+                max_running = std::max(max_running, running);
+                r->state = 1;
+                r->slot = i;
+                r->t_start = now_ms(t0);
+            }
+            int rc = EXIT_SUCCESS;
+            if (!sv) {
+                sv = bem_solver_create(&r->mesh);
+                if (!sv) rc = EXIT_FAILURE;
+            } else {
+                rc = bem_solver_set_mesh(sv, &r->mesh);
+            }
+            if (rc == EXIT_SUCCESS) {
+                r->x.assign((size_t)r->mesh.m.N() * r->nrhs, bem_zcomplex{0, 0});
+                rc = run_one(sv, r->k, r->precision, r->kind, r->params.data(), r->nrhs, r->amp, r->sp, r->x.data(),
+                             r->mesh.m.N(), &r->st);
+            }
+            if (rc != EXIT_SUCCESS) r->error = last_error();
+            if (sv) (void)bem_solver_release(sv);  // the footprint goes back to the budget
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                in_use -= r->footprint;
+                running--;
+                r->t_end = now_ms(t0);
+                r->state = rc == EXIT_SUCCESS ? 2 : 3;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+bem_server *bem_server_create(int slots, double hbm_budget_bytes) {
+    Context *c = nullptr;
+    if (ctx_init(&c) != EXIT_SUCCESS) return nullptr;
+    if (slots <= 0 || slots > 16) {
+        set_error(__FILE__, __LINE__, "slots must be in [1, 16]");
+        return nullptr;
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        set_error(__FILE__, __LINE__, "hipMemGetInfo failed");
+        return nullptr;
+    }
+    auto *s = new bem_server();
+    s->budget = hbm_budget_bytes > 0 ? std::min(hbm_budget_bytes, 0.95 * (double)free_b) : 0.9 * (double)free_b;
+    s->slots.assign(slots, nullptr);
+    for (int i = 0; i < slots; i++) s->threads.emplace_back(&bem_server::worker, s, i);
+    return s;
+}
+
+long bem_server_submit(bem_server *s, const bem_mesh *m, float k, int precision, int rhs_kind, const float *params,
+                       int nrhs, float amp, const bem_solve_params *sp) {
+    if (!s || !m || !params || nrhs <= 0 || nrhs > kMaxRhs) {
+        set_error(__FILE__, __LINE__, "bad request arguments (nrhs must be in [1, %d])", kMaxRhs);
+        return -1;
+    }
+    if (precision != BEM_PREC_FP32 && precision != BEM_PREC_FP64) {
+        set_error(__FILE__, __LINE__, "precision must be 32 or 64");
+        return -1;
+    }
+    if (validate_mesh(m->m) != EXIT_SUCCESS) return -1;
+    auto r = std::make_shared<Request>();
+    r->mesh.m = m->m;
+    r->k = k;
+    r->precision = precision;
+    r->kind = rhs_kind;
+    r->params.assign(params, params + 3 * (size_t)nrhs);
+    r->nrhs = nrhs;
+    r->amp = amp;
+    r->sp = sp ? *sp : default_params();
+    const int N = m->m.N(), M = N + m->m.C(), E = m->m.E();
+    r->footprint = bem_request_footprint(M, N, E, nrhs, precision, r->sp.method, r->sp.restart);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->footprint > s->budget) {
+            set_error(__FILE__, __LINE__, "request needs %.3g bytes of HBM, more than the budget %.3g", r->footprint,
+                      s->budget);
+            return -1;
+        }
+        r->id = s->next_id++;
+        r->t_submit = now_ms(s->t0);
+        s->reqs[r->id] = r;
+        s->queue.push_back(r);
+    }
+    s->cv_work.notify_all();
+    return r->id;
+}
+
+int bem_server_wait(bem_server *s, long id, bem_zcomplex *x, int ldx, bem_request_info *info) {
+    if (!s) BEM_FAIL("null server");
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) BEM_FAIL("unknown request %ld", id);
+        r = it->second;
+        s->cv_done.wait(lk, [&] { return r->state >= 2; });
+        s->reqs.erase(it);
+    }
+    const int N = r->mesh.m.N();
+    if (info) {
+        info->id = r->id;
+        info->status = r->state == 2 ? 0 : 1;
+        info->slot = r->slot;
+        info->N = N;
+        info->M = N + r->mesh.m.C();
+        info->nrhs = r->nrhs;
+        info->footprint_bytes = r->footprint;
+        info->submit_ms = r->t_submit;
+        info->start_ms = r->t_start;
+        info->end_ms = r->t_end;
+        info->solve = r->st;
+    }
+    if (r->state != 2) BEM_FAIL("request %ld failed: %s", id, r->error.c_str());
+    if (x) {
+        if (ldx < N) BEM_FAIL("ldx must be >= N (%d)", N);
+        for (int q = 0; q < r->nrhs; q++)
+            std::copy(r->x.begin() + (size_t)q * N, r->x.begin() + (size_t)(q + 1) * N, x + (size_t)q * ldx);
+    }
+    return EXIT_SUCCESS;
+}
+
+int bem_server_status(bem_server *s, bem_server_info *info) {
+    if (!s || !info) BEM_FAIL("null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    info->queued = (int)s->queue.size();
+    info->running = s->running;
+    info->max_running = s->max_running;
+    info->slots = (int)s->slots.size();
+    info->budget_bytes = s->budget;
+    info->in_use_bytes = s->in_use;
+    info->peak_in_use_bytes = s->peak_in_use;
+    return EXIT_SUCCESS;
+}
+
+void bem_server_destroy(bem_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        // requests still queued are cancelled; running ones finish first
+        for (auto &r : s->queue) r->state = 4;
+        s->queue.clear();
+    }
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return s->running == 0; });
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    for (auto &t : s->threads) t.join();
+    for (auto *sv : s->slots) bem_solver_destroy(sv);
+    delete s;
+}
+
+}  // extern "C"

@@ -200,6 +200,73 @@ int bem_solver_field(bem_solver *s, const bem_cart_coord *pts, int P, bem_comple
 int bem_solver_dims(const bem_solver *s, int *M, int *N, int *nrhs);
 /* Last solve statistics. */
 int bem_solver_stats(const bem_solver *s, bem_solve_stats *stats);
+/* Point the solver at another mesh, keeping its stream; device buffers are
+ * reallocated (stream-ordered) to the new sizes. */
+int bem_solver_set_mesh(bem_solver *s, const bem_mesh *m);
+/* Free every device buffer of the solver but keep its stream; a later
+ * bem_solver_set_mesh makes it usable again. */
+int bem_solver_release(bem_solver *s);
+
+/* ------------------------------------- batches and serving (extension) --- */
+/* Solve settings shared by the batch and server calls (the arguments of
+ * bem_solver_solve). */
+typedef struct bem_solve_params {
+    int method;         /* BEM_SOLVE_GMRES or BEM_SOLVE_QR */
+    int restart;
+    float tol;
+    int maxit;
+    double refine_tol;
+    int max_refine;
+} bem_solve_params;
+
+/* One mesh at nk wavenumbers k[f], each with nrhs right-hand sides: the same
+ * 3 * nrhs params for every frequency, or (params_per_freq != 0) nk blocks of
+ * 3 * nrhs, one per frequency.  X (may be null) receives N x (nrhs * nk) complex128 solutions, column
+ * f * nrhs + r at X + (f * nrhs + r) * ldx; stats (nk entries, may be null)
+ * each frequency's solve statistics.  `workers` resident solvers (<= 8), each
+ * on its own stream and host thread, take frequencies in turn.  sp == NULL:
+ * GMRES(30, 1e-6) refined to 1e-8.  Results equal those of one solver running
+ * the frequencies one by one. */
+int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, int rhs_kind, const float *params,
+                    int nrhs, int params_per_freq, float amp, const bem_solve_params *sp, int workers,
+                    bem_zcomplex *X, int ldx, bem_solve_stats *stats);
+
+/* Request server: `slots` resident solvers (<= 16) on their own streams take
+ * queued requests in arrival order as soon as a slot is free and the request's
+ * HBM footprint fits in the budget left by the running ones
+ * (hbm_budget_bytes <= 0: 90% of the free device memory). */
+typedef struct bem_server bem_server;
+
+typedef struct bem_request_info {
+    long id;
+    int status;                 /* 0 solved, 1 failed or cancelled */
+    int slot;                   /* slot that ran it */
+    int N, M, nrhs;
+    double footprint_bytes;     /* HBM admitted for it */
+    double submit_ms, start_ms, end_ms;  /* ms since the server was created */
+    bem_solve_stats solve;
+} bem_request_info;
+
+typedef struct bem_server_info {
+    int queued, running, max_running, slots;
+    double budget_bytes, in_use_bytes, peak_in_use_bytes;
+} bem_server_info;
+
+bem_server *bem_server_create(int slots, double hbm_budget_bytes);
+/* Queue one request: the mesh is copied (the caller may free it at once).
+ * Returns the request id (> 0), or -1 (bad arguments, or a footprint above
+ * the budget). */
+long bem_server_submit(bem_server *s, const bem_mesh *m, float k, int precision, int rhs_kind, const float *params,
+                       int nrhs, float amp, const bem_solve_params *sp);
+/* Block until request `id` ends; copy its N x nrhs solution (x may be null)
+ * and its record, and forget it. */
+int bem_server_wait(bem_server *s, long id, bem_zcomplex *x, int ldx, bem_request_info *info);
+int bem_server_status(bem_server *s, bem_server_info *info);
+/* Cancel queued requests, let running ones finish, free everything. */
+void bem_server_destroy(bem_server *s);
+/* HBM bytes a request of these sizes is admitted with (upper bound of what
+ * its solve allocates). */
+double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int method, int restart);
 
 /* ----------------------------------------------------- timing (extension) --- */
 /* HIP-event timing of the library's kernels on the stream they run on.

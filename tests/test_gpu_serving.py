@@ -1,0 +1,145 @@
+"""Multi-frequency batches and the request server.
+
+Checks:
+  * a multi-frequency batch on several concurrent workers returns exactly the
+    bits one solver returns running the frequencies one by one, and matches
+    the FP64 oracle at the 1e-5 bar;
+  * a resident solver re-pointed at another mesh returns exactly what a fresh
+    solver returns;
+  * every request solved in a mixed stream on 4 concurrent slots returns
+    exactly the bits it returns alone (kernel shapes and reduction orders
+    depend only on the request's own sizes), and the slots did overlap;
+  * admission never lets the admitted footprints exceed the HBM budget, and a
+    request larger than the budget is refused at submission.
+"""
+import numpy as np
+import pytest
+
+import bem_gpu_clean_cpp_amd as bem
+import bemcases as C
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+RHO, SPEED = float(np.float32(1.21)), float(np.float32(343.0))
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _setup():
+    bem.quad_init(6)
+    O.quad_init(6)
+    bem.set_medium(RHO, SPEED)
+    yield
+    bem.set_medium(1.2041, 343.21)
+
+
+def _ellipsoid(s, seed, nchief=8):
+    m = bem.Mesh().ellipsoid(s, (0.075, 0.09, 0.11), nbumps=8, lmax=6, amp=0.05, seed=seed)
+    m.set_chief(C.ball_points(seed + 1000, nchief, 0.03))
+    return m
+
+
+def _sphere(s, seed, r=0.1, nchief=8):
+    m = bem.Mesh().icosphere(s, r)
+    m.set_chief(C.ball_points(seed, nchief, r / 2))
+    return m
+
+
+def test_batch_equals_sequential_and_oracle():
+    m = _ellipsoid(3, 7)
+    freqs = 100.0 * 80.0 ** (np.arange(6) / 5.0)
+    ks = (2 * np.pi * freqs / SPEED).astype(np.float32)
+    d = C.sphere_points(21, 5, 1.0)
+    X, stats = bem.batch_solve(m, ks, bem.RHS_PLANE, d, workers=3, refine_tol=1e-9)
+    assert X.shape == (m.num_pnts, 5, 6)
+    assert all(s["converged"] == 1 for s in stats), stats
+    s = bem.Solver(m)
+    for f, k in enumerate(ks):
+        s.assemble(float(k), bem.PREC_FP32)
+        s.set_rhs(bem.RHS_PLANE, d, 1.0)
+        s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-9, max_refine=50)
+        assert np.array_equal(s.solution(), X[:, :, f]), f
+    # two frequencies against the FP64 oracle
+    p, e, bc = C.mesh_arrays(m)
+    pts = np.vstack([p, m.chief()]).astype(np.float64)
+    for f in (0, 5):
+        k = float(ks[f])
+        binit = np.exp(-1j * k * pts @ np.asarray(d, np.float64).T)
+        A, B = O.assemble(k, pts, p.shape[0], e, bc, binit=binit, dens=RHO, speed=SPEED)
+        xo = np.linalg.lstsq(A, B, rcond=None)[0]
+        err = np.linalg.norm(X[:, :, f] - xo, axis=0) / np.linalg.norm(xo, axis=0)
+        assert err.max() < 1e-5, (f, err)
+
+
+def test_solver_set_mesh_matches_fresh_solver():
+    m1, m2 = _sphere(3, 5), _ellipsoid(2, 9)
+    d = C.sphere_points(4, 3, 1.0)
+    s = bem.Solver(m1)
+    s.assemble(20.0).set_rhs(bem.RHS_PLANE, d).solve()
+    s.release()
+    s.set_mesh(m2)
+    s.assemble(25.0).set_rhs(bem.RHS_PLANE, d).solve()
+    x_re = s.solution()
+    f = bem.Solver(m2)
+    f.assemble(25.0).set_rhs(bem.RHS_PLANE, d).solve()
+    assert np.array_equal(x_re, f.solution())
+
+
+def _requests(seed=3, n=10):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        s = int(rng.integers(2, 4)) if i else 4
+        m = _sphere(s, 100 + i) if rng.random() < 0.5 else _ellipsoid(s, 200 + i)
+        ka = rng.uniform(0.5, 10.0)
+        k = float(ka / 0.1)
+        R = int(rng.integers(1, 9))
+        if rng.random() < 0.5:
+            kind, params = bem.RHS_PLANE, C.sphere_points(300 + i, R, 1.0)
+        else:
+            kind, params = bem.RHS_POINT, C.ball_points(400 + i, R, 0.02)
+        out.append((m, k, kind, params))
+    return out
+
+
+def test_server_alone_equals_mix():
+    reqs = _requests()
+    alone = []
+    srv = bem.Server(slots=1)
+    for m, k, kind, params in reqs:
+        x, info = srv.wait(srv.submit(m, k, kind, params, refine_tol=1e-9))
+        assert info["status"] == 0 and info["solve"]["converged"] == 1, info
+        alone.append(x)
+    srv.close()
+    srv = bem.Server(slots=4)
+    ids = [srv.submit(m, k, kind, params, refine_tol=1e-9) for m, k, kind, params in reqs]
+    mix = [srv.wait(i) for i in ids]
+    st = srv.status()
+    srv.close()
+    for q, ((x, info), xa) in enumerate(zip(mix, alone)):
+        assert info["status"] == 0, info
+        assert np.array_equal(x, xa), q
+    assert st["max_running"] > 1, st
+    assert st["peak_in_use_bytes"] <= st["budget_bytes"]
+
+
+def test_server_admission_budget():
+    reqs = _requests(seed=8, n=6)
+    fps = []
+    for m, k, kind, params in reqs:
+        M, N, E = m.num_pnts + m.num_chief, m.num_pnts, m.num_elems
+        fps.append(bem.request_footprint(M, N, E, np.asarray(params).reshape(-1, 3).shape[0]))
+    budget = 1.2 * max(fps)
+    srv = bem.Server(slots=4, hbm_budget_bytes=budget)
+    ids = [srv.submit(m, k, kind, params) for m, k, kind, params in reqs]
+    res = [srv.wait(i)[1] for i in ids]
+    st = srv.status()
+    assert all(r["status"] == 0 for r in res)
+    assert st["peak_in_use_bytes"] <= budget
+    # a request that cannot fit is refused up front
+    big = bem.Mesh().icosphere(4, 0.1)
+    tiny = bem.Server(slots=1, hbm_budget_bytes=1e6)
+    with pytest.raises(bem.BemError):
+        tiny.submit(big, 10.0, bem.RHS_PLANE, [[0, 0, 1]])
+    tiny.close()
+    srv.close()
