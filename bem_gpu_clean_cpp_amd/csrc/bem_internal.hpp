@@ -201,13 +201,14 @@ int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long 
 int mres_f64(const double2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
              double2 *out, long ldo, int R, void *ws, hipStream_t s);
 int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
-            hipStream_t s);
+            void *ws, hipStream_t s);
 int mh_f64(const double2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
            hipStream_t s);
 
+// X holds the initial guess (x_zero: known to be zero, which saves one residual)
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
-                 double2 *X, long ldx, int R, int restart, double tol, int maxit, double outer_tol, int max_cycles,
-                 bem_solve_stats *st, hipStream_t s);
+                 double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
+                 int max_cycles, bem_solve_stats *st, hipStream_t s);
 int qr_lsq_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, hipStream_t s);
 int qr_lsq_f64(double2 *A, long lda, int M, int N, double2 *B, long ldb, int R, hipStream_t s);
 int qr_lsq_with_q_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, float2 *Q, long ldq,

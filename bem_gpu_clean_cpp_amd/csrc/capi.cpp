@@ -297,7 +297,7 @@ int bem_gmres(const bem_complex *A, const bem_complex *b, int m, int maxIter, fl
     HIP_CALL(hipMemcpyAsync(bd, hb.data(), sizeof(double2) * m, hipMemcpyHostToDevice, s));
     HIP_CALL(hipMemcpyAsync(xd, hx.data(), sizeof(double2) * m, hipMemcpyHostToDevice, s));
     const int restart = std::min(std::min(maxIter, m), 256);
-    HOST_CALL(gmres_device(Ad, nullptr, ld, m, m, bd, ld, xd, ld, 1, restart, (double)threshold, maxIter,
+    HOST_CALL(gmres_device(Ad, nullptr, ld, m, m, bd, ld, xd, ld, false, 1, restart, (double)threshold, maxIter,
                            (double)threshold, 1 << 20, nullptr, s));
     HIP_CALL(hipMemcpyAsync(hx.data(), xd, sizeof(double2) * m, hipMemcpyDeviceToHost, s));
     HIP_CALL(hipStreamSynchronize(s));
@@ -524,7 +524,8 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
         const double outer = refine_tol > 0 ? refine_tol : (double)tol;
         const int cycles = max_refine > 0 ? max_refine : (1 << 20);
         if (maxit <= 0) maxit = 100000;
-        HOST_CALL(gmres_device(sv->A32, sv->A64, ld, dp.M, dp.N, sv->B64, ld, sv->X64, ld, R, restart, (double)tol,
+        HOST_CALL(gmres_device(sv->A32, sv->A64, ld, dp.M, dp.N, sv->B64, ld, sv->X64, ld, true, R, restart,
+                               (double)tol,
                                maxit, outer, cycles, &st, s));
     } else {
         if (sv->A32) {

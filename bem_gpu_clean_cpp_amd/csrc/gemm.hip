@@ -1,19 +1,26 @@
-// Multi-right-hand-side complex matrix products on MFMA (v_mfma_f32_16x16x4f32):
-// Y = A X (M x R) and Z = A^H T (N x R), A complex64 column-major (M x N, lda).
+// Multi-right-hand-side complex matrix products on MFMA: Y = A X (M x R) and
+// Z = A^H T (N x R) for a complex64 column-major A (M x N, lda).  The
+// products run on v_mfma_f32_16x16x4f32 with FP32 operands (the Krylov
+// operator), or on v_mfma_f64_16x16x4f64 with A widened to FP64 and FP64
+// vectors (the refinement residuals, FP64 accumulation).
 //
-// The interleaved memory of A is read as a real (2M x N) matrix Ar~ whose even
+// The interleaved memory of A is read as a real (2M x N) matrix whose even
 // rows hold Re A and odd rows Im A, so one 16-byte load per lane gives 4 real
 // rows of one column and feeds 4 MFMAs (one per element).  For Y = A X the
-// products P = Ar~ Re(X) and Q = Ar~ Im(X) land in the same lane and register
+// products P = A~ Re(X) and Q = A~ Im(X) land in the same lane and register
 // for the real and imaginary row of each complex row, so Y = (P_re - Q_im,
 // P_im + Q_re) combines lane-locally.  For Z = A^H T the second operand is T's
 // own interleaved memory (and a swapped, negated copy for Im Z).  Columns (or
 // rows, for A^H) are split into chunks over the grid; chunk partials are summed
-// in a fixed order by gemm_reduce (deterministic, no atomics).
+// in a fixed order by gemm_reduce (deterministic, no atomics).  Each wave
+// holds the operands of the next 16 columns (or 32 rows) in registers while
+// it multiplies the current ones, so the HBM latency of the A stream is
+// covered by MFMA work.
 //
 // MFMA lane layout (16x16x4, one block): A operand A[i][k] from lane i + 16 k,
-// B operand B[k][j] from lane j + 16 k, result D[4 (l / 16) + v][l % 16] in
-// lane l, register v.
+// B operand B[k][j] from lane j + 16 k (both precisions); the result
+// D[4 (l / 16) + v][l % 16] (f32) or D[l / 16 + 4 v][l % 16] (f64) in lane l,
+// register v.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,79 +29,161 @@
 
 namespace bem {
 
-typedef float v4f __attribute__((ext_vector_type(4)));
+template <typename T>
+struct MF;
+template <>
+struct MF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    typedef float2 c2;
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // result row held by lane group q = lane / 16 in register v
+    static __device__ __forceinline__ int drow(int q, int v) { return 4 * q + v; }
+};
+template <>
+struct MF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    typedef double2 c2;
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int q, int v) { return q + 4 * v; }
+};
 
-__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+constexpr int kU = 4;  // MFMA steps (of 4 columns / 8 rows) per prefetched block
+
+// keep v where k holds, else zero, without a branch
+template <typename S>
+__device__ __forceinline__ S keep(bool k, S v) {
+    return k ? v : S(0);
+}
+__device__ __forceinline__ float4 keep4(bool k01, bool k23, float4 v) {
+    return make_float4(keep(k01, v.x), keep(k01, v.y), keep(k23, v.z), keep(k23, v.w));
+}
+template <typename C2>
+__device__ __forceinline__ C2 keep2(bool k, C2 v) {
+    C2 r;
+    r.x = keep(k, (decltype(r.x + 0))v.x);
+    r.y = keep(k, (decltype(r.y + 0))v.y);
+    return r;
 }
 
 // part[(chunk * R + n) * ldp + i] = sum_{j in chunk} A[i, j] X[j, n]
 // wave: 32 complex rows x (16 G) right-hand sides x one column chunk;
-// requires lda even and rows row0..row0+31 inside the allocation (lda >= roundup(M, 32)).
-template <int G>
+// requires lda even and lda >= roundup(M, 32) (rows row0..row0+31 allocated).
+// Loads are unconditional at clamped (in-bounds) addresses and masked to zero
+// afterwards, so the loop body is straight-line and the next block's loads stay
+// in flight while the current block is multiplied (ping-pong buffers).
+template <typename T, int G>
 __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A, long lda, int M, int N,
-                                                   const float2 *__restrict__ X, long ldx, int R, int rg0, int kc,
-                                                   float2 *__restrict__ part, long ldp) {
+                                                   const typename MF<T>::c2 *__restrict__ X, long ldx, int R, int rg0,
+                                                   int kc, typename MF<T>::c2 *__restrict__ part, long ldp) {
+    typedef typename MF<T>::v4 v4;
+    typedef typename MF<T>::c2 c2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row0 = (blockIdx.x * 4 + wave) * 32;
     if (row0 >= M) return;
     const int chunk = blockIdx.y;
     const int j0 = chunk * kc, j1 = min(N, j0 + kc);
     const int m = lane & 15, kk = lane >> 4;
-    v4f acc[G][2][4];
+    v4 acc[G][2][4];
 #pragma unroll
     for (int g = 0; g < G; g++)
 #pragma unroll
         for (int p = 0; p < 2; p++)
 #pragma unroll
-            for (int t = 0; t < 4; t++) acc[g][p][t] = (v4f){0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < 4; t++) acc[g][p][t] = (v4){0, 0, 0, 0};
     const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * m);
-    for (int j = j0; j < j1; j += 4) {
-        const int J = j + kk;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (J < j1) a = Ab[(size_t)J * (lda / 2)];
+    const long lda4 = lda / 2;
+    // right-hand sides past R read column R - 1 (their results are not written)
+    const c2 *Xg[G];
 #pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int n = rg0 + g * 16 + m;
-            float2 xv = make_float2(0.f, 0.f);
-            if (J < j1 && n < R) xv = X[(size_t)n * ldx + J];
-            const float av[4] = {a.x, a.y, a.z, a.w};
+    for (int g = 0; g < G; g++) Xg[g] = X + (size_t)min(rg0 + g * 16 + m, R - 1) * ldx;
+    struct Buf {
+        float4 a[kU];
+        c2 x[kU][G];
+    };
+    // full blocks: every column inside the chunk, no masking; the address is
+    // clamped only for the prefetch past the last pair of blocks
+    auto load = [&](int jb, Buf &B) {
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                acc[g][0][t] = mfma4(av[t], xv.x, acc[g][0][t]);
-                acc[g][1][t] = mfma4(av[t], xv.y, acc[g][1][t]);
-            }
+        for (int u = 0; u < kU; u++) {
+            const int Jc = min(jb + 4 * u + kk, j1 - 1);
+            B.a[u] = Ab[(size_t)Jc * lda4];
+#pragma unroll
+            for (int g = 0; g < G; g++) B.x[u][g] = Xg[g][Jc];
         }
+    };
+    auto load_masked = [&](int jb, Buf &B) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int J = jb + 4 * u + kk;
+            const bool in = J < j1;
+            const int Jc = min(J, j1 - 1);
+            B.a[u] = keep4(in, in, Ab[(size_t)Jc * lda4]);
+#pragma unroll
+            for (int g = 0; g < G; g++) B.x[u][g] = keep2(in, Xg[g][Jc]);
+        }
+    };
+    auto mul = [&](const Buf &B) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const T av[4] = {(T)B.a[u].x, (T)B.a[u].y, (T)B.a[u].z, (T)B.a[u].w};
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    acc[g][0][t] = MF<T>::mma(av[t], B.x[u][g].x, acc[g][0][t]);
+                    acc[g][1][t] = MF<T>::mma(av[t], B.x[u][g].y, acc[g][1][t]);
+                }
+        }
+    };
+    Buf b0, b1;
+    const int jpair = j0 + (j1 - j0) / (8 * kU) * (8 * kU);  // end of the paired full blocks
+    load(j0, b0);
+    for (int jb = j0; jb < jpair; jb += 8 * kU) {
+        load(jb + 4 * kU, b1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
+        mul(b0);
+        load(jb + 8 * kU, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mul(b1);
     }
-    // D_t[m'][n'] (m' = 4 (lane / 16) + v) is real row 4 m' + t of the tile:
+    for (int jb = jpair; jb < j1; jb += 4 * kU) {
+        load_masked(jb, b1);
+        mul(b1);
+    }
+    // D_t[m'][n'] (m' = drow(lane / 16, v)) is real row 4 m' + t of the tile:
     // complex row 2 m' + t / 2, real part for even t
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int n = rg0 + g * 16 + m;
         if (n >= R) continue;
-        float2 *pc = part + ((size_t)chunk * R + n) * ldp;
+        c2 *pc = part + ((size_t)chunk * R + n) * ldp;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int mp = 4 * kk + v;
+            const int mp = MF<T>::drow(kk, v);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int i = row0 + 2 * mp + h;
-                if (i < M) {
-                    const float yr = acc[g][0][2 * h][v] - acc[g][1][2 * h + 1][v];
-                    const float yi = acc[g][0][2 * h + 1][v] + acc[g][1][2 * h][v];
-                    pc[i] = make_float2(yr, yi);
-                }
+                if (i < M) pc[i] = c2{acc[g][0][2 * h][v] - acc[g][1][2 * h + 1][v],
+                                      acc[g][0][2 * h + 1][v] + acc[g][1][2 * h][v]};
             }
         }
     }
 }
 
 // part[(chunk * R + n) * ldp + j] = sum_{i in chunk} conj(A[i, j]) T[i, n]
-// wave: 16 columns of A x (16 G) right-hand sides x one row chunk (multiple of 8).
-template <int G>
+// wave: 16 columns of A x (16 G) right-hand sides x one row chunk (i0 even).
+// Rows read in pairs: row + 1 may be M, which needs lda, ldt > M for odd M
+// (the callers' leading dimensions are M rounded up to 64).
+template <typename T, int G>
 __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A, long lda, int M, int N,
-                                                   const float2 *__restrict__ T, long ldt, int R, int rg0, int kr,
-                                                   float2 *__restrict__ part, long ldp) {
+                                                   const typename MF<T>::c2 *__restrict__ Tv, long ldt, int R, int rg0,
+                                                   int kr, typename MF<T>::c2 *__restrict__ part, long ldp) {
+    typedef typename MF<T>::v4 v4;
+    typedef typename MF<T>::c2 c2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c0 = (blockIdx.x * 4 + wave) * 16;
     if (c0 >= N) return;
@@ -102,147 +191,223 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
     const int i0 = chunk * kr, i1 = min(M, i0 + kr);
     const int m = lane & 15, kk = lane >> 4;
     const int J = c0 + m;
-    v4f accR[G], accI[G];
+    v4 accR[G], accI[G];
 #pragma unroll
     for (int g = 0; g < G; g++) {
-        accR[g] = (v4f){0.f, 0.f, 0.f, 0.f};
-        accI[g] = (v4f){0.f, 0.f, 0.f, 0.f};
+        accR[g] = (v4){0, 0, 0, 0};
+        accI[g] = (v4){0, 0, 0, 0};
     }
-    for (int i = i0; i < i1; i += 8) {
-        const int row = i + 2 * kk;  // complex rows row, row + 1 of this lane
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (J < N && row < i1) {
-            a = *reinterpret_cast<const float4 *>(A + (size_t)J * lda + row);
-            if (row + 1 >= i1) {
-                a.z = 0.f;
-                a.w = 0.f;
+    // columns past N and right-hand sides past R read the last valid one
+    // (their results are not written)
+    const float2 *col = A + (size_t)min(J, N - 1) * lda;
+    const c2 *Tg[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) Tg[g] = Tv + (size_t)min(rg0 + g * 16 + m, R - 1) * ldt;
+    const int rlast = (i1 - 1) & ~1;  // last even row of the chunk
+    struct Buf {
+        float4 a[kU];
+        c2 t0[kU][G], t1[kU][G];
+    };
+    auto load = [&](int ib, Buf &B) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int rc = min(ib + 8 * u + 2 * kk, rlast);  // complex rows rc, rc + 1 of this lane
+            B.a[u] = *reinterpret_cast<const float4 *>(col + rc);
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                B.t0[u][g] = Tg[g][rc];
+                B.t1[u][g] = Tg[g][rc + 1];
             }
         }
-        const float av[4] = {a.x, a.y, a.z, a.w};
+    };
+    auto load_masked = [&](int ib, Buf &B) {
 #pragma unroll
-        for (int g = 0; g < G; g++) {
-            const int n = rg0 + g * 16 + m;
-            float4 tt = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (n < R && row < i1) {
-                tt = *reinterpret_cast<const float4 *>(T + (size_t)n * ldt + row);
-                if (row + 1 >= i1) {
-                    tt.z = 0.f;
-                    tt.w = 0.f;
+        for (int u = 0; u < kU; u++) {
+            const int row = ib + 8 * u + 2 * kk;
+            const bool in0 = row < i1, in1 = row + 1 < i1;
+            const int rc = min(row, rlast);
+            B.a[u] = keep4(in0, in1, *reinterpret_cast<const float4 *>(col + rc));
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                B.t0[u][g] = keep2(in0, Tg[g][rc]);
+                B.t1[u][g] = keep2(in1, Tg[g][rc + 1]);
+            }
+        }
+    };
+    auto mul = [&](const Buf &B) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const T av[4] = {(T)B.a[u].x, (T)B.a[u].y, (T)B.a[u].z, (T)B.a[u].w};
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                // Re Z: sum Ar Tr + Ai Ti ; Im Z: sum Ar Ti - Ai Tr
+                const T br[4] = {B.t0[u][g].x, B.t0[u][g].y, B.t1[u][g].x, B.t1[u][g].y};
+                const T bi[4] = {B.t0[u][g].y, -B.t0[u][g].x, B.t1[u][g].y, -B.t1[u][g].x};
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    accR[g] = MF<T>::mma(av[t], br[t], accR[g]);
+                    accI[g] = MF<T>::mma(av[t], bi[t], accI[g]);
                 }
             }
-            // Re Z: sum Ar Tr + Ai Ti ; Im Z: sum Ar Ti - Ai Tr
-            const float br[4] = {tt.x, tt.y, tt.z, tt.w};
-            const float bi[4] = {tt.y, -tt.x, tt.w, -tt.z};
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                accR[g] = mfma4(av[t], br[t], accR[g]);
-                accI[g] = mfma4(av[t], bi[t], accI[g]);
-            }
         }
+    };
+    Buf b0, b1;
+    const int ipair = i0 + (i1 - i0) / (16 * kU) * (16 * kU);  // end of the paired full blocks
+    load(i0, b0);
+    for (int ib = i0; ib < ipair; ib += 16 * kU) {
+        load(ib + 8 * kU, b1);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
+        mul(b0);
+        load(ib + 16 * kU, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mul(b1);
+    }
+    for (int ib = ipair; ib < i1; ib += 8 * kU) {
+        load_masked(ib, b1);
+        mul(b1);
     }
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int n = rg0 + g * 16 + m;
         if (n >= R) continue;
-        float2 *pc = part + ((size_t)chunk * R + n) * ldp;
+        c2 *pc = part + ((size_t)chunk * R + n) * ldp;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int j = c0 + 4 * kk + v;
-            if (j < N) pc[j] = make_float2(accR[g][v], accI[g][v]);
+            const int j = c0 + MF<T>::drow(kk, v);
+            if (j < N) pc[j] = c2{accR[g][v], accI[g][v]};
         }
     }
 }
 
-// Y[n][i] = sum_chunks part[(chunk * R + n) * ldp + i]  (fixed order)
-__global__ __launch_bounds__(256) void gemm_reduce(const float2 *__restrict__ part, long ldp, int nchunks, int R,
-                                                   int M, float2 *__restrict__ Y, long ldy) {
+// Y[n][i] = (b[n][i] -) sum_chunks part[(chunk * R + n) * ldp + i]  (fixed order)
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_reduce(const typename MF<T>::c2 *__restrict__ part, long ldp, int nchunks,
+                                                   int R, int M, const typename MF<T>::c2 *__restrict__ b, long ldb,
+                                                   typename MF<T>::c2 *__restrict__ Y, long ldy) {
+    typedef typename MF<T>::c2 C2;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int n = blockIdx.y;
     if (i >= M) return;
-    float sr = 0.f, si = 0.f;
+    T sr = 0, si = 0;
     for (int c = 0; c < nchunks; c++) {
-        const float2 p = part[((size_t)c * R + n) * ldp + i];
+        const C2 p = part[((size_t)c * R + n) * ldp + i];
         sr += p.x;
         si += p.y;
     }
-    Y[(size_t)n * ldy + i] = make_float2(sr, si);
+    if (b) {
+        const C2 bb = b[(size_t)n * ldb + i];
+        Y[(size_t)n * ldy + i] = C2{bb.x - sr, bb.y - si};
+    } else {
+        Y[(size_t)n * ldy + i] = C2{sr, si};
+    }
 }
 
 static void gemm_n_shape(long M, long N, int &nch, int &kc) {
     const long tiles = (M + 31) / 32;
     long target = std::max(1L, 4096 / std::max(1L, tiles));
-    kc = (int)std::max(16L, ((N + target - 1) / target + 3) / 4 * 4);
+    kc = (int)std::max(32L, ((N + target - 1) / target + 31) / 32 * 32);
     nch = (int)((N + kc - 1) / kc);
 }
 
 static void gemm_h_shape(long M, long N, int &nch, int &kr) {
     const long tiles = (N + 15) / 16;
     long target = std::max(1L, 4096 / std::max(1L, tiles));
-    kr = (int)std::max(64L, ((M + target - 1) / target + 7) / 8 * 8);
+    kr = (int)std::max(64L, ((M + target - 1) / target + 63) / 64 * 64);
     nch = (int)((M + kr - 1) / kr);
 }
 
+// partials of either product at FP64 for R right-hand sides
 size_t ws_gemm(long M, long N, int R) {
     int a, b, c, d;
     gemm_n_shape(M, N, a, b);
     gemm_h_shape(M, N, c, d);
     const long ldp = (std::max(M, N) + 63) / 64 * 64;
-    return (size_t)std::max(a, c) * (size_t)ldp * (size_t)std::max(1, R) * sizeof(float2);
+    return (size_t)std::max(a, c) * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
+}
+
+template <typename T>
+static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *X, long ldx,
+                       const typename MF<T>::c2 *b, long ldb, typename MF<T>::c2 *Y, long ldy, int R, void *ws,
+                       const char *name, hipStream_t s) {
+    if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
+    if (lda % 2 || lda < (M + 31) / 32 * 32) BEM_FAIL("gemm_n needs an even lda >= roundup(M, 32)");
+    int nch, kc;
+    gemm_n_shape(M, N, nch, kc);
+    const long ldp = (M + 63) / 64 * 64;
+    auto *part = (typename MF<T>::c2 *)ws;
+    TimedRegion tr;
+    timing_begin(tr, name, s);
+    const unsigned gx = (unsigned)((M + 127) / 128);
+    for (int rg = 0; rg < R; rg += 32) {
+        dim3 grid(gx, nch);
+        if (R - rg > 16)
+            gemm_n_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
+        else
+            gemm_n_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
+        HIP_CALL(hipGetLastError());
+    }
+    timing_end(tr, s);
+    dim3 g2((unsigned)((M + 255) / 256), R);
+    gemm_reduce<T><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)M, b, ldb, Y, ldy);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
+template <typename T>
+static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *Tv, long ldt,
+                       typename MF<T>::c2 *Z, long ldz, int R, void *ws, const char *name, hipStream_t s) {
+    if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
+    if (lda % 2) BEM_FAIL("gemm_h needs an even lda");
+    int nch, kr;
+    gemm_h_shape(M, N, nch, kr);
+    const long ldp = (N + 63) / 64 * 64;
+    auto *part = (typename MF<T>::c2 *)ws;
+    TimedRegion tr;
+    timing_begin(tr, name, s);
+    const unsigned gx = (unsigned)((N + 63) / 64);
+    for (int rg = 0; rg < R; rg += 32) {
+        dim3 grid(gx, nch);
+        if (R - rg > 16)
+            gemm_h_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+        else
+            gemm_h_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+        HIP_CALL(hipGetLastError());
+    }
+    timing_end(tr, s);
+    dim3 g2((unsigned)((N + 255) / 256), R);
+    gemm_reduce<T><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)N, nullptr, 0, Z, ldz);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
 }
 
 // Y (M x R, ldy) = A X ; ws >= ws_gemm(M, N, R)
 int gemm_n_f32(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, float2 *Y, long ldy, int R,
                void *ws, hipStream_t s) {
-    if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
-    if (lda % 2 || lda < (M + 31) / 32 * 32) BEM_FAIL("gemm_n_f32 needs an even lda >= roundup(M, 32)");
-    int nch, kc;
-    gemm_n_shape(M, N, nch, kc);
-    const long ldp = (M + 63) / 64 * 64;
-    float2 *part = (float2 *)ws;
-    TimedRegion tr;
-    timing_begin(tr, "gemm_n_f32", s);
-    const unsigned gx = (unsigned)((M + 127) / 128);
-    for (int rg = 0; rg < R; rg += 32) {
-        dim3 grid(gx, nch);
-        if (R - rg > 16) gemm_n_mfma<2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
-        else gemm_n_mfma<1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
-        HIP_CALL(hipGetLastError());
-    }
-    timing_end(tr, s);
-    dim3 g2((unsigned)((M + 255) / 256), R);
-    gemm_reduce<<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)M, Y, ldy);
-    HIP_CALL(hipGetLastError());
-    return EXIT_SUCCESS;
+    return gemm_n_impl<float>(A, lda, M, N, X, ldx, nullptr, 0, Y, ldy, R, ws, "gemm_n_f32", s);
 }
 
-// Z (N x R, ldz) = A^H T, T: M x R (ldt even); ws >= ws_gemm(M, N, R)
+// Z (N x R, ldz) = A^H T, T: M x R (ldt); ws >= ws_gemm(M, N, R)
 int gemm_h_f32(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R,
                void *ws, hipStream_t s) {
-    if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
-    if (lda % 2 || ldt % 2) BEM_FAIL("gemm_h_f32 needs even lda and ldt");
-    int nch, kr;
-    gemm_h_shape(M, N, nch, kr);
-    const long ldp = (N + 63) / 64 * 64;
-    float2 *part = (float2 *)ws;
-    TimedRegion tr;
-    timing_begin(tr, "gemm_h_f32", s);
-    const unsigned gx = (unsigned)((N + 63) / 64);
-    for (int rg = 0; rg < R; rg += 32) {
-        dim3 grid(gx, nch);
-        if (R - rg > 16) gemm_h_mfma<2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, T, ldt, R, rg, kr, part, ldp);
-        else gemm_h_mfma<1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, T, ldt, R, rg, kr, part, ldp);
-        HIP_CALL(hipGetLastError());
-    }
-    timing_end(tr, s);
-    dim3 g2((unsigned)((N + 255) / 256), R);
-    gemm_reduce<<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)N, Z, ldz);
-    HIP_CALL(hipGetLastError());
-    return EXIT_SUCCESS;
+    return gemm_h_impl<float>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "gemm_h_f32", s);
 }
 
-// ------------------------------------------------- FP64-accumulated residuals ---
-// The refinement residuals for many right-hand sides: FP32 (or FP64) A, FP64
-// vectors, FP64 accumulation.  VALU kernels, 8 right-hand sides per pass.
+// out (M x R) = b - A X (b == nullptr: out = A X): FP32 A widened to FP64,
+// FP64 vectors, FP64 MFMA accumulation; ws >= ws_gemm(M, N, R)
+int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
+              double2 *out, long ldo, int R, void *ws, hipStream_t s) {
+    return gemm_n_impl<double>(A, lda, M, N, X, ldx, b, ldb, out, ldo, R, ws, "mres_f64acc", s);
+}
+
+// Z (N x R) = A^H T with FP64 accumulation; ws >= ws_gemm(M, N, R)
+int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
+            void *ws, hipStream_t s) {
+    return gemm_h_impl<double>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "mh_f64acc", s);
+}
+
+// ------------------------------------------------- FP64-matrix residuals ---
+// Residuals and adjoint products for an FP64 matrix (the all-FP64 path) and
+// several right-hand sides: VALU FP64 kernels, 8 right-hand sides per pass.
 template <typename TA>
 struct A2T;
 template <>
@@ -404,17 +569,9 @@ static int mh_impl(const typename A2T<TA>::c2 *A, long lda, long M, long N, cons
     return EXIT_SUCCESS;
 }
 
-int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
-              double2 *out, long ldo, int R, void *ws, hipStream_t s) {
-    return mres_impl<float>(A, lda, M, N, X, ldx, b, ldb, out, ldo, R, ws, s);
-}
 int mres_f64(const double2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
              double2 *out, long ldo, int R, void *ws, hipStream_t s) {
     return mres_impl<double>(A, lda, M, N, X, ldx, b, ldb, out, ldo, R, ws, s);
-}
-int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
-            hipStream_t s) {
-    return mh_impl<float>(A, lda, M, N, T, ldt, Z, ldz, R, s);
 }
 int mh_f64(const double2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
            hipStream_t s) {

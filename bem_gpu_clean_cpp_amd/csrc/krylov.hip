@@ -245,6 +245,21 @@ struct Operator {
         }
         return EXIT_SUCCESS;
     }
+    // r <- the residual at x = 0 (square: b; normal equations: A^H b)
+    int rhs(KrylovWS &ws, const double2 *B, long ldb, double2 *r, hipStream_t s) const {
+        const int R = ws.R;
+        if (!normal()) {
+            HIP_CALL(hipMemcpy2DAsync(r, sizeof(double2) * ws.ldv, B, sizeof(double2) * ldb, sizeof(double2) * N, R,
+                                      hipMemcpyDeviceToDevice, s));
+        } else if (A32) {
+            if (R == 1) HOST_CALL(gemv_h_f32a(A32, lda, M, N, B, 0, r, 0, 1, s));
+            else HOST_CALL(mh_f32a(A32, lda, M, N, B, ldb, r, ws.ldv, R, ws.gws, s));
+        } else {
+            if (R == 1) HOST_CALL(gemv_h_f64(A64, lda, M, N, B, 0, r, 0, 1, s));
+            else HOST_CALL(mh_f64(A64, lda, M, N, B, ldb, r, ws.ldv, R, s));
+        }
+        return EXIT_SUCCESS;
+    }
     // r <- rhs - op(x) in FP64 for every system: square b - A x; normal A^H (b - A x)
     int residual(KrylovWS &ws, const double2 *B, long ldb, const double2 *X, long ldx, double2 *r,
                  hipStream_t s) const {
@@ -261,7 +276,7 @@ struct Operator {
                 HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, r, ws.ldv, R, ws.gws, s));
             } else {
                 HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, ws.t64, ws.ldv, R, ws.gws, s));
-                HOST_CALL(mh_f32a(A32, lda, M, N, ws.t64, ws.ldv, r, ws.ldv, R, s));
+                HOST_CALL(mh_f32a(A32, lda, M, N, ws.t64, ws.ldv, r, ws.ldv, R, ws.gws, s));
             }
         } else if (R == 1) {
             if (!normal()) {
@@ -291,8 +306,8 @@ struct Operator {
 // FP64 residual is below `outer_tol` relative to ||rhs|| (or maxit / max_cycles).
 // B: M x R (ldb), X: N x R (ldx, initial guess on entry).
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
-                 double2 *X, long ldx, int R, int restart, double tol, int maxit, double outer_tol, int max_cycles,
-                 bem_solve_stats *st, hipStream_t s) {
+                 double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
+                 int max_cycles, bem_solve_stats *st, hipStream_t s) {
     if (M < N) BEM_FAIL("GMRES needs M >= N (got %d x %d)", M, N);
     if (R <= 0) return EXIT_SUCCESS;
     restart = std::max(1, std::min(restart, N));
@@ -306,9 +321,8 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     op.N = N;
     const int bs = 256;
     const long ldv = ws.ldv;
-    // ||rhs|| per system: residual at x = 0
-    HIP_CALL(hipMemsetAsync(ws.w64, 0, sizeof(double2) * ldv * R, s));
-    HOST_CALL(op.residual(ws, B, ldb, ws.w64, ldv, ws.r, s));
+    // ||rhs|| per system: the residual at x = 0
+    HOST_CALL(op.rhs(ws, B, ldb, ws.r, s));
     std::vector<double2> hbuf((size_t)ws.hstride * R);
     auto norms = [&](const double2 *w, std::vector<double> &out) -> int {
         HOST_CALL(bdot(ws, w, ldv, 1, w, ldv, N, ws.h, ws.hstride, s));
@@ -330,9 +344,16 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     std::vector<int> tlen(R, 0);
     std::vector<char> live(R, 0);
     int total = 0, cycles = 0;
+    // ws.r and beta hold the residual of the current X (and ws.h[.., 0] its
+    // squared norms) while res_current is set
+    bool res_current = x_zero;
+    if (x_zero) beta = nb;
     for (int cyc = 0; cyc < max_cycles; cyc++) {
-        HOST_CALL(op.residual(ws, B, ldb, X, ldx, ws.r, s));
-        HOST_CALL(norms(ws.r, beta));
+        if (!res_current) {
+            HOST_CALL(op.residual(ws, B, ldb, X, ldx, ws.r, s));
+            HOST_CALL(norms(ws.r, beta));
+            res_current = true;
+        }
         bool any = false;
         for (int q = 0; q < R; q++) {
             if (nb[q] == 0) continue;
@@ -435,10 +456,13 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
         kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, tmax, ws.h, ws.hstride, 1.0, X, ldx, N);
         HIP_CALL(hipGetLastError());
         HIP_CALL(hipStreamSynchronize(s));  // yh goes out of scope
+        res_current = false;
     }
-    // final FP64 residuals
-    HOST_CALL(op.residual(ws, B, ldb, X, ldx, ws.r, s));
-    HOST_CALL(norms(ws.r, beta));
+    // final FP64 residuals (unless the last check already saw the current X)
+    if (!res_current) {
+        HOST_CALL(op.residual(ws, B, ldb, X, ldx, ws.r, s));
+        HOST_CALL(norms(ws.r, beta));
+    }
     bool conv = true;
     double worst = 0;
     for (int q = 0; q < R; q++) {

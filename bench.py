@@ -49,6 +49,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batching", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scale for FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -114,6 +120,181 @@ def cpu_baseline(mesh, k, rho, c, target_s, threads):
         "sample": "complex-FP32 oracle assembly of %d of %d rows on %d threads in %.1f s, scaled to the full "
                   "%d x %d system: %.2f s per solve (solve not counted)" % (rows, M, threads, t, M, N, full),
         "full_assembly_s": full,
+    }
+
+
+def bench_batching(bem, C, workers, log_every=16):
+    """Multi-frequency, multi-right-hand-side batch: seeded perturbed ellipsoid
+    (semi-axes 0.075/0.09/0.11 m, 8 spherical-harmonic bumps up to degree 6,
+    amplitude <= 5%) at icosphere subdivision 4 (2562 nodes), rigid, 64
+    frequencies log-spaced 100 Hz - 8 kHz, 16 plane-wave directions per
+    frequency drawn uniformly on the sphere; FP32 assembly + lockstep GMRES
+    (MFMA multi-RHS products) with FP64 refinement to 1e-8."""
+    c = 343.0
+    mesh = bem.Mesh().ellipsoid(4, (0.075, 0.09, 0.11), nbumps=8, lmax=6, amp=0.05, seed=103)
+    mesh.set_bc(0, 1, 0)
+    N, E, M = mesh.num_pnts, mesh.num_elems, mesh.num_pnts
+    nf, R = 64, 16
+    freqs = 100.0 * 80.0 ** (np.arange(nf) / (nf - 1.0))
+    ks = (2 * np.pi * freqs / c).astype(np.float32)
+    dirs = np.stack([C.sphere_points(1030 + f, R, 1.0) for f in range(nf)]).astype(np.float32)
+    kw = dict(restart=30, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
+    bem.batch_solve(mesh, ks[-workers:], bem.RHS_PLANE, dirs[-workers:], workers=workers, **kw)  # warm-up
+    bem.sync()
+    t0 = time.perf_counter()
+    X, stats = bem.batch_solve(mesh, ks, bem.RHS_PLANE, dirs, workers=workers, **kw)
+    wall = time.perf_counter() - t0
+    conv = all(s["converged"] == 1 for s in stats)
+    its = [s["iterations"] for s in stats]
+    log("batching: %d x %d solves in %.3f s (%d workers), iterations min/median/max %d/%d/%d, converged %s" %
+        (nf, R, wall, workers, min(its), int(np.median(its)), max(its), conv))
+    # kernel roofline of the MFMA product, measured with one worker so the
+    # kernel has the GPU to itself
+    bem.timing_reset()
+    bem.timing_enable(True)
+    bem.batch_solve(mesh, ks[-2:], bem.RHS_PLANE, dirs[-2:], workers=1, **kw)
+    bem.sync()
+    bem.timing_enable(False)
+    g_ms, g_n = bem.timing_get("gemm_n_f32")
+    bem.timing_reset()
+    g_avg = g_ms / max(g_n, 1)
+    flops = 8.0 * M * N * R
+    gbytes = 8.0 * M * N + 8.0 * (M + N) * R
+    # accuracy against the all-FP64 path and the 1x yardstick (plain single-RHS
+    # FP64 solve) at the lowest, middle and highest frequency
+    errs, t64 = [], []
+    for f in (0, nf // 2, nf - 1):
+        s64 = bem.Solver(mesh)
+        s64.assemble(float(ks[f]), bem.PREC_FP64)
+        s64.set_rhs(bem.RHS_PLANE, dirs[f], 1.0)
+        s64.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=20000, refine_tol=0, max_refine=0)
+        x64 = s64.solution()
+        errs.append(float(np.max(np.linalg.norm(X[:, :, f] - x64, axis=0) / np.linalg.norm(x64, axis=0))))
+        s64.set_rhs(bem.RHS_PLANE, dirs[f][:1], 1.0)
+        bem.sync()
+        ts = time.perf_counter()
+        s64.assemble(float(ks[f]), bem.PREC_FP64)
+        s64.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=20000, refine_tol=0, max_refine=0)
+        bem.sync()
+        t64.append(time.perf_counter() - ts)
+        s64.close()
+    value = nf * R / wall
+    fp64_rate = 1.0 / float(np.mean(t64))
+    return {
+        "workload": "batching: perturbed ellipsoid s4 (N=%d, E=%d), rigid, 64 freqs 100 Hz-8 kHz x 16 plane waves; "
+                    "FP32 assembly + lockstep GMRES(30,1e-6) with MFMA multi-RHS products, FP64 refinement to 1e-8, "
+                    "%d concurrent workers" % (N, E, workers),
+        "value": round(value, 2), "unit": "solves/s", "wall_s": round(wall, 4), "solves": nf * R,
+        "iterations": {"min": int(min(its)), "median": int(np.median(its)), "max": int(max(its))},
+        "converged": conv,
+        "rel_err_vs_fp64_max": max(errs),
+        "fp64_single_rhs_solves_per_s": round(fp64_rate, 3),
+        "speedup_vs_fp64_single_rhs": round(value / fp64_rate, 2),
+        "roofline_gemm": {"bound": "mfma", "kernel": "gemm_n_f32", "avg_ms": round(g_avg, 5), "launches": g_n,
+                          "achieved": round(flops / (g_avg * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
+                          "unit": "TFLOP/s", "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                          "hbm_gbs": round(gbytes / (g_avg * 1e-3) / 1e9, 1), "flops_per_launch": flops,
+                          "bytes_per_launch": gbytes},
+    }
+
+
+def serving_requests(bem, C, n, seed):
+    """Seeded request mix: sphere (r = 0.1 m) or perturbed ellipsoid with
+    p = 1/2, icosphere subdivision uniform in 3..6, ka uniform in [0.5, 20],
+    1..32 right-hand sides, plane waves or point sources 1 cm radially outside
+    the surface with p = 1/2; 16 CHIEF points inside each body."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(n):
+        s = int(rng.integers(3, 7))
+        if rng.random() < 0.5:
+            m = bem.Mesh().icosphere(s, 0.1)
+            a = 0.1
+            m.set_chief(C.ball_points(seed * 1000 + i, 16, 0.05))
+        else:
+            m = bem.Mesh().ellipsoid(s, (0.075, 0.09, 0.11), nbumps=8, lmax=6, amp=0.05, seed=seed * 1000 + i)
+            a = 0.11
+            m.set_chief(C.ball_points(seed * 1000 + i, 16, 0.03))
+        m.set_bc(0, 1, 0)
+        k = float(rng.uniform(0.5, 20.0) / a)
+        R = int(rng.integers(1, 33))
+        if rng.random() < 0.5:
+            kind, params = bem.RHS_PLANE, C.sphere_points(seed * 7000 + i, R, 1.0)
+        else:
+            pn = m.pnts()
+            v = pn[rng.integers(0, pn.shape[0], R)].astype(np.float64)
+            params = v * (1.0 + 0.01 / np.linalg.norm(v, axis=1))[:, None]
+            kind = bem.RHS_POINT
+        reqs.append(dict(mesh=m, k=k, kind=kind, params=np.asarray(params, np.float32), s=s, R=R))
+    return reqs
+
+
+def bench_serving(bem, C, n, rate, slots, seed=104):
+    """Poisson request stream into the request server; latency = completion -
+    arrival (queueing included)."""
+    import threading
+    reqs = serving_requests(bem, C, n, seed)
+    gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    kw = dict(restart=30, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
+    srv = bem.Server(slots=slots)
+    # warm-up: one small request
+    srv.wait(srv.submit(reqs[0]["mesh"], reqs[0]["k"], reqs[0]["kind"], reqs[0]["params"], **kw))
+    ids = [None] * n
+    results = [None] * n
+    ready = [threading.Event() for _ in range(n)]
+
+    def collect():
+        for i in range(n):
+            ready[i].wait()
+            results[i] = srv.wait(ids[i])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        dt = arrivals[i] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids[i] = srv.submit(r["mesh"], r["k"], r["kind"], r["params"], **kw)
+        ready[i].set()
+        if (i + 1) % 8 == 0:
+            log("serving: submitted %d/%d" % (i + 1, n))
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.status()
+    srv.close()
+    lat = np.array([res[1]["end_ms"] - res[1]["submit_ms"] for res in results])
+    queue = np.array([res[1]["start_ms"] - res[1]["submit_ms"] for res in results])
+    conv = all(res[1]["solve"]["converged"] == 1 for res in results)
+    cols = sum(r["R"] for r in reqs)
+    # accuracy: the smallest requests against the all-FP64 path
+    errs = []
+    for i in sorted(range(n), key=lambda j: reqs[j]["mesh"].num_pnts)[:2]:
+        r = reqs[i]
+        s64 = bem.Solver(r["mesh"])
+        s64.assemble(r["k"], bem.PREC_FP64)
+        s64.set_rhs(r["kind"], r["params"], 1.0)
+        s64.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=20000, refine_tol=0, max_refine=0)
+        x64 = s64.solution()
+        s64.close()
+        errs.append(float(np.max(np.linalg.norm(results[i][0] - x64, axis=0) / np.linalg.norm(x64, axis=0))))
+    sizes = {}
+    for r in reqs:
+        sizes[r["s"]] = sizes.get(r["s"], 0) + 1
+    return {
+        "workload": "serving: %d Poisson requests (rate %.3g/s, seed %d), sphere/perturbed ellipsoid s3-s6, "
+                    "ka U[0.5,20], 1-32 plane-wave or point-source RHS, 16 CHIEF; %d slots, HBM-footprint admission" %
+                    (n, rate, seed, slots),
+        "requests": n, "rhs_columns": cols, "wall_s": round(wall, 3),
+        "solves_per_s": round(cols / wall, 2), "requests_per_s": round(n / wall, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
+                       "max": round(float(lat.max()), 2)},
+        "queue_ms": {"p50": round(float(np.percentile(queue, 50)), 2), "p99": round(float(np.percentile(queue, 99)), 2)},
+        "converged": conv, "subdiv_counts": {str(k): v for k, v in sorted(sizes.items())},
+        "max_running": st["max_running"], "peak_hbm_gb": round(st["peak_in_use_bytes"] / 1e9, 2),
+        "budget_gb": round(st["budget_bytes"] / 1e9, 1),
+        "rel_err_vs_fp64_smallest2": errs,
     }
 
 
@@ -216,6 +397,15 @@ def main():
         cpu = cpu_baseline(mesh, k, rho, c, args.cpu_seconds, threads)
         log("cpu baseline", cpu)
 
+    batching = None
+    if not args.no_batching:
+        batching = bench_batching(bem, C, args.workers)
+        log("batching", batching)
+    serving = None
+    if not args.no_serving:
+        serving = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
+        log("serving", serving)
+
     value = 1000.0 / ms_per_step * R
     out = {
         "metric": "BEM solves/sec (surface pressure, <=1e-5 rel err vs FP64)",
@@ -254,6 +444,8 @@ def main():
                           "traffic": measured_traffic("gemv_n_f32", args.config),
                           "launches": gn_n},
         "cpu_baseline": cpu and {kk: cpu[kk] for kk in ("value", "unit", "cores", "kind", "sample")},
+        "batching": batching,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 
