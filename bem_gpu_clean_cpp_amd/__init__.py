@@ -23,6 +23,7 @@ RHS_POINT = 0
 RHS_PLANE = 1
 SOLVE_QR = 0
 SOLVE_GMRES = 1
+SOLVE_CGLS = 2
 
 _lib = None
 
@@ -132,6 +133,7 @@ _SIGS = {
     "bem_solver_field": (_I, [_P, _P, _I, _P, _I]),
     "bem_solver_dims": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "bem_solver_stats": (_I, [_P, ctypes.POINTER(bem_solve_stats)]),
+    "bem_solver_get_rows": (_I, [_P, _P, _I, _P]),
     "bem_solver_set_mesh": (_I, [_P, _P]),
     "bem_solver_release": (_I, [_P]),
     "bem_batch_solve": (_I, [_P, _P, _I, _I, _I, _P, _I, _I, _F, ctypes.POINTER(bem_solve_params), _I, _P,
@@ -432,6 +434,14 @@ class Solver:
         B = np.zeros((max(R, 1), M), np.complex64)
         _check(lib().bem_solver_get_system(self._h, _ptr(A), M, _ptr(B) if R else None, M), "get_system")
         return A.T.copy(), B[:R].T.copy()
+
+    def rows(self, rows):
+        """Rows of the assembled A (len(rows) x N, complex128)."""
+        r = np.ascontiguousarray(rows, np.int32).ravel()
+        M, N, R = self.dims()
+        out = np.zeros((r.size, N), np.complex128)
+        _check(lib().bem_solver_get_rows(self._h, _ptr(r), r.size, _ptr(out)), "get_rows")
+        return out
 
     def field(self, pts):
         p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)

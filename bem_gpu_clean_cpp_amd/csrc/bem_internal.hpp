@@ -209,6 +209,11 @@ int mh_f64(const double2 *A, long lda, long M, long N, const double2 *T, long ld
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
                  int max_cycles, bem_solve_stats *st, hipStream_t s);
+// CGLS (conjugate gradients on A^H A x = A^H b) for R systems in lockstep,
+// FP64 residual refinement; X is overwritten (the solve starts from zero)
+int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
+                double2 *X, long ldx, int R, double tol, int maxit, double outer_tol, int max_cycles,
+                bem_solve_stats *st, hipStream_t s);
 int qr_lsq_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, hipStream_t s);
 int qr_lsq_f64(double2 *A, long lda, int M, int N, double2 *B, long ldb, int R, hipStream_t s);
 int qr_lsq_with_q_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, float2 *Q, long ldq,

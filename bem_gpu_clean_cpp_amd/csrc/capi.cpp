@@ -50,6 +50,15 @@ Scratch g_sa, g_sb, g_sc;
 
 long round64(long m) { return (m + 63) / 64 * 64; }
 
+template <typename C2>
+__global__ void k_gather_rows(const C2 *A, long lda, int N, const int *rows, int nr, double2 *out) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nr * N) return;
+    const int q = (int)(t / N), j = (int)(t % N);
+    const C2 v = A[(size_t)j * lda + rows[q]];
+    out[(size_t)q * N + j] = make_double2(v.x, v.y);
+}
+
 }  // namespace
 
 extern "C" {
@@ -499,7 +508,8 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
     if (!sv) BEM_FAIL("null solver");
     if (!sv->prec) BEM_FAIL("solver not assembled");
     if (sv->rhs.nrhs <= 0) BEM_FAIL("no right-hand sides set");
-    if (method != BEM_SOLVE_QR && method != BEM_SOLVE_GMRES) BEM_FAIL("unknown method %d", method);
+    if (method != BEM_SOLVE_QR && method != BEM_SOLVE_GMRES && method != BEM_SOLVE_CGLS)
+        BEM_FAIL("unknown method %d", method);
     std::lock_guard<std::mutex> lk(sv->mu);
     hipStream_t s = sv->stream;
     DeviceProblem &dp = sv->dp;
@@ -519,7 +529,13 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
     HOST_CALL(rhs_fp64(dp, (double)sv->k, sv->B64, ld, sv->rhs, s));
     bem_solve_stats st{};
     st.method = method;
-    if (method == BEM_SOLVE_GMRES) {
+    if (method == BEM_SOLVE_CGLS) {
+        const double outer = refine_tol > 0 ? refine_tol : (double)tol;
+        const int cycles = max_refine > 0 ? max_refine : (1 << 20);
+        if (maxit <= 0) maxit = 100000;
+        HOST_CALL(cgls_device(sv->A32, sv->A64, ld, dp.M, dp.N, sv->B64, ld, sv->X64, ld, R, (double)tol, maxit,
+                              outer, cycles, &st, s));
+    } else if (method == BEM_SOLVE_GMRES) {
         HIP_CALL(hipMemsetAsync(sv->X64, 0, sizeof(double2) * ld * R, s));
         const double outer = refine_tol > 0 ? refine_tol : (double)tol;
         const int cycles = max_refine > 0 ? max_refine : (1 << 20);
@@ -617,6 +633,32 @@ int bem_solver_get_system(const bem_solver *sv, bem_complex *A, int lda, bem_com
         HOST_CALL(rc);
         HIP_CALL(e);
     }
+    return EXIT_SUCCESS;
+}
+
+int bem_solver_get_rows(const bem_solver *sv, const int *rows, int nrows, bem_zcomplex *out) {
+    if (!sv || !rows || !out || nrows <= 0) BEM_FAIL("bad arguments");
+    if (!sv->prec) BEM_FAIL("solver not assembled");
+    const DeviceProblem &dp = sv->dp;
+    for (int q = 0; q < nrows; q++)
+        if (rows[q] < 0 || rows[q] >= dp.M) BEM_FAIL("row %d out of range [0, %d)", rows[q], dp.M);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
+    int *rd = nullptr;
+    double2 *od = nullptr;
+    HOST_CALL(salloc(&rd, nrows, s));
+    HOST_CALL(salloc(&od, (size_t)nrows * dp.N, s));
+    HIP_CALL(hipMemcpyAsync(rd, rows, sizeof(int) * nrows, hipMemcpyHostToDevice, s));
+    const long n = (long)nrows * dp.N;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (sv->A32) k_gather_rows<float2><<<nb, 256, 0, s>>>(sv->A32, dp.ldA, dp.N, rd, nrows, od);
+    else k_gather_rows<double2><<<nb, 256, 0, s>>>(sv->A64, dp.ldA, dp.N, rd, nrows, od);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, od, sizeof(double2) * n, hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    sfree(rd, s);
+    sfree(od, s);
+    HIP_CALL(e);
     return EXIT_SUCCESS;
 }
 

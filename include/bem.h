@@ -159,10 +159,10 @@ typedef struct bem_solver bem_solver;
 
 enum { BEM_PREC_FP32 = 32, BEM_PREC_FP64 = 64 };
 enum { BEM_RHS_POINT = 0, BEM_RHS_PLANE = 1 };
-enum { BEM_SOLVE_QR = 0, BEM_SOLVE_GMRES = 1 };
+enum { BEM_SOLVE_QR = 0, BEM_SOLVE_GMRES = 1, BEM_SOLVE_CGLS = 2 };
 
 typedef struct bem_solve_stats {
-    int method;           /* BEM_SOLVE_QR or BEM_SOLVE_GMRES */
+    int method;           /* BEM_SOLVE_QR, BEM_SOLVE_GMRES or BEM_SOLVE_CGLS */
     int iterations;       /* total inner Krylov iterations (all refinement steps) */
     int refinements;      /* outer FP64 refinement steps */
     int converged;        /* 1 when the requested tolerance was reached */
@@ -186,7 +186,9 @@ int bem_solver_set_rhs(bem_solver *s, int kind, const float *params, int nrhs, f
  * A^H A x = A^H b when CHIEF rows make it overdetermined; the FP32 path is
  * wrapped in FP64 residual refinement until refine_tol (<= 0: one pass).
  * method BEM_SOLVE_QR: blocked Householder least squares (FP32 path also
- * refined in FP64). */
+ * refined in FP64).  method BEM_SOLVE_CGLS: conjugate gradients on the
+ * normal equations (restart unused), for the overdetermined CHIEF systems,
+ * with the same FP64 refinement. */
 int bem_solver_solve(bem_solver *s, int method, int restart, float tol, int maxit, double refine_tol,
                      int max_refine, bem_solve_stats *stats);
 /* Copy the nodal solution (N x nrhs, ldx) to the host. */
@@ -194,6 +196,9 @@ int bem_solver_get_solution(const bem_solver *s, bem_complex *x, int ldx);
 int bem_solver_get_solution_z(const bem_solver *s, bem_zcomplex *x, int ldx);
 /* Copy the assembled A (M x N) / B (M x nrhs) to the host as complex64. */
 int bem_solver_get_system(const bem_solver *s, bem_complex *A, int lda, bem_complex *B, int ldb);
+/* Rows rows[0..nrows) of the assembled A as complex128, row-major (nrows x N):
+ * spot checks of very large systems without copying all of A. */
+int bem_solver_get_rows(const bem_solver *s, const int *rows, int nrows, bem_zcomplex *out);
 /* Exterior field at P points for every right-hand side: out is P x nrhs (ldo). */
 int bem_solver_field(bem_solver *s, const bem_cart_coord *pts, int P, bem_complex *out, int ldo);
 /* Sizes: M rows, N columns, current nrhs. */
@@ -211,7 +216,7 @@ int bem_solver_release(bem_solver *s);
 /* Solve settings shared by the batch and server calls (the arguments of
  * bem_solver_solve). */
 typedef struct bem_solve_params {
-    int method;         /* BEM_SOLVE_GMRES or BEM_SOLVE_QR */
+    int method;         /* BEM_SOLVE_GMRES, BEM_SOLVE_QR or BEM_SOLVE_CGLS */
     int restart;
     float tol;
     int maxit;

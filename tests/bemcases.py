@@ -70,3 +70,38 @@ def mesh_arrays(mesh):
     """(pnts float32 N x 3, elems int32 E x 3, bc complex E x 3) of a bem Mesh."""
     e = mesh.elems()
     return mesh.pnts(), np.ascontiguousarray(e["nodes"]), np.ascontiguousarray(e["bc"]).astype(np.complex128)
+
+
+def outward_vertex_normals(pnts, elems):
+    """Area-weighted vertex normals pointing out of the body.  Element normals
+    (p1 - p3) x (p2 - p3) point into the body for the reference's winding, so
+    they are negated."""
+    p = np.asarray(pnts, np.float64)
+    e = np.asarray(elems)
+    fn = np.cross(p[e[:, 0]] - p[e[:, 2]], p[e[:, 1]] - p[e[:, 2]])
+    vn = np.zeros_like(p)
+    for j in range(3):
+        np.add.at(vn, e[:, j], fn)
+    vn = -vn
+    return vn / np.linalg.norm(vn, axis=1, keepdims=True)
+
+
+def sources_outside(mesh, seed, n, offset):
+    """n point sources `offset` metres outside the surface along the outward
+    normal at n seeded random vertices."""
+    p, e, _ = mesh_arrays(mesh)
+    idx = np.random.default_rng(seed).choice(p.shape[0], n, replace=False)
+    vn = outward_vertex_normals(p, e)
+    return (p[idx].astype(np.float64) + offset * vn[idx]).astype(np.float32)
+
+
+def large_case(bem):
+    """Large workload: seeded perturbed ellipsoid (semi-axes 0.075/0.09/0.11 m,
+    8 spherical-harmonic bumps up to degree 6, amplitude <= 5%) at icosphere
+    subdivision 6 (40962 nodes, 81920 triangles), rigid, 12 kHz, 32 CHIEF points
+    in r < 0.05 m, 32 unit point sources 1 cm outside the surface."""
+    m = bem.Mesh().ellipsoid(6, (0.075, 0.09, 0.11), nbumps=8, lmax=6, amp=0.05, seed=105)
+    m.set_bc(0, 1, 0)
+    m.set_chief(ball_points(1051, 32, 0.05))
+    src = sources_outside(m, 1052, 32, 0.01)
+    return m, src

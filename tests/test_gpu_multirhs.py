@@ -144,3 +144,23 @@ def test_multi_rhs_qr_matches_gmres():
     xq = s.solution()
     assert st["converged"] == 1
     assert _relcols(xq, xg).max() < 1e-7
+
+
+@pytest.mark.parametrize("R", [1, 20])
+@pytest.mark.parametrize("prec", [bem.PREC_FP32, bem.PREC_FP64])
+def test_cgls_matches_oracle(R, prec):
+    """Conjugate gradients on the CHIEF normal equations (lockstep, FP64
+    refinement) against the oracle's least-squares solution."""
+    m = _mesh(3, 8)
+    k = 31.3
+    d = _dirs(R, seed=13)
+    s = bem.Solver(m)
+    s.assemble(k, prec)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    st = s.solve(bem.SOLVE_CGLS, tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=20)
+    x = s.solution()
+    xo = _oracle_lstsq(m, k, d)
+    err = _relcols(x, xo)
+    print("R %d prec %d stats %s worst err %.3g" % (R, prec, st, err.max()))
+    assert st["converged"] == 1
+    assert err.max() < (1e-5 if prec == bem.PREC_FP32 else 1e-8)
