@@ -51,9 +51,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
     ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scale for FETCH/WRITE_SIZE counters)")
@@ -198,6 +199,74 @@ def bench_batching(bem, C, workers, log_every=16):
     }
 
 
+def bench_large(bem, C, fp64=True):
+    """Large: seeded perturbed ellipsoid at icosphere subdivision 6 (40962
+    nodes, 81920 triangles, 13.4 GB complex64 matrix), rigid, 12 kHz, 32 CHIEF
+    points, 32 point sources 1 cm outside the surface; FP32 assembly + CG on
+    the normal equations with FP32 MFMA products (R = 32) and FP64 refinement
+    to a normal-equation residual of 1e-11."""
+    c = 343.0
+    m, src = C.large_case(bem)
+    N, E, Cn = m.num_pnts, m.num_elems, m.num_chief
+    M = N + Cn
+    R = src.shape[0]
+    k = 2 * np.pi * 12000.0 / c
+    kw = dict(tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=10)
+    sv = bem.Solver(m)
+    sv.set_rhs(bem.RHS_POINT, src, 1.0)
+    sv.assemble(k, bem.PREC_FP32)  # warm-up of the assembly kernels at this size
+    bem.sync()
+    bem.timing_reset()
+    bem.timing_enable(True)
+    t0 = time.perf_counter()
+    sv.assemble(k, bem.PREC_FP32)
+    st = sv.solve(bem.SOLVE_CGLS, **kw)
+    bem.sync()
+    wall = time.perf_counter() - t0
+    bem.timing_enable(False)
+    g_ms, g_n = bem.timing_get("gemm_n_f32")
+    h_ms, h_n = bem.timing_get("gemm_h_f32")
+    a_ms, a_n = bem.timing_get("asm_tile_f32")
+    bem.timing_reset()
+    x32 = sv.solution()
+    sv.close()
+    log("large: %.2f s, stats %s" % (wall, st))
+    g_avg, h_avg = g_ms / max(g_n, 1), h_ms / max(h_n, 1)
+    flops = 8.0 * M * N * R
+    out = {
+        "workload": "large: perturbed ellipsoid s6 (N=%d, E=%d, M=%d, A %.1f GB), rigid, 12 kHz, 32 CHIEF, "
+                    "32 point sources 1 cm outside; FP32 assembly + CG on the normal equations (FP32 MFMA "
+                    "products, R=32) with FP64 refinement to 1e-11" % (N, E, M, 8.0 * M * N / 1e9),
+        "value": round(R / wall, 3), "unit": "solves/s", "wall_s": round(wall, 3),
+        "assembly_ms": round(a_ms / max(a_n, 1), 2),
+        "solve": {kk: st[kk] for kk in ("iterations", "refinements", "converged", "residual")},
+        "roofline_gemm": {"bound": "mfma", "kernel": "gemm_n_f32", "avg_ms": round(g_avg, 3), "launches": g_n,
+                          "achieved": round(flops / (g_avg * 1e-3) / 1e12, 2), "peak": PEAK_FP32_TFLOPS,
+                          "unit": "TFLOP/s", "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                          "adjoint_avg_ms": round(h_avg, 3),
+                          "adjoint_achieved": round(flops / (h_avg * 1e-3) / 1e12, 2),
+                          "hbm_gbs": round(8.0 * M * N / (g_avg * 1e-3) / 1e9, 1), "flops_per_launch": flops},
+    }
+    if fp64:
+        # 1x yardstick: plain single-RHS FP64 solve (FP64 assembly + FP64 CG),
+        # also the reference solution for the first source
+        s64 = bem.Solver(m)
+        s64.set_rhs(bem.RHS_POINT, src[:1], 1.0)
+        bem.sync()
+        ts = time.perf_counter()
+        s64.assemble(k, bem.PREC_FP64)
+        st64 = s64.solve(bem.SOLVE_CGLS, tol=1e-6, maxit=6000, refine_tol=1e-12, max_refine=10)
+        bem.sync()
+        t64 = time.perf_counter() - ts
+        x64 = s64.solution()
+        s64.close()
+        err = float(np.linalg.norm(x32[:, 0] - x64[:, 0]) / np.linalg.norm(x64[:, 0]))
+        log("large fp64: %.2f s, stats %s, err %.3g" % (t64, st64, err))
+        out.update({"rel_err_vs_fp64": err, "fp64_single_rhs_s": round(t64, 3),
+                    "speedup_vs_fp64_single_rhs": round((R / wall) * t64, 2)})
+    return out
+
+
 def serving_requests(bem, C, n, seed):
     """Seeded request mix: sphere (r = 0.1 m) or perturbed ellipsoid with
     p = 1/2, icosphere subdivision uniform in 3..6, ka uniform in [0.5, 20],
@@ -229,14 +298,16 @@ def serving_requests(bem, C, n, seed):
     return reqs
 
 
-def bench_serving(bem, C, n, rate, slots, seed=104):
+def bench_serving(bem, C, n, rate, slots, seed=104, method=None):
     """Poisson request stream into the request server; latency = completion -
-    arrival (queueing included)."""
+    arrival (queueing included).  The CHIEF systems are solved with CG on the
+    normal equations (FP32 MFMA products, FP64 refinement to 1e-8)."""
     import threading
     reqs = serving_requests(bem, C, n, seed)
     gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, n)
     arrivals = np.cumsum(gaps) - gaps[0]
-    kw = dict(restart=30, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
+    kw = dict(method=bem.SOLVE_CGLS if method is None else method, restart=30, tol=1e-6, maxit=6000,
+              refine_tol=1e-8, max_refine=50)
     srv = bem.Server(slots=slots)
     # warm-up: one small request
     srv.wait(srv.submit(reqs[0]["mesh"], reqs[0]["k"], reqs[0]["kind"], reqs[0]["params"], **kw))
@@ -275,7 +346,7 @@ def bench_serving(bem, C, n, rate, slots, seed=104):
         s64 = bem.Solver(r["mesh"])
         s64.assemble(r["k"], bem.PREC_FP64)
         s64.set_rhs(r["kind"], r["params"], 1.0)
-        s64.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=20000, refine_tol=0, max_refine=0)
+        s64.solve(bem.SOLVE_CGLS, tol=1e-6, maxit=20000, refine_tol=1e-12, max_refine=10)
         x64 = s64.solution()
         s64.close()
         errs.append(float(np.max(np.linalg.norm(results[i][0] - x64, axis=0) / np.linalg.norm(x64, axis=0))))
@@ -284,8 +355,8 @@ def bench_serving(bem, C, n, rate, slots, seed=104):
         sizes[r["s"]] = sizes.get(r["s"], 0) + 1
     return {
         "workload": "serving: %d Poisson requests (rate %.3g/s, seed %d), sphere/perturbed ellipsoid s3-s6, "
-                    "ka U[0.5,20], 1-32 plane-wave or point-source RHS, 16 CHIEF; %d slots, HBM-footprint admission" %
-                    (n, rate, seed, slots),
+                    "ka U[0.5,20], 1-32 plane-wave or point-source RHS, 16 CHIEF; FP32 assembly + CG on the normal "
+                    "equations with FP64 refinement; %d slots, HBM-footprint admission" % (n, rate, seed, slots),
         "requests": n, "rhs_columns": cols, "wall_s": round(wall, 3),
         "solves_per_s": round(cols / wall, 2), "requests_per_s": round(n / wall, 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
@@ -401,6 +472,10 @@ def main():
     if not args.no_batching:
         batching = bench_batching(bem, C, args.workers)
         log("batching", batching)
+    large = None
+    if not args.no_large:
+        large = bench_large(bem, C, fp64=not args.no_fp64)
+        log("large", large)
     serving = None
     if not args.no_serving:
         serving = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
@@ -445,6 +520,7 @@ def main():
                           "launches": gn_n},
         "cpu_baseline": cpu and {kk: cpu[kk] for kk in ("value", "unit", "cores", "kind", "sample")},
         "batching": batching,
+        "large": large,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)
