@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--only", choices=["batching", "large", "serving"],
+                    help="run one secondary workload alone and print its section (not the benchmark line)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (requests/s)")
@@ -124,7 +126,7 @@ def cpu_baseline(mesh, k, rho, c, target_s, threads):
     }
 
 
-def bench_batching(bem, C, workers, log_every=16):
+def bench_batching(bem, C, workers, fp64=True):
     """Multi-frequency, multi-right-hand-side batch: seeded perturbed ellipsoid
     (semi-axes 0.075/0.09/0.11 m, 8 spherical-harmonic bumps up to degree 6,
     amplitude <= 5%) at icosphere subdivision 4 (2562 nodes), rigid, 64
@@ -164,7 +166,7 @@ def bench_batching(bem, C, workers, log_every=16):
     # accuracy against the all-FP64 path and the 1x yardstick (plain single-RHS
     # FP64 solve) at the lowest, middle and highest frequency
     errs, t64 = [], []
-    for f in (0, nf // 2, nf - 1):
+    for f in ((0, nf // 2, nf - 1) if fp64 else ()):
         s64 = bem.Solver(mesh)
         s64.assemble(float(ks[f]), bem.PREC_FP64)
         s64.set_rhs(bem.RHS_PLANE, dirs[f], 1.0)
@@ -180,7 +182,7 @@ def bench_batching(bem, C, workers, log_every=16):
         t64.append(time.perf_counter() - ts)
         s64.close()
     value = nf * R / wall
-    fp64_rate = 1.0 / float(np.mean(t64))
+    fp64_rate = 1.0 / float(np.mean(t64)) if t64 else None
     return {
         "workload": "batching: perturbed ellipsoid s4 (N=%d, E=%d), rigid, 64 freqs 100 Hz-8 kHz x 16 plane waves; "
                     "FP32 assembly + lockstep GMRES(30,1e-6) with MFMA multi-RHS products, FP64 refinement to 1e-8, "
@@ -188,9 +190,9 @@ def bench_batching(bem, C, workers, log_every=16):
         "value": round(value, 2), "unit": "solves/s", "wall_s": round(wall, 4), "solves": nf * R,
         "iterations": {"min": int(min(its)), "median": int(np.median(its)), "max": int(max(its))},
         "converged": conv,
-        "rel_err_vs_fp64_max": max(errs),
-        "fp64_single_rhs_solves_per_s": round(fp64_rate, 3),
-        "speedup_vs_fp64_single_rhs": round(value / fp64_rate, 2),
+        "rel_err_vs_fp64_max": max(errs) if errs else None,
+        "fp64_single_rhs_solves_per_s": fp64_rate and round(fp64_rate, 3),
+        "speedup_vs_fp64_single_rhs": fp64_rate and round(value / fp64_rate, 2),
         "roofline_gemm": {"bound": "mfma", "kernel": "gemm_n_f32", "avg_ms": round(g_avg, 5), "launches": g_n,
                           "achieved": round(flops / (g_avg * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
                           "unit": "TFLOP/s", "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
@@ -381,6 +383,19 @@ def main():
     import bem_gpu_clean_cpp_amd as bem
     import bemcases as C
 
+    if args.only:
+        # one secondary workload alone (for profiling); prints that section's JSON
+        bem.quad_init(6)
+        bem.set_medium(1.21, 343.0)
+        if args.only == "batching":
+            res = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
+        elif args.only == "large":
+            res = bench_large(bem, C, fp64=not args.no_fp64)
+        else:
+            res = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
+        print(json.dumps({args.only: res}), flush=True)
+        return
+
     cfg = CONFIGS[args.config]
     rho, c = 1.21, 343.0
     a = cfg["radius"]
@@ -470,7 +485,7 @@ def main():
 
     batching = None
     if not args.no_batching:
-        batching = bench_batching(bem, C, args.workers)
+        batching = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
         log("batching", batching)
     large = None
     if not args.no_large:
