@@ -191,11 +191,15 @@ int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 
 // multi-right-hand-side products (gemm.hip)
 size_t ws_gemm(long M, long N, int R);
+size_t ws_gemm_h_f32(long M, long N, int R);
 int gemm_n_f32(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, float2 *Y, long ldy, int R,
                void *ws, hipStream_t s);
 int gemm_h_f32(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R,
                void *ws, hipStream_t s);
 size_t ws_mres(long M, long N, int R);
+// C -= V W (QR trailing update, V with 32 columns)
+int rank_update_f32(const float2 *V, long ldv, int M, const float2 *W, int ldw, int nc, float2 *C, long ldc,
+                    hipStream_t s);
 int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
               double2 *out, long ldo, int R, void *ws, hipStream_t s);
 int mres_f64(const double2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,

@@ -316,6 +316,13 @@ static void gemm_h_shape(long M, long N, int &nch, int &kr) {
     nch = (int)((M + kr - 1) / kr);
 }
 
+// partials of the FP32 adjoint product alone (Z = A^H T, N x R)
+size_t ws_gemm_h_f32(long M, long N, int R) {
+    int nch, kr;
+    gemm_h_shape(M, N, nch, kr);
+    return (size_t)nch * (size_t)((N + 63) / 64 * 64) * (size_t)std::max(1, R) * sizeof(float2);
+}
+
 // partials of either product at FP64 for R right-hand sides
 size_t ws_gemm(long M, long N, int R) {
     int a, b, c, d;
@@ -403,6 +410,86 @@ int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long 
 int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
             void *ws, hipStream_t s) {
     return gemm_h_impl<double>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "mh_f64acc", s);
+}
+
+// ------------------------------------------------ QR rank-nb update ---
+// C (M x nc, ldc) -= V (M x 32, ldv) W, W stored per column of C (W[c * ldw + l],
+// l < 32; columns l >= nb of V and rows l >= nb of W are zero).  The
+// trailing update of the blocked Householder QR: wave tile 32 complex rows x
+// (16 G) columns of C, K = 32; no split-K (K is one panel), C updated in place.
+template <int G>
+__global__ __launch_bounds__(256) void rank_update_mfma(const float2 *__restrict__ V, long ldv, int M,
+                                                        const float2 *__restrict__ W, int ldw, int nc,
+                                                        float2 *__restrict__ Cm, long ldc) {
+    typedef MF<float>::v4 v4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = (blockIdx.x * 4 + wave) * 32;
+    if (row0 >= M) return;
+    const int c0 = blockIdx.y * 16 * G;
+    const int m = lane & 15, kk = lane >> 4;
+    v4 acc[G][2][4];
+#pragma unroll
+    for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[g][p][t] = (v4){0, 0, 0, 0};
+    const float4 *Vb = reinterpret_cast<const float4 *>(V + row0 + 2 * m);
+    const long ldv4 = ldv / 2;
+    const float2 *Wg[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) Wg[g] = W + (size_t)min(c0 + g * 16 + m, nc - 1) * ldw;
+    float4 a[8];
+    float2 w[8][G];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const int L = 4 * u + kk;
+        a[u] = Vb[(size_t)L * ldv4];
+#pragma unroll
+        for (int g = 0; g < G; g++) w[u][g] = Wg[g][L];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+        const float av[4] = {a[u].x, a[u].y, a[u].z, a[u].w};
+#pragma unroll
+        for (int g = 0; g < G; g++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                acc[g][0][t] = MF<float>::mma(av[t], w[u][g].x, acc[g][0][t]);
+                acc[g][1][t] = MF<float>::mma(av[t], w[u][g].y, acc[g][1][t]);
+            }
+    }
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        const int n = c0 + g * 16 + m;
+        if (n >= nc) continue;
+        float2 *cc = Cm + (size_t)n * ldc;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int mp = MF<float>::drow(kk, v);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int i = row0 + 2 * mp + h;
+                if (i < M) {
+                    float2 o = cc[i];
+                    o.x -= acc[g][0][2 * h][v] - acc[g][1][2 * h + 1][v];
+                    o.y -= acc[g][0][2 * h + 1][v] + acc[g][1][2 * h][v];
+                    cc[i] = o;
+                }
+            }
+        }
+    }
+}
+
+// C -= V W as above; V needs 32 columns and ldv >= roundup(M, 32), even
+int rank_update_f32(const float2 *V, long ldv, int M, const float2 *W, int ldw, int nc, float2 *C, long ldc,
+                    hipStream_t s) {
+    if (M <= 0 || nc <= 0) return EXIT_SUCCESS;
+    if (ldv % 2 || ldv < (M + 31) / 32 * 32) BEM_FAIL("rank_update needs an even ldv >= roundup(M, 32)");
+    dim3 grid((unsigned)((M + 127) / 128), (unsigned)((nc + 31) / 32));
+    rank_update_mfma<2><<<grid, 256, 0, s>>>(V, ldv, M, W, ldw, nc, C, ldc);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
 }
 
 // ------------------------------------------------- FP64-matrix residuals ---

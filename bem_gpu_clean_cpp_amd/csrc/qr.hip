@@ -11,10 +11,14 @@
 //   qr_vhc      W = T^H (V^H C) for the trailing columns C (one workgroup per
 //               column, the nb dot products in one pass over the column);
 //   qr_update   C -= V W (the rank-nb trailing update, GEMM-shaped);
+//   FP32 trailing updates run on MFMA instead: V^H C as the adjoint product
+//               (C^H V, gemm.hip) against an explicit copy of the panel's V,
+//               W = T^H (V^H C), then C -= V W with the rank-32 MFMA update;
 //   trsv_upper  R x = (Q^H b)[0:N] by column-oriented back substitution.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "bem_internal.hpp"
@@ -267,6 +271,48 @@ __global__ __launch_bounds__(256) void qr_update(const typename QT<T>::c2 *A, lo
     *p = o;
 }
 
+// Vd[l][i - j0] = V[i, l] of the panel (explicit unit diagonal and zeros above
+// it; columns l >= nb zero), rows j0..M-1, 32 columns, ld ldvd
+template <typename T>
+__global__ void k_build_v(const typename QT<T>::c2 *A, long lda, int M, int j0, int nb, typename QT<T>::c2 *Vd,
+                          long ldvd) {
+    using C2 = typename QT<T>::c2;
+    const int i = j0 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (i >= M) return;
+    C2 z;
+    if (l < nb) {
+        z = vget(A, lda, j0, l, i);
+    } else {
+        z.x = 0;
+        z.y = 0;
+    }
+    Vd[(size_t)l * ldvd + (i - j0)] = z;
+}
+
+// W[c][l] = (T^H V^H C)[l][c] = conj(sum_q T[q][l] Z[q][c]) with Z[q][c] =
+// (C^H V)[c][q] (the adjoint product's output); rows l >= nb zero
+template <typename T>
+__global__ void k_qr_w(const typename QT<T>::c2 *Tm, int nb, const typename QT<T>::c2 *Z, long ldz, int nc,
+                       typename QT<T>::c2 *W) {
+    using C2 = typename QT<T>::c2;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (c >= nc) return;
+    T sr = 0, si = 0;
+    if (l < nb) {
+        for (int q = 0; q <= l; q++) {  // T upper triangular
+            const C2 t = Tm[q * kNB + l], z = Z[(size_t)q * ldz + c];
+            sr += t.x * z.x - t.y * z.y;
+            si += t.x * z.y + t.y * z.x;
+        }
+    }
+    C2 o;
+    o.x = sr;
+    o.y = -si;
+    W[(size_t)c * kNB + l] = o;
+}
+
 // R x = b (upper, non-unit), one workgroup per right-hand side, in place in B.
 template <typename T>
 __global__ __launch_bounds__(1024) void trsv_upper(const typename QT<T>::c2 *Rm, long ldr, int N,
@@ -308,10 +354,17 @@ struct QRFactor {
     C2 *Tb = nullptr;  // per-panel T factors
     C2 *W = nullptr;   // workspace nb x max(ncols)
     int wcols = 0;
+    C2 *Vd = nullptr;  // explicit panel V (FP32 MFMA trailing update)
+    C2 *Z = nullptr;   // (C^H V) of the trailing columns
+    char *gws = nullptr;
+    long ldv = 0;
     hipStream_t st = nullptr;
     ~QRFactor() {
         sfree(Tb, st);
         sfree(W, st);
+        sfree(Vd, st);
+        sfree(Z, st);
+        sfree(gws, st);
     }
 };
 
@@ -340,6 +393,20 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
     HOST_CALL(salloc(&f.Tb, (size_t)np * kNB * kNB, s));
     f.wcols = std::max(f.N, extra_cols);
     HOST_CALL(salloc(&f.W, (size_t)kNB * f.wcols, s));
+    // FP32: trailing updates on MFMA (Z = C^H V with the adjoint product, then
+    // C -= V W with the rank-32 update); FP64 keeps the VALU kernels
+    const bool mfma = sizeof(T) == 4 && !getenv("BEM_QR_VALU");
+    if (mfma) {
+        f.ldv = (f.lda + 63) / 64 * 64;
+        HOST_CALL(salloc(&f.Vd, (size_t)f.ldv * kNB, s));
+        HOST_CALL(salloc(&f.Z, (size_t)kNB * f.N, s));
+        size_t wsz = 0;
+        for (int p = 0; p < np; p++) {
+            const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
+            if (f.N - (j0 + nb) > 0) wsz = std::max(wsz, ws_gemm_h_f32(f.M - j0, f.N - (j0 + nb), kNB));
+        }
+        HOST_CALL(salloc(&f.gws, std::max<size_t>(wsz, 1), s));
+    }
     for (int p = 0; p < np; p++) {
         const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
         TimedRegion tr;
@@ -352,11 +419,27 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
             C2 *Ct = f.A + (size_t)(j0 + nb) * f.lda;
             TimedRegion tu;
             timing_begin(tu, sizeof(T) == 4 ? "qr_trailing_f32" : "qr_trailing_f64", s);
-            qr_vhc<T><<<nc, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Tb + (size_t)p * kNB * kNB, Ct, f.lda, f.W, 1);
-            HIP_CALL(hipGetLastError());
-            dim3 g((unsigned)((f.M - j0 + 255) / 256), (unsigned)nc);
-            qr_update<T><<<g, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.W, Ct, f.lda);
-            HIP_CALL(hipGetLastError());
+            if (mfma) {
+                const int Mp = f.M - j0;
+                float2 *Cb = (float2 *)(Ct + j0);  // rows j0.., columns j0+nb..
+                dim3 gv((unsigned)((Mp + 255) / 256), kNB);
+                k_build_v<T><<<gv, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Vd, f.ldv);
+                HIP_CALL(hipGetLastError());
+                HOST_CALL(gemm_h_f32(Cb, f.lda, Mp, nc, (const float2 *)f.Vd, f.ldv, (float2 *)f.Z, f.N, kNB, f.gws,
+                                     s));
+                dim3 gw((unsigned)((nc + 255) / 256), kNB);
+                k_qr_w<T><<<gw, 256, 0, s>>>(f.Tb + (size_t)p * kNB * kNB, nb, f.Z, f.N, nc, f.W);
+                HIP_CALL(hipGetLastError());
+                HOST_CALL(rank_update_f32((const float2 *)f.Vd, f.ldv, Mp, (const float2 *)f.W, kNB, nc, Cb, f.lda,
+                                          s));
+            } else {
+                qr_vhc<T><<<nc, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Tb + (size_t)p * kNB * kNB, Ct, f.lda,
+                                             f.W, 1);
+                HIP_CALL(hipGetLastError());
+                dim3 g((unsigned)((f.M - j0 + 255) / 256), (unsigned)nc);
+                qr_update<T><<<g, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.W, Ct, f.lda);
+                HIP_CALL(hipGetLastError());
+            }
             timing_end(tu, s);
         }
     }
