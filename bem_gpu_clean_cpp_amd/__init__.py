@@ -134,6 +134,9 @@ _SIGS = {
     "bem_solver_dims": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "bem_solver_stats": (_I, [_P, ctypes.POINTER(bem_solve_stats)]),
     "bem_solver_get_rows": (_I, [_P, _P, _I, _P]),
+    "bem_solver_field_matrix": (_I, [_P, _P, _I]),
+    "bem_solver_field_apply": (_I, [_P, _P, _I]),
+    "bem_solver_get_field_matrix": (_I, [_P, _P, _I]),
     "bem_solver_set_mesh": (_I, [_P, _P]),
     "bem_solver_release": (_I, [_P]),
     "bem_batch_solve": (_I, [_P, _P, _I, _I, _I, _P, _I, _I, _F, ctypes.POINTER(bem_solve_params), _I, _P,
@@ -442,6 +445,28 @@ class Solver:
         out = np.zeros((r.size, N), np.complex128)
         _check(lib().bem_solver_get_rows(self._h, _ptr(r), r.size, _ptr(out)), "get_rows")
         return out
+
+    def field_matrix(self, pts):
+        """Assemble the radiation operator of the field points (kept in HBM)."""
+        p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        _check(lib().bem_solver_field_matrix(self._h, _ptr(p), p.shape[0]), "field_matrix")
+        self._fP = p.shape[0]
+        return self
+
+    def field_apply(self):
+        """Field at the points of field_matrix for every current solution (P x nrhs)."""
+        M, N, R = self.dims()
+        P = self._fP
+        out = np.zeros((R, P), np.complex64)
+        _check(lib().bem_solver_field_apply(self._h, _ptr(out), P), "field_apply")
+        return out.T.copy()
+
+    def get_field_matrix(self):
+        M, N, R = self.dims()
+        P = self._fP
+        F = np.zeros((N, P), np.complex64)
+        _check(lib().bem_solver_get_field_matrix(self._h, _ptr(F), P), "get_field_matrix")
+        return F.T.copy()
 
     def field(self, pts):
         p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)

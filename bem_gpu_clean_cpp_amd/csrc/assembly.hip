@@ -81,6 +81,7 @@ struct AsmArgs {
     long ldA;
     long Mp;
     int M, N, nrbg, nwg;
+    int r0;  // first row assembled (rows r0..M-1 land in A rows 0..M-r0-1)
     R k, kf, rw;  // k, k/(2 pi), rho*omega
     int impedance;  // any element with ca != 0
     QTab<R> q;
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(256) void asm_tile(AsmArgs<R> a) {
     const int L = xcd_remap(blockIdx.x, a.nwg);
     const int tile = L / a.nrbg;
     const int rbg = L - tile * a.nrbg;
-    const int row = (rbg * 4 + wave) * RPW + lane / QG;
+    const int row = a.r0 + (rbg * 4 + wave) * RPW + lane / QG;
     const int qg = (QG == 1) ? 0 : (lane % QG);
     const bool active = row < a.M;
     const float4 xp = a.pts[active ? row : 0];
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(256) void asm_tile(AsmArgs<R> a) {
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 if ((ent >> j) & 1) {
-                    C2 *dst = a.A + (size_t)rec.nodes[j] * a.ldA + row;
+                    C2 *dst = a.A + (size_t)rec.nodes[j] * a.ldA + (row - a.r0);
                     C2 v;
                     if ((ent >> (3 + j)) & 1) {
                         v.x = pr[j];
@@ -375,11 +376,11 @@ __global__ __launch_bounds__(256) void asm_tile(AsmArgs<R> a) {
         }
     }
     if (active && (QG == 1 || qg == 0)) {
-        a.cpart[(size_t)tile * a.Mp + row] = c_acc;
+        a.cpart[(size_t)tile * a.Mp + (row - a.r0)] = c_acc;
         C2 b;
         b.x = br_acc;
         b.y = bi_acc;
-        a.bpart[(size_t)tile * a.Mp + row] = b;
+        a.bpart[(size_t)tile * a.Mp + (row - a.r0)] = b;
     }
 }
 
@@ -433,7 +434,7 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
     const int L = xcd_remap(blockIdx.x, a.nwg);
     const int tile = L / a.nrbg;
     const int rbg = L - tile * a.nrbg;
-    const int row = (rbg * 4 + wave) * 64 + lane;
+    const int row = a.r0 + (rbg * 4 + wave) * 64 + lane;
     const bool active = row < a.M;
     const float4 xp = a.pts[active ? row : 0];
     const float k = a.k;
@@ -540,7 +541,7 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 if ((ent >> j) & 1) {
-                    float2 *dst = a.A + (size_t)rec.nodes[j] * a.ldA + row;
+                    float2 *dst = a.A + (size_t)rec.nodes[j] * a.ldA + (row - a.r0);
                     float2 v;
                     if ((ent >> (3 + j)) & 1) {
                         v.x = pr[j];
@@ -561,8 +562,8 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
         }
     }
     if (active) {
-        a.cpart[(size_t)tile * a.Mp + row] = c_acc;
-        a.bpart[(size_t)tile * a.Mp + row] = make_float2(br_acc, bi_acc);
+        a.cpart[(size_t)tile * a.Mp + (row - a.r0)] = c_acc;
+        a.bpart[(size_t)tile * a.Mp + (row - a.r0)] = make_float2(br_acc, bi_acc);
     }
 }
 
@@ -579,11 +580,13 @@ __global__ void zero_columns(C2 *A, long ldA, int M, const int *__restrict__ col
 }
 
 // Per row: fixed-order sum of the tile partials in FP64.
+// Rows r0..M-1 (local index i = row - r0 in A, the partials and bterm).
 template <typename R, typename C2>
-__global__ void asm_finalize(C2 *A, long ldA, int M, int N, int P, long Mp, const R *__restrict__ cpart,
+__global__ void asm_finalize(C2 *A, long ldA, int M, int N, int P, long Mp, int r0, const R *__restrict__ cpart,
                              const C2 *__restrict__ bpart, double2 *__restrict__ bterm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
+    const int row = r0 + i;
+    if (row >= M) return;
     double cs = 0, br = 0, bi = 0;
     for (int t = 0; t < P; t++) {
         cs += (double)cpart[(size_t)t * Mp + i];
@@ -591,8 +594,8 @@ __global__ void asm_finalize(C2 *A, long ldA, int M, int N, int P, long Mp, cons
         br += (double)b.x;
         bi += (double)b.y;
     }
-    if (i < N) {
-        C2 *d = A + (size_t)i * ldA + i;
+    if (row < N) {
+        C2 *d = A + (size_t)row * ldA + i;
         C2 v = *d;
         v.x = (R)((double)v.x + (1.0 - cs));
         *d = v;
@@ -966,8 +969,10 @@ static void sing_tab(SingTab &t) {
     }
 }
 
+// Rows r0..M-1 of the system into A (ld ldA, row r0 first) and their
+// boundary-condition sums into bterm; r0 = 0 is the whole collocation system.
 template <typename R, typename C2>
-static int assemble_impl(DeviceProblem &dp, R k, C2 *A, hipStream_t s) {
+static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double2 *bterm, hipStream_t s) {
     HOST_CALL(ensure_quad());
     if (dp.M == 0) BEM_FAIL("device problem not built");
     const Config &cfg = config();
@@ -995,12 +1000,13 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, hipStream_t s) {
     a.A = A;
     a.cpart = sizeof(R) == 4 ? (R *)dp.cpart : (R *)dp.cpart64;
     a.bpart = sizeof(R) == 4 ? (C2 *)dp.bpart : (C2 *)dp.bpart64;
-    a.ldA = dp.ldA;
+    a.ldA = ldA;
     a.Mp = dp.Mp;
     a.M = dp.M;
     a.N = dp.N;
+    a.r0 = r0;
     const int rpw = 64 / dp.qg;
-    const int nrb = (dp.M + rpw - 1) / rpw;
+    const int nrb = (dp.M - r0 + rpw - 1) / rpw;
     a.nrbg = (nrb + 3) / 4;
     a.nwg = a.nrbg * dp.P;
     a.k = k;
@@ -1029,25 +1035,28 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, hipStream_t s) {
         HIP_CALL(hipGetLastError());
         timing_end(tr, s);
     }
+    const int nrows = dp.M - r0;
     if (dp.n_isolated > 0) {
-        dim3 grid((dp.M + 255) / 256, dp.n_isolated);
-        zero_columns<C2><<<grid, 256, 0, s>>>(A, dp.ldA, dp.M, dp.isolated, dp.n_isolated);
+        dim3 grid((nrows + 255) / 256, dp.n_isolated);
+        zero_columns<C2><<<grid, 256, 0, s>>>(A, ldA, nrows, dp.isolated, dp.n_isolated);
         HIP_CALL(hipGetLastError());
     }
     {
         TimedRegion tr;
         timing_begin(tr, "asm_finalize", s);
-        asm_finalize<R, C2><<<(dp.M + 255) / 256, 256, 0, s>>>(A, dp.ldA, dp.M, dp.N, dp.P, dp.Mp, a.cpart, a.bpart,
-                                                            dp.bterm);
+        asm_finalize<R, C2><<<(nrows + 255) / 256, 256, 0, s>>>(A, ldA, dp.M, dp.N, dp.P, dp.Mp, r0, a.cpart,
+                                                             a.bpart, bterm);
         HIP_CALL(hipGetLastError());
         timing_end(tr, s);
     }
     return EXIT_SUCCESS;
 }
 
-int assemble_fp32(DeviceProblem &dp, float k, float2 *A, hipStream_t s) { return assemble_impl<float, float2>(dp, k, A, s); }
+int assemble_fp32(DeviceProblem &dp, float k, float2 *A, hipStream_t s) {
+    return assemble_impl<float, float2>(dp, k, A, dp.ldA, 0, dp.bterm, s);
+}
 int assemble_fp64(DeviceProblem &dp, double k, double2 *A, hipStream_t s) {
-    return assemble_impl<double, double2>(dp, k, A, s);
+    return assemble_impl<double, double2>(dp, k, A, dp.ldA, 0, dp.bterm, s);
 }
 
 template <typename C2>
@@ -1061,6 +1070,59 @@ static int rhs_impl(DeviceProblem &dp, double k, C2 *B, long ldB, const RhsSpec 
     rhs_fill<C2><<<grid, 256, 0, s>>>(dp.pts64, dp.M, k, rhs.kind, (double)rhs.amp, params, rhs.nrhs, dp.bterm, B, ldB);
     hipError_t e = hipGetLastError();
     (void)hipStreamSynchronize(s);
+    sfree(params, s);
+    HIP_CALL(e);
+    return EXIT_SUCCESS;
+}
+
+// ------------------------------------------------- field as a GEMM ---
+// fdp is the mesh with the P field points as its extra rows (the CHIEF
+// position).  Rows N..N+P-1 of the collocation formula are the radiation
+// operator: p(x) = inc(x) - bterm(x) - sum_j F[x, j] p_j (genExtPressure,
+// mesh.cu:1038-1073, as a matrix).  F is P x N (ldF), fbterm P values.
+int field_matrix_fp32(DeviceProblem &fdp, float k, float2 *F, long ldF, double2 *fbterm, hipStream_t s) {
+    if (fdp.M <= fdp.N) BEM_FAIL("field problem has no field points");
+    if (ldF < fdp.M - fdp.N) BEM_FAIL("ldF must be >= P");
+    return assemble_impl<float, float2>(fdp, k, F, ldF, fdp.N, fbterm, s);
+}
+
+__global__ void k_sub_c(float2 *out, long ldo, const float2 *__restrict__ Y, long ldy, int P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= P) return;
+    const float2 y = Y[(size_t)r * ldy + i];
+    float2 o = out[(size_t)r * ldo + i];
+    o.x -= y.x;
+    o.y -= y.y;
+    out[(size_t)r * ldo + i] = o;
+}
+
+// out (P x R, ldo) = inc_r(x_p) - bterm_p - (F X)_p for the FP32 solutions X
+// (N x R, ldx): one GEMV (R = 1) or one MFMA product (R > 1) over F.
+// ws >= max(ws_gemv_n(P, N, 1), ws_gemm(P, N, R)); Y: P x R scratch (ldF).
+int field_apply_fp32(DeviceProblem &fdp, double k, const float2 *F, long ldF, const double2 *fbterm, const float2 *X,
+                     long ldx, const RhsSpec &rhs, float2 *Y, float2 *out, long ldo, void *ws, hipStream_t s) {
+    const int N = fdp.N, P = fdp.M - fdp.N, R = rhs.nrhs;
+    if (R <= 0 || P <= 0) return EXIT_SUCCESS;
+    {
+        TimedRegion tr;
+        timing_begin(tr, "field_gemm", s);
+        if (R == 1) HOST_CALL(gemv_n_f32(F, ldF, P, N, X, 0, Y, 0, 1, ws, s));
+        else HOST_CALL(gemm_n_f32(F, ldF, P, N, X, ldx, Y, ldF, R, ws, s));
+        timing_end(tr, s);
+    }
+    float *params = nullptr;
+    HOST_CALL(salloc(&params, 3 * (size_t)R, s));
+    HIP_CALL(hipMemcpyAsync(params, rhs.params.data(), 3 * sizeof(float) * R, hipMemcpyHostToDevice, s));
+    dim3 grid((P + 255) / 256, R);
+    rhs_fill<float2><<<grid, 256, 0, s>>>(fdp.pts64 + 3 * (size_t)N, P, k, rhs.kind, (double)rhs.amp, params, R,
+                                          fbterm, out, ldo);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        k_sub_c<<<grid, 256, 0, s>>>(out, ldo, Y, ldF, P);
+        e = hipGetLastError();
+    }
+    (void)hipStreamSynchronize(s);  // params is a host-side copy source
     sfree(params, s);
     HIP_CALL(e);
     return EXIT_SUCCESS;

@@ -174,6 +174,11 @@ int rhs_fp64(DeviceProblem &dp, double k, double2 *B, long ldB, const RhsSpec &r
 int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2 *X, long ldx, int R,
                const RhsSpec &rhs, float2 *out, long ldo, hipStream_t s);
 
+// Exterior field as a GEMM: fdp = mesh with the field points as extra rows.
+int field_matrix_fp32(DeviceProblem &fdp, float k, float2 *F, long ldF, double2 *fbterm, hipStream_t s);
+int field_apply_fp32(DeviceProblem &fdp, double k, const float2 *F, long ldF, const double2 *fbterm, const float2 *X,
+                     long ldx, const RhsSpec &rhs, float2 *Y, float2 *out, long ldo, void *ws, hipStream_t s);
+
 // ---------------------------------------------------------- linear algebra ---
 size_t ws_gemv_n(long M, long N, int R);
 int gemv_n_f32(const float2 *A, long lda, long M, long N, const float2 *x, long ldx, float2 *y, long ldy, int R,

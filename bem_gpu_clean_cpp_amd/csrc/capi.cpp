@@ -408,7 +408,11 @@ static void solver_free_mats(bem_solver *s) {
     sfree(s->B64, s->stream);
     sfree(s->X64, s->stream);
     sfree(s->X32, s->stream);
-    s->a_cap = s->x_cap = 0;
+    sfree(s->F, s->stream);
+    sfree(s->fbterm, s->stream);
+    device_problem_free(s->fdp);
+    s->a_cap = s->x_cap = s->f_cap = 0;
+    s->fP = 0;
     s->xr = 0;
     s->prec = 0;
 }
@@ -688,6 +692,78 @@ int bem_solver_field(bem_solver *sv, const bem_cart_coord *pts, int P, bem_compl
     sfree(od, s);
     HOST_CALL(rc);
     HIP_CALL(e);
+    return EXIT_SUCCESS;
+}
+
+int bem_solver_field_matrix(bem_solver *sv, const bem_cart_coord *pts, int P) {
+    if (!sv || !pts || P <= 0) BEM_FAIL("bad field-matrix arguments");
+    if (!sv->prec) BEM_FAIL("solver not assembled");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
+    sv->fmesh = sv->mesh;
+    sv->fmesh.chief.assign((const float *)pts, (const float *)pts + 3 * (size_t)P);
+    sv->fmesh.version++;
+    sv->fP = 0;
+    HOST_CALL(device_problem_build(sv->fmesh, sv->fdp, s));
+    const long ldF = ((long)P + 63) / 64 * 64;
+    const size_t need = (size_t)ldF * sv->dp.N;
+    if (sv->f_cap < need) {
+        sfree(sv->F, s);
+        HOST_CALL(salloc(&sv->F, need, s));
+        sv->f_cap = need;
+    }
+    sfree(sv->fbterm, s);
+    HOST_CALL(salloc(&sv->fbterm, (size_t)P, s));
+    sv->ldF = ldF;
+    HOST_CALL(field_matrix_fp32(sv->fdp, sv->k, sv->F, ldF, sv->fbterm, s));
+    HIP_CALL(hipStreamSynchronize(s));
+    sv->fP = P;
+    sv->fk = sv->k;
+    return EXIT_SUCCESS;
+}
+
+int bem_solver_field_apply(bem_solver *sv, bem_complex *out, int ldo) {
+    if (!sv || !out) BEM_FAIL("bad field-apply arguments");
+    if (sv->fP <= 0) BEM_FAIL("no field matrix (bem_solver_field_matrix)");
+    if (sv->fk != sv->k) BEM_FAIL("field matrix assembled at k = %g, solver at k = %g", sv->fk, sv->k);
+    if (sv->xr <= 0) BEM_FAIL("no solution");
+    if (ldo < sv->fP) BEM_FAIL("ldo must be >= P");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
+    const DeviceProblem &dp = sv->dp;
+    const int R = sv->rhs.nrhs, P = sv->fP;
+    const long n = dp.ldA * (long)R;
+    k_z2c_copy<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(sv->X64, sv->X32, n);
+    HIP_CALL(hipGetLastError());
+    float2 *Y = nullptr, *od = nullptr;
+    char *ws = nullptr;
+    HOST_CALL(salloc(&Y, (size_t)sv->ldF * R, s));
+    HOST_CALL(salloc(&od, (size_t)sv->ldF * R, s));
+    HOST_CALL(salloc(&ws, std::max(ws_gemv_n(P, dp.N, 1), ws_gemm(P, dp.N, R)), s));
+    int rc = field_apply_fp32(sv->fdp, (double)sv->k, sv->F, sv->ldF, sv->fbterm, sv->X32, dp.ldA, sv->rhs, Y, od,
+                              sv->ldF, ws, s);
+    hipError_t e = hipSuccess;
+    if (rc == EXIT_SUCCESS)
+        e = hipMemcpy2DAsync(out, sizeof(bem_complex) * ldo, od, sizeof(float2) * sv->ldF, sizeof(float2) * P, R,
+                             hipMemcpyDeviceToHost, s);
+    (void)hipStreamSynchronize(s);
+    sfree(Y, s);
+    sfree(od, s);
+    sfree(ws, s);
+    HOST_CALL(rc);
+    HIP_CALL(e);
+    return EXIT_SUCCESS;
+}
+
+int bem_solver_get_field_matrix(const bem_solver *sv, bem_complex *F, int ldf) {
+    if (!sv || !F) BEM_FAIL("null argument");
+    if (sv->fP <= 0) BEM_FAIL("no field matrix (bem_solver_field_matrix)");
+    if (ldf < sv->fP) BEM_FAIL("ldf must be >= P");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    hipStream_t s = sv->stream;
+    HIP_CALL(hipMemcpy2DAsync(F, sizeof(bem_complex) * ldf, sv->F, sizeof(float2) * sv->ldF, sizeof(float2) * sv->fP,
+                              sv->dp.N, hipMemcpyDeviceToHost, s));
+    HIP_CALL(hipStreamSynchronize(s));
     return EXIT_SUCCESS;
 }
 

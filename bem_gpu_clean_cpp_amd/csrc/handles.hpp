@@ -27,6 +27,15 @@ struct bem_solver {
     size_t x_cap = 0;  // elements allocated for each of B64, X64, X32
     int xr = 0;        // columns of the last solution in X64
     bem_solve_stats stats{};
+    // exterior field as a matrix: the mesh with the field points as extra rows
+    bem::Mesh fmesh;
+    bem::DeviceProblem fdp;
+    float2 *F = nullptr;  // P x N radiation operator, ld ldF
+    double2 *fbterm = nullptr;
+    long ldF = 0;
+    size_t f_cap = 0;
+    float fk = 0.f;
+    int fP = 0;
     hipStream_t stream = nullptr;  // each solver runs on its own stream
     mutable std::mutex mu;         // one call at a time per solver
 };

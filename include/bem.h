@@ -201,6 +201,17 @@ int bem_solver_get_system(const bem_solver *s, bem_complex *A, int lda, bem_comp
 int bem_solver_get_rows(const bem_solver *s, const int *rows, int nrows, bem_zcomplex *out);
 /* Exterior field at P points for every right-hand side: out is P x nrhs (ldo). */
 int bem_solver_field(bem_solver *s, const bem_cart_coord *pts, int P, bem_complex *out, int ldo);
+/* Exterior field as a matrix (genExtPressure, mesh.cu:1038-1073, for many
+ * points and solutions at once): assemble the P x N radiation operator F of
+ * the field points at the solver's current wavenumber with the collocation
+ * kernel (the field points are rows off the surface, like CHIEF rows), and
+ * keep it in HBM.  bem_solver_field_apply then evaluates
+ * out (P x nrhs, ldo) = incident - boundary term - F x for every current
+ * solution with one MFMA product; bem_solver_get_field_matrix copies F
+ * (complex64, column-major P x N, ldf >= P) to the host. */
+int bem_solver_field_matrix(bem_solver *s, const bem_cart_coord *pts, int P);
+int bem_solver_field_apply(bem_solver *s, bem_complex *out, int ldo);
+int bem_solver_get_field_matrix(const bem_solver *s, bem_complex *F, int ldf);
 /* Sizes: M rows, N columns, current nrhs. */
 int bem_solver_dims(const bem_solver *s, int *M, int *N, int *nrhs);
 /* Last solve statistics. */
