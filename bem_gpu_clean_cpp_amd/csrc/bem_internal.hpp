@@ -201,6 +201,10 @@ int gemm_n_f32(const float2 *A, long lda, long M, long N, const float2 *X, long 
                void *ws, hipStream_t s);
 int gemm_h_f32(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R,
                void *ws, hipStream_t s);
+int gemm_n_f32z(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, double2 *Y, long ldy, int R,
+                void *ws, hipStream_t s);
+int gemm_h_f32z(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R,
+                void *ws, hipStream_t s);
 size_t ws_mres(long M, long N, int R);
 // C -= V W (QR trailing update, V with 32 columns)
 int rank_update_f32(const float2 *V, long ldv, int M, const float2 *W, int ldw, int nc, float2 *C, long ldc,

@@ -280,25 +280,26 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
 }
 
 // Y[n][i] = (b[n][i] -) sum_chunks part[(chunk * R + n) * ldp + i]  (fixed order)
-template <typename T>
+// (partials of type MF<T>::c2, output OC2 accumulated in S)
+template <typename T, typename S, typename OC2>
 __global__ __launch_bounds__(256) void gemm_reduce(const typename MF<T>::c2 *__restrict__ part, long ldp, int nchunks,
-                                                   int R, int M, const typename MF<T>::c2 *__restrict__ b, long ldb,
-                                                   typename MF<T>::c2 *__restrict__ Y, long ldy) {
+                                                   int R, int M, const OC2 *__restrict__ b, long ldb,
+                                                   OC2 *__restrict__ Y, long ldy) {
     typedef typename MF<T>::c2 C2;
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int n = blockIdx.y;
     if (i >= M) return;
-    T sr = 0, si = 0;
+    S sr = 0, si = 0;
     for (int c = 0; c < nchunks; c++) {
         const C2 p = part[((size_t)c * R + n) * ldp + i];
-        sr += p.x;
-        si += p.y;
+        sr += (S)p.x;
+        si += (S)p.y;
     }
     if (b) {
-        const C2 bb = b[(size_t)n * ldb + i];
-        Y[(size_t)n * ldy + i] = C2{bb.x - sr, bb.y - si};
+        const OC2 bb = b[(size_t)n * ldb + i];
+        Y[(size_t)n * ldy + i] = OC2{bb.x - sr, bb.y - si};
     } else {
-        Y[(size_t)n * ldy + i] = C2{sr, si};
+        Y[(size_t)n * ldy + i] = OC2{sr, si};
     }
 }
 
@@ -332,10 +333,9 @@ size_t ws_gemm(long M, long N, int R) {
     return (size_t)std::max(a, c) * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
 }
 
-template <typename T>
+template <typename T, typename S = T, typename OC2 = typename MF<T>::c2>
 static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *X, long ldx,
-                       const typename MF<T>::c2 *b, long ldb, typename MF<T>::c2 *Y, long ldy, int R, void *ws,
-                       const char *name, hipStream_t s) {
+                       const OC2 *b, long ldb, OC2 *Y, long ldy, int R, void *ws, const char *name, hipStream_t s) {
     if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
     if (lda % 2 || lda < (M + 31) / 32 * 32) BEM_FAIL("gemm_n needs an even lda >= roundup(M, 32)");
     int nch, kc;
@@ -355,14 +355,14 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
     }
     timing_end(tr, s);
     dim3 g2((unsigned)((M + 255) / 256), R);
-    gemm_reduce<T><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)M, b, ldb, Y, ldy);
+    gemm_reduce<T, S, OC2><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)M, b, ldb, Y, ldy);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
 
-template <typename T>
-static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *Tv, long ldt,
-                       typename MF<T>::c2 *Z, long ldz, int R, void *ws, const char *name, hipStream_t s) {
+template <typename T, typename S = T, typename OC2 = typename MF<T>::c2>
+static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *Tv, long ldt, OC2 *Z,
+                       long ldz, int R, void *ws, const char *name, hipStream_t s) {
     if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
     if (lda % 2) BEM_FAIL("gemm_h needs an even lda");
     int nch, kr;
@@ -382,7 +382,7 @@ static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename
     }
     timing_end(tr, s);
     dim3 g2((unsigned)((N + 255) / 256), R);
-    gemm_reduce<T><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)N, nullptr, 0, Z, ldz);
+    gemm_reduce<T, S, OC2><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)N, (const OC2 *)nullptr, 0, Z, ldz);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
@@ -390,13 +390,24 @@ static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename
 // Y (M x R, ldy) = A X ; ws >= ws_gemm(M, N, R)
 int gemm_n_f32(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, float2 *Y, long ldy, int R,
                void *ws, hipStream_t s) {
-    return gemm_n_impl<float>(A, lda, M, N, X, ldx, nullptr, 0, Y, ldy, R, ws, "gemm_n_f32", s);
+    return gemm_n_impl<float>(A, lda, M, N, X, ldx, (const float2 *)nullptr, 0, Y, ldy, R, ws, "gemm_n_f32", s);
 }
 
 // Z (N x R, ldz) = A^H T, T: M x R (ldt); ws >= ws_gemm(M, N, R)
 int gemm_h_f32(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R,
                void *ws, hipStream_t s) {
     return gemm_h_impl<float>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "gemm_h_f32", s);
+}
+
+// FP32 products whose fixed-order reduce writes FP64 vectors (the Krylov basis input)
+int gemm_n_f32z(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, double2 *Y, long ldy, int R,
+                void *ws, hipStream_t s) {
+    return gemm_n_impl<float, double, double2>(A, lda, M, N, X, ldx, (const double2 *)nullptr, 0, Y, ldy, R, ws,
+                                               "gemm_n_f32", s);
+}
+int gemm_h_f32z(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R,
+                void *ws, hipStream_t s) {
+    return gemm_h_impl<float, double, double2>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "gemm_h_f32", s);
 }
 
 // out (M x R) = b - A X (b == nullptr: out = A X): FP32 A widened to FP64,

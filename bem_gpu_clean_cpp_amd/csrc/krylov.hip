@@ -41,52 +41,40 @@ using zc = std::complex<double>;
 // ---------------------------------------------------------------- kernels ---
 // CGS2 steps for R systems at once: system r = blockIdx.y, its vectors at
 // fixed strides (basis vstride, vectors wstride, Hessenberg columns hstride).
+// part[blk * nv + l] of system r (blockIdx.y) = sum over the 1024 rows of
+// block blk of conj(V_l) w; one wave per basis vector l (4 per workgroup,
+// blockIdx.z selects the group), no workgroup barriers.
 __global__ __launch_bounds__(256) void kb_mdot_part(const double2 *__restrict__ V, long ldv, long vstride, int nv,
                                                     const double2 *__restrict__ w, long wstride, int n,
                                                     double2 *__restrict__ part, long pstride) {
     const int r = blockIdx.y;
-    V += (size_t)r * vstride;
-    w += (size_t)r * wstride;
-    part += (size_t)r * pstride;
-    __shared__ double red[2][4];
+    const int l = blockIdx.z * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (l >= nv) return;
+    const double2 *vl = V + (size_t)r * vstride + (size_t)l * ldv;
+    const double2 *wr = w + (size_t)r * wstride;
     const int base = blockIdx.x * 1024;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    double2 wv[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-        const int i = base + threadIdx.x + 256 * u;
-        wv[u] = i < n ? w[i] : make_double2(0, 0);
+    double sr = 0, si = 0;
+#pragma unroll 4
+    for (int u = 0; u < 16; u++) {
+        const int i = base + lane + 64 * u;
+        if (i < n) {
+            const double2 v = vl[i], x = wr[i];
+            sr = fma(v.x, x.x, fma(v.y, x.y, sr));
+            si = fma(v.x, x.y, fma(-v.y, x.x, si));
+        }
     }
-    for (int l = 0; l < nv; l++) {
-        double sr = 0, si = 0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int i = base + threadIdx.x + 256 * u;
-            if (i < n) {
-                const double2 v = V[(size_t)l * ldv + i];
-                sr = fma(v.x, wv[u].x, fma(v.y, wv[u].y, sr));
-                si = fma(v.x, wv[u].y, fma(-v.y, wv[u].x, si));
-            }
-        }
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-            sr += __shfl_xor(sr, m, 64);
-            si += __shfl_xor(si, m, 64);
-        }
-        if (lane == 0) {
-            red[0][wid] = sr;
-            red[1][wid] = si;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            part[(size_t)blockIdx.x * nv + l] = make_double2((red[0][0] + red[0][1]) + (red[0][2] + red[0][3]),
-                                                             (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
-        __syncthreads();
+    for (int m = 32; m >= 1; m >>= 1) {
+        sr += __shfl_xor(sr, m, 64);
+        si += __shfl_xor(si, m, 64);
     }
+    if (lane == 0) part[(size_t)r * pstride + (size_t)blockIdx.x * nv + l] = make_double2(sr, si);
 }
 
+// out[r * ostride + l] = sum_blocks part; with acc: acc[r * ostride + l] += the sum
 __global__ void kb_mdot_final(const double2 *__restrict__ part, long pstride, int nblk, int nv,
-                              double2 *__restrict__ out, long ostride) {
+                              double2 *__restrict__ out, long ostride, double2 *__restrict__ acc) {
     const int r = blockIdx.y, l = threadIdx.x;
     if (l >= nv) return;
     part += (size_t)r * pstride;
@@ -97,6 +85,28 @@ __global__ void kb_mdot_final(const double2 *__restrict__ part, long pstride, in
         si += p.y;
     }
     out[(size_t)r * ostride + l] = make_double2(sr, si);
+    if (acc) {
+        double2 a = acc[(size_t)r * ostride + l];
+        a.x += sr;
+        a.y += si;
+        acc[(size_t)r * ostride + l] = a;
+    }
+}
+
+// out[r * ostride] = (|w_r|^2, 0): one workgroup per system, fixed-order tree
+__global__ __launch_bounds__(1024) void kb_norm2(const double2 *__restrict__ w, long wstride, int n,
+                                                 double2 *__restrict__ out, long ostride) {
+    __shared__ double red[1024];
+    const double2 *wr = w + (size_t)blockIdx.x * wstride;
+    double sum = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) sum = fma(wr[i].x, wr[i].x, fma(wr[i].y, wr[i].y, sum));
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int m = 512; m >= 1; m >>= 1) {
+        if (threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)blockIdx.x * ostride] = make_double2(red[0], 0.0);
 }
 
 __global__ void kb_maxpy(const double2 *__restrict__ V, long ldv, long vstride, int nv, const double2 *__restrict__ h,
@@ -118,13 +128,6 @@ __global__ void kb_maxpy(const double2 *__restrict__ V, long ldv, long vstride, 
     o.x += sign * sr;
     o.y += sign * si;
     w[i] = o;
-}
-
-__global__ void kb_hadd(double2 *__restrict__ h1, const double2 *__restrict__ h2, long hstride, int nv) {
-    const int r = blockIdx.y, l = threadIdx.x;
-    if (l >= nv) return;
-    h1[(size_t)r * hstride + l].x += h2[(size_t)r * hstride + l].x;
-    h1[(size_t)r * hstride + l].y += h2[(size_t)r * hstride + l].y;
 }
 
 __global__ void kb_normalize(const double2 *__restrict__ w, long wstride, const double2 *__restrict__ nrm2,
@@ -188,15 +191,22 @@ struct KrylovWS {
     }
 };
 
-// batched dots <V_r[:, 0..nv), w_r> -> out[r * ostride + l]
+// batched dots <V_r[:, 0..nv), w_r> -> out[r * ostride + l] (and acc += out)
 static int bdot(KrylovWS &ws, const double2 *V, long vstride, int nv, const double2 *w, long wstride, int n,
-                double2 *out, long ostride, hipStream_t s) {
+                double2 *out, long ostride, hipStream_t s, double2 *acc = nullptr) {
     const int nblk = (n + 1023) / 1024;
-    dim3 g1(nblk, ws.R);
+    dim3 g1(nblk, ws.R, (nv + 3) / 4);
     kb_mdot_part<<<g1, 256, 0, s>>>(V, ws.ldv, vstride, nv, w, wstride, n, ws.part, ws.pstride);
     HIP_CALL(hipGetLastError());
     dim3 g2(1, ws.R);
-    kb_mdot_final<<<g2, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride);
+    kb_mdot_final<<<g2, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
+// |w_r|^2 -> out[r * ostride]
+static int bnorm2(KrylovWS &ws, const double2 *w, long wstride, int n, double2 *out, long ostride, hipStream_t s) {
+    kb_norm2<<<ws.R, 1024, 0, s>>>(w, wstride, n, out, ostride);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
@@ -220,17 +230,18 @@ struct Operator {
                     HOST_CALL(gemv_n_f32(A32, lda, M, N, ws.v32, 0, ws.t32, 0, 1, ws.gws, s));
                     HOST_CALL(gemv_h_f32(A32, lda, M, N, ws.t32, 0, ws.w32, 0, 1, s));
                 }
+                dim3 g((N + bs - 1) / bs, R);
+                kb_cvt<<<g, bs, 0, s>>>(ws.w32, ws.ldv, ws.w64, ws.ldv, N);
+                HIP_CALL(hipGetLastError());
             } else {
+                // the products' fixed-order reduce writes the FP64 vector directly
                 if (!normal()) {
-                    HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.w32, ws.ldv, R, ws.gws, s));
+                    HOST_CALL(gemm_n_f32z(A32, lda, M, N, ws.v32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
                 } else {
                     HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.t32, ws.ldv, R, ws.gws, s));
-                    HOST_CALL(gemm_h_f32(A32, lda, M, N, ws.t32, ws.ldv, ws.w32, ws.ldv, R, ws.gws, s));
+                    HOST_CALL(gemm_h_f32z(A32, lda, M, N, ws.t32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
                 }
             }
-            dim3 g((N + bs - 1) / bs, R);
-            kb_cvt<<<g, bs, 0, s>>>(ws.w32, ws.ldv, ws.w64, ws.ldv, N);
-            HIP_CALL(hipGetLastError());
         } else {
             for (int q = 0; q < R; q++) {
                 const double2 *vj = ws.V + (size_t)q * ws.vstride + (size_t)j * ws.ldv;
@@ -325,7 +336,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     HOST_CALL(op.rhs(ws, B, ldb, ws.r, s));
     std::vector<double2> hbuf((size_t)ws.hstride * R);
     auto norms = [&](const double2 *w, std::vector<double> &out) -> int {
-        HOST_CALL(bdot(ws, w, ldv, 1, w, ldv, N, ws.h, ws.hstride, s));
+        HOST_CALL(bnorm2(ws, w, ldv, N, ws.h, ws.hstride, s));
         HIP_CALL(hipMemcpyAsync(hbuf.data(), ws.h, sizeof(double2) * ws.hstride * R, hipMemcpyDeviceToHost, s));
         HIP_CALL(hipStreamSynchronize(s));
         for (int q = 0; q < R; q++) out[q] = std::sqrt(std::max(0.0, hbuf[(size_t)q * ws.hstride].x));
@@ -386,12 +397,10 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             HOST_CALL(bdot(ws, ws.V, ws.vstride, j + 1, ws.w64, ldv, N, h1, ws.hstride, s));
             kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, j + 1, h1, ws.hstride, -1.0, ws.w64, ldv, N);
             HIP_CALL(hipGetLastError());
-            HOST_CALL(bdot(ws, ws.V, ws.vstride, j + 1, ws.w64, ldv, N, h2, ws.hstride, s));
+            HOST_CALL(bdot(ws, ws.V, ws.vstride, j + 1, ws.w64, ldv, N, h2, ws.hstride, s, h1));  // h1 += h2
             kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, j + 1, h2, ws.hstride, -1.0, ws.w64, ldv, N);
             HIP_CALL(hipGetLastError());
-            kb_hadd<<<dim3(1, R), 64 * ((j + 1 + 63) / 64), 0, s>>>(h1, h2, ws.hstride, j + 1);
-            HIP_CALL(hipGetLastError());
-            HOST_CALL(bdot(ws, ws.w64, ldv, 1, ws.w64, ldv, N, h2, ws.hstride, s));
+            HOST_CALL(bnorm2(ws, ws.w64, ldv, N, h2, ws.hstride, s));
             kb_normalize<<<gg, bs, 0, s>>>(ws.w64, ldv, h2, ws.hstride, ws.V + (size_t)(j + 1) * ldv, ws.vstride,
                                            op.A32 ? ws.v32 : nullptr, ldv, N);
             HIP_CALL(hipGetLastError());

@@ -107,14 +107,20 @@ int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, in
             BEM_FAIL("batch: solver creation failed: %s", e.c_str());
         }
     }
+    // highest wavenumbers first: their solves take the most iterations, so
+    // starting them first keeps the workers' finishing times close
+    std::vector<int> order(nk);
+    for (int f = 0; f < nk; f++) order[f] = f;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return k[a] > k[b]; });
     std::atomic<int> next{0};
     std::atomic<int> failed{0};
     std::string first_error;
     std::mutex emu;
     auto work = [&](int w) {
         for (;;) {
-            const int f = next.fetch_add(1);
-            if (f >= nk || failed.load()) break;
+            const int q = next.fetch_add(1);
+            if (q >= nk || failed.load()) break;
+            const int f = order[q];
             bem_zcomplex *xf = X ? X + (size_t)f * nrhs * ldx : nullptr;
             bem_solve_stats st{};
             const float *pf = params_per_freq ? params + (size_t)f * 3 * nrhs : params;
