@@ -70,6 +70,14 @@ class bem_request_info(ctypes.Structure):
         return d
 
 
+class bem_tune_result(ctypes.Structure):
+    _fields_ = [("lanes_per_row", ctypes.c_int), ("waves", ctypes.c_int), ("best_ms", ctypes.c_double),
+                ("default_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class bem_server_info(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int), ("running", ctypes.c_int), ("max_running", ctypes.c_int),
                 ("slots", ctypes.c_int), ("budget_bytes", ctypes.c_double), ("in_use_bytes", ctypes.c_double),
@@ -147,6 +155,10 @@ _SIGS = {
     "bem_server_status": (_I, [_P, ctypes.POINTER(bem_server_info)]),
     "bem_server_destroy": (None, [_P]),
     "bem_request_footprint": (_D, [_I, _I, _I, _I, _I, _I, _I]),
+    "bem_autotune_assembly": (_I, [_P, _F, _I, ctypes.POINTER(bem_tune_result)]),
+    "bem_tune_save": (_I, [ctypes.c_char_p]),
+    "bem_tune_load": (_I, [ctypes.c_char_p]),
+    "bem_tune_clear": (_I, []),
     "bem_timing_enable": (_I, [_I]),
     "bem_timing_reset": (_I, []),
     "bem_timing_get": (_I, [ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long)]),
@@ -557,6 +569,27 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+# ---------------------------------------------------------------- tuning ---
+def autotune_assembly(mesh, k, precision=PREC_FP32):
+    """Time candidate assembly launch shapes for this mesh; the fastest is kept
+    for meshes of its size class.  Returns the choice and both timings."""
+    r = bem_tune_result()
+    _check(lib().bem_autotune_assembly(mesh.handle, k, precision, ctypes.byref(r)), "autotune_assembly")
+    return r.as_dict()
+
+
+def tune_save(path):
+    _check(lib().bem_tune_save(str(path).encode()), "tune_save")
+
+
+def tune_load(path):
+    _check(lib().bem_tune_load(str(path).encode()), "tune_load")
+
+
+def tune_clear():
+    _check(lib().bem_tune_clear(), "tune_clear")
 
 
 # ---------------------------------------------------------------- timing ---

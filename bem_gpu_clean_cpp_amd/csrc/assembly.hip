@@ -26,6 +26,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "bem_internal.hpp"
 
@@ -822,11 +826,44 @@ int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2
 }
 
 // --------------------------------------------------------------- host ---
-static int choose_tiles(int M, int N, int rows_per_wave) {
+// Launch shape of the assembly: qg lanes per row (1 or 4) and P column tiles,
+// P x ceil(M / (64 / qg)) waves in all.  More tiles cost more halo elements
+// (an element straddling tiles is integrated once per tile) but give more
+// waves to balance over the 1024 SIMDs; the default aims at kDefaultWaves.
+// bem_autotune_assembly times candidate shapes for a mesh on the device and
+// records the fastest for its size class (M and E in half-octave buckets),
+// which device problems of that class built afterwards use.
+constexpr int kDefaultWaves = 16384;
+
+namespace {
+struct TileShape {
+    int qg = 0, waves = 0;
+};
+std::mutex g_tune_mu;
+std::map<std::pair<int, int>, TileShape> g_tune;
+thread_local const TileShape *t_force = nullptr;  // candidate under test
+
+int half_octave(long v) { return (int)std::lround(2.0 * std::log2((double)std::max(1L, v))); }
+
+bool tuned_shape(int M, int E, TileShape &out) {
+    if (t_force) {
+        out = *t_force;
+        return true;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find({half_octave(M), half_octave(E)});
+    if (it == g_tune.end()) return false;
+    out = it->second;
+    return true;
+}
+}  // namespace
+
+static int choose_tiles(int M, int N, int E, int rows_per_wave) {
     const char *env = getenv("BEM_TILES");
     if (env && atoi(env) > 0) return std::min(atoi(env), N);
+    TileShape t;
+    const int target_waves = tuned_shape(M, E, t) ? t.waves : kDefaultWaves;
     const int nrb = (M + rows_per_wave - 1) / rows_per_wave;
-    const int target_waves = 4096;
     int P = (target_waves + nrb - 1) / nrb;
     P = std::max(1, std::min(P, std::max(1, N / 8)));
     return P;
@@ -838,7 +875,94 @@ static int choose_qg(int M, int E) {
         int q = atoi(env);
         if (q == 1 || q == 4) return q;
     }
+    TileShape t;
+    if (tuned_shape(M, E, t)) return t.qg;
     return ((long)M * E < 4000000L) ? 4 : 1;
+}
+
+// Times the FP32 (or FP64) assembly of mesh m at wavenumber k for each
+// candidate shape (2 warm + 3 timed launches each, HIP events on stream s)
+// and records the fastest for the mesh's size class.
+int autotune_assembly(const Mesh &m, double k, int precision, int *best_qg, int *best_waves, double *best_ms,
+                      double *default_ms, hipStream_t s) {
+    const int M = m.N() + m.C(), E = m.E();
+    std::vector<TileShape> cand;
+    for (int qg : {1, 4})
+        for (int w : {4096, 8192, 16384, 32768}) cand.push_back(TileShape{qg, w});
+    TileShape def{((long)M * E < 4000000L) ? 4 : 1, kDefaultWaves};
+    double bms = 1e300, dms = -1;
+    TileShape best = def;
+    hipEvent_t e0, e1;
+    HIP_CALL(hipEventCreate(&e0));
+    HIP_CALL(hipEventCreate(&e1));
+    size_t n = ((size_t)M + 63) / 64 * 64 * (size_t)m.N();
+    void *A = nullptr;
+    HIP_CALL(hipMallocAsync(&A, n * (precision == BEM_PREC_FP64 ? sizeof(double2) : sizeof(float2)), s));
+    int rc = EXIT_SUCCESS;
+    for (const TileShape &c : cand) {
+        DeviceProblem dp;
+        t_force = &c;
+        rc = device_problem_build(m, dp, s);
+        t_force = nullptr;
+        if (rc != EXIT_SUCCESS) break;
+        float ms = 0;
+        for (int rep = 0; rep < 5 && rc == EXIT_SUCCESS; rep++) {
+            if (rep == 2) (void)hipEventRecord(e0, s);
+            rc = precision == BEM_PREC_FP64 ? assemble_fp64(dp, k, (double2 *)A, s)
+                                            : assemble_fp32(dp, (float)k, (float2 *)A, s);
+        }
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        device_problem_free(dp);
+        if (rc != EXIT_SUCCESS) break;
+        const double avg = ms / 3.0;
+        if (c.qg == def.qg && c.waves == def.waves) dms = avg;
+        if (avg < bms) {
+            bms = avg;
+            best = c;
+        }
+    }
+    (void)hipFreeAsync(A, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HOST_CALL(rc);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[{half_octave(M), half_octave(E)}] = best;
+    }
+    if (best_qg) *best_qg = best.qg;
+    if (best_waves) *best_waves = best.waves;
+    if (best_ms) *best_ms = bms;
+    if (default_ms) *default_ms = dms;
+    return EXIT_SUCCESS;
+}
+
+// The tuned shapes as text lines "M-bucket E-bucket qg waves" (and back).
+int tune_save(const char *file) {
+    FILE *f = fopen(file, "w");
+    if (!f) BEM_FAIL("cannot write %s", file);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto &kv : g_tune) fprintf(f, "%d %d %d %d\n", kv.first.first, kv.first.second, kv.second.qg, kv.second.waves);
+    fclose(f);
+    return EXIT_SUCCESS;
+}
+
+int tune_load(const char *file) {
+    FILE *f = fopen(file, "r");
+    if (!f) BEM_FAIL("cannot read %s", file);
+    int a, b, q, w;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    while (fscanf(f, "%d %d %d %d", &a, &b, &q, &w) == 4)
+        if ((q == 1 || q == 4) && w > 0) g_tune[{a, b}] = TileShape{q, w};
+    fclose(f);
+    return EXIT_SUCCESS;
+}
+
+void tune_clear() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
 }
 
 // Upper bound of the HBM bytes device_problem_build allocates for a mesh of
@@ -846,7 +970,7 @@ static int choose_qg(int M, int E) {
 size_t device_problem_bytes(int M, int N, int E) {
     const size_t Mp = ((size_t)M + 63) / 64 * 64;
     const int qg = choose_qg(M, E);
-    const size_t P = (size_t)choose_tiles(M, N, 64 / qg);
+    const size_t P = (size_t)choose_tiles(M, N, E, 64 / qg);
     size_t b = 0;
     b += sizeof(float4) * M + sizeof(double) * 3 * M + sizeof(double2) * M;        // points, bterm
     b += (sizeof(int) + sizeof(float2)) * 3 * (size_t)E;                            // elements, bc
@@ -884,7 +1008,7 @@ int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
         }
     TilePlan plan;
     const int qg = choose_qg(dp.M, dp.E);
-    build_tile_plan(m, choose_tiles(dp.M, dp.N, 64 / qg), plan);
+    build_tile_plan(m, choose_tiles(dp.M, dp.N, dp.E, 64 / qg), plan);
     dp.P = plan.P;
     HOST_CALL(salloc(&dp.pts, dp.M, s));
     HOST_CALL(salloc(&dp.pts64, 3 * (size_t)dp.M, s));

@@ -156,6 +156,12 @@ struct DeviceProblem {
 int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s);
 void device_problem_free(DeviceProblem &dp);
 size_t device_problem_bytes(int M, int N, int E);
+// assembly launch-shape tuning (assembly.hip)
+int autotune_assembly(const Mesh &m, double k, int precision, int *best_qg, int *best_waves, double *best_ms,
+                      double *default_ms, hipStream_t s);
+int tune_save(const char *file);
+int tune_load(const char *file);
+void tune_clear();
 
 struct RhsSpec {
     int kind = BEM_RHS_POINT;

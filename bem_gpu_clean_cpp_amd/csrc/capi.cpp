@@ -781,6 +781,32 @@ int bem_solver_stats(const bem_solver *sv, bem_solve_stats *st) {
     return EXIT_SUCCESS;
 }
 
+// ---------------------------------------------------------------- tuning ---
+int bem_autotune_assembly(const bem_mesh *m, float k, int precision, bem_tune_result *out) {
+    if (!m) BEM_FAIL("null mesh");
+    if (precision != BEM_PREC_FP32 && precision != BEM_PREC_FP64) BEM_FAIL("precision must be 32 or 64");
+    std::lock_guard<std::mutex> lk(g_mu);
+    hipStream_t s;
+    HOST_CALL(stream(&s));
+    bem_tune_result r{};
+    HOST_CALL(autotune_assembly(m->m, (double)k, precision, &r.lanes_per_row, &r.waves, &r.best_ms, &r.default_ms,
+                                s));
+    if (out) *out = r;
+    return EXIT_SUCCESS;
+}
+int bem_tune_save(const char *file) {
+    if (!file) BEM_FAIL("null file");
+    return tune_save(file);
+}
+int bem_tune_load(const char *file) {
+    if (!file) BEM_FAIL("null file");
+    return tune_load(file);
+}
+int bem_tune_clear(void) {
+    tune_clear();
+    return EXIT_SUCCESS;
+}
+
 // ------------------------------------------------------------------ timing ---
 int bem_timing_enable(int on) { return timing_enable(on); }
 int bem_timing_reset(void) { return timing_reset(); }

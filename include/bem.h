@@ -284,6 +284,25 @@ void bem_server_destroy(bem_server *s);
  * its solve allocates). */
 double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int method, int restart);
 
+/* ------------------------------------------------ launch tuning (extension) --- */
+/* The assembly's launch shape: lanes per collocation row (1 or 4) and the
+ * number of column tiles, given as the total wave count of the launch.  The
+ * defaults suit most sizes; bem_autotune_assembly times candidate shapes for
+ * this mesh at wavenumber k on the device and keeps the fastest for meshes of
+ * its size class (rows and elements in half-octave buckets) in this process;
+ * solvers and device problems created afterwards use it.  bem_tune_save /
+ * bem_tune_load keep the table in a text file. */
+typedef struct bem_tune_result {
+    int lanes_per_row;
+    int waves;
+    double best_ms;      /* assembly time with the chosen shape */
+    double default_ms;   /* assembly time with the default shape */
+} bem_tune_result;
+int bem_autotune_assembly(const bem_mesh *m, float k, int precision, bem_tune_result *out);
+int bem_tune_save(const char *file);
+int bem_tune_load(const char *file);
+int bem_tune_clear(void);
+
 /* ----------------------------------------------------- timing (extension) --- */
 /* HIP-event timing of the library's kernels on the stream they run on.
  * bem_timing_enable(1) starts recording; bem_timing_get returns the total

@@ -289,3 +289,21 @@ def test_rigid_sphere_series_cfg1():
     err = np.linalg.norm(x - ser) / np.linalg.norm(ser)
     print("rigid sphere series error %.4g" % err)
     assert err < 0.005
+
+
+def test_autotune_assembly_keeps_parity():
+    """The launch-shape autotuner picks a shape for the mesh's size class; the
+    assembly with it still matches the oracle."""
+    m = _icosphere(3, nchief=8)
+    k = 20.0
+    try:
+        r = bem.autotune_assembly(m, k)
+        assert r["lanes_per_row"] in (1, 4) and r["waves"] > 0
+        assert 0 < r["best_ms"] <= r["default_ms"] * 1.0001 or r["default_ms"] < 0
+        s = bem.Solver(m)
+        s.assemble(k, bem.PREC_FP32)
+        A, _ = s.system()
+        Ao, _ = _oracle_system(m, k)
+        assert _rel_max(A, Ao) < 2e-5
+    finally:
+        bem.tune_clear()
