@@ -338,7 +338,13 @@ def bench_serving(bem, C, n, rate, slots, seed=104, method=None):
     th.join()
     wall = time.perf_counter() - t0
     st = srv.status()
+    # the same requests all at once: the server's capacity (saturated queue)
+    t0 = time.perf_counter()
+    bids = [srv.submit(r["mesh"], r["k"], r["kind"], r["params"], **kw) for r in reqs]
+    bres = [srv.wait(i) for i in bids]
+    bwall = time.perf_counter() - t0
     srv.close()
+    blat = np.array([res[1]["end_ms"] - res[1]["submit_ms"] for res in bres])
     lat = np.array([res[1]["end_ms"] - res[1]["submit_ms"] for res in results])
     queue = np.array([res[1]["start_ms"] - res[1]["submit_ms"] for res in results])
     conv = all(res[1]["solve"]["converged"] == 1 for res in results)
@@ -366,6 +372,10 @@ def bench_serving(bem, C, n, rate, slots, seed=104, method=None):
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
                        "max": round(float(lat.max()), 2)},
         "queue_ms": {"p50": round(float(np.percentile(queue, 50)), 2), "p99": round(float(np.percentile(queue, 99)), 2)},
+        "burst": {"wall_s": round(bwall, 3), "solves_per_s": round(cols / bwall, 2),
+                  "requests_per_s": round(n / bwall, 3),
+                  "latency_ms": {"p50": round(float(np.percentile(blat, 50)), 2),
+                                 "p99": round(float(np.percentile(blat, 99)), 2)}},
         "converged": conv, "subdiv_counts": {str(k): v for k, v in sorted(sizes.items())},
         "max_running": st["max_running"], "peak_hbm_gb": round(st["peak_in_use_bytes"] / 1e9, 2),
         "budget_gb": round(st["budget_bytes"] / 1e9, 1),
@@ -456,6 +466,21 @@ def main():
     gemv_gbs = gemv_bytes / (gn_ms / max(gn_n, 1) * 1e-3) / 1e9 if gn_n else None
     gemvh_gbs = gemv_bytes / (gh_ms / max(gh_n, 1) * 1e-3) / 1e9 if gh_n else None
 
+    # the same solves run two at a time (two resident solvers on their own
+    # streams): one solve's assembly overlaps the other's solve
+    conc = None
+    if args.steps >= 4:
+        ks = np.full(args.steps, k, np.float32)
+        bem.batch_solve(mesh, ks[:2], bem.RHS_PLANE, [[0, 0, 1]], workers=2, restart=30, tol=1e-6,
+                        refine_tol=1e-8, max_refine=50)
+        bem.sync()
+        tc = time.perf_counter()
+        bem.batch_solve(mesh, ks, bem.RHS_PLANE, [[0, 0, 1]], workers=2, restart=30, tol=1e-6, refine_tol=1e-8,
+                        max_refine=50)
+        bem.sync()
+        conc = round(args.steps / (time.perf_counter() - tc), 3)
+        log("two concurrent solvers: %.3f solves/s" % conc)
+
     # accuracy: plain all-FP64 GPU solve of the same discretisation (also timed:
     # the 1x yardstick), and the rigid-sphere series
     err_fp64 = None
@@ -518,6 +543,7 @@ def main():
                                 "GMRES(30,1e-6) on normal equations" if method == bem.SOLVE_GMRES else "QR"),
                    "N": N, "E": E, "M": M, "rhs": R},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "solves_per_s_two_concurrent_solvers": conc,
         "assembly_gbs": round(asm_gbs, 2),
         "assembly_ms": round(asm_total_ms, 3),
         "solve": {"iterations": st["iterations"], "refinements": st["refinements"], "converged": st["converged"],
