@@ -142,3 +142,38 @@ def test_wrt_cplx_mat_golden_format():
                                      re.findall(r"\(([-\d.]+),([-\d.]+)\)", txt)]), v[0])
     finally:
         os.unlink(name)
+
+
+def test_python_binding_covers_every_declared_symbol():
+    assert set(_header_functions()) <= set(bem.EXPORTED)
+
+
+def test_request_footprint_bounds_the_allocations():
+    """The HBM footprint used for admission covers A and grows with the
+    right-hand sides and the Krylov basis; QR keeps a factored copy of A."""
+    M, N, E = 10258, 10242, 20480
+    ld = (M + 63) // 64 * 64
+    f1 = bem.request_footprint(M, N, E, 1)
+    f32 = bem.request_footprint(M, N, E, 32)
+    assert f1 >= 8.0 * ld * N
+    assert f32 > f1 + 16.0 * ld * 31 * 31  # 31 more FP64 bases of 31 vectors
+    assert bem.request_footprint(M, N, E, 1, restart=100) > f1
+    assert bem.request_footprint(M, N, E, 1, method=bem.SOLVE_QR) >= 16.0 * ld * N
+    assert bem.request_footprint(M, N, E, 1, precision=bem.PREC_FP64) >= 16.0 * ld * N
+    assert bem.request_footprint(N - 1, N, E, 1) < 0  # underdetermined: refused
+
+
+def test_tune_table_roundtrip(tmp_path):
+    src = tmp_path / "tune.txt"
+    src.write_text("26 29 1 16384\n22 25 4 8192\nbad line\n")
+    bem.tune_clear()
+    try:
+        bem.tune_load(src)
+        out = tmp_path / "out.txt"
+        bem.tune_save(out)
+        rows = sorted(tuple(map(int, l.split())) for l in out.read_text().splitlines())
+        assert rows == [(22, 25, 4, 8192), (26, 29, 1, 16384)]
+        with pytest.raises(bem.BemError):
+            bem.tune_load(tmp_path / "missing.txt")
+    finally:
+        bem.tune_clear()
