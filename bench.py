@@ -80,16 +80,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel, config):
+def measured_traffic(kernel, workload):
     """HBM bytes per launch measured with FETCH_SIZE / WRITE_SIZE counters (scaled
-    by a copy kernel of known size) and recorded under profiles/; None if absent."""
-    path = os.path.join(ROOT, "profiles", "%s_s5_hbm_traffic.txt" % config)
-    if not os.path.exists(path):
+    by a copy kernel of known size) and recorded under profiles/ as lines
+    'traffic_bytes_per_launch <kernel> <workload> <bytes>'; None if absent."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
         return None
-    for line in open(path):
-        f = line.split()
-        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == config:
-            return float(f[3])
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith("_hbm_traffic.txt"):
+            continue
+        for line in open(os.path.join(pdir, name)):
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel and f[2] == workload:
+                return float(f[3])
     return None
 
 
@@ -247,6 +251,8 @@ def bench_large(bem, C, fp64=True):
         "roofline_gemm": {"bound": "mfma", "kernel": "gemm_n_f32", "avg_ms": round(g_avg, 3), "launches": g_n,
                           "achieved": round(flops / (g_avg * 1e-3) / 1e12, 2), "peak": PEAK_FP32_TFLOPS,
                           "unit": "TFLOP/s", "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                          "traffic": measured_traffic("gemm_n_f32", "large"),
+                          "adjoint_traffic": measured_traffic("gemm_h_f32", "large"),
                           "adjoint_avg_ms": round(h_avg, 3),
                           "adjoint_achieved": round(flops / (h_avg * 1e-3) / 1e12, 2),
                           "hbm_gbs": round(8.0 * M * N / (g_avg * 1e-3) / 1e9, 1), "flops_per_launch": flops},
