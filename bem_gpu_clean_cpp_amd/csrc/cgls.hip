@@ -17,7 +17,8 @@
 // right-hand side); the recurrence vectors and scalars are FP64 and stay on
 // the device (alpha and beta are formed by the update kernels from device
 // scalars), so an iteration needs no host round trip; the host reads the
-// per-system activity flags every few steps.
+// per-system activity flags every 8 steps, and those 8 steps are one captured
+// HIP graph, replayed for every chunk of the solve.
 //
 // Accuracy comes from FP64 residual refinement, as for GMRES: each cycle
 // computes r = b - A x and s = A^H r with FP64 accumulation, runs CG on the
@@ -138,6 +139,8 @@ __global__ void kc_act_init(int R, const double *__restrict__ g0, const int *__r
 
 namespace {
 
+constexpr int kChunk = 8;  // CG steps between reads of the activity flags
+
 struct CglsWS {
     long ld = 0;
     hipStream_t st = nullptr;
@@ -241,6 +244,37 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         return EXIT_SUCCESS;
     };
 
+    const double tol2 = tol * tol;
+    // n CG steps (n even, so every chunk starts on gamma slot 0)
+    auto chunk = [&](int n) -> int {
+        for (int it = 0; it < n; it++) {
+            const int cur = it & 1;
+            double *gc = w.gam + (size_t)cur * R, *gn = w.gam + (size_t)(1 - cur) * R;
+            HOST_CALL(prod_n());
+            if (f32) kc_norm2<float2><<<R, 1024, 0, s>>>(w.q32, ld, M, w.del, nullptr, nullptr, 0.0);
+            else kc_norm2<double2><<<R, 1024, 0, s>>>(w.q64, ld, M, w.del, nullptr, nullptr, 0.0);
+            HIP_CALL(hipGetLastError());
+            HOST_CALL(prod_h());
+            if (f32) kc_upd_xs<float2><<<gN, bs, 0, s>>>(N, gc, w.del, w.act, w.p, w.d, w.w32, w.s64, ld);
+            else kc_upd_xs<double2><<<gN, bs, 0, s>>>(N, gc, w.del, w.act, w.p, w.d, w.w64, w.s64, ld);
+            kc_norm2<double2><<<R, 1024, 0, s>>>(w.s64, ld, N, gn, w.g0, w.act, tol2);
+            kc_upd_p<double2><<<gN, bs, 0, s>>>(N, gn, gc, w.act, w.s64, w.p, f32 ? w.p32 : nullptr, ld);
+            HIP_CALL(hipGetLastError());
+        }
+        return EXIT_SUCCESS;
+    };
+    // Chunks of 8 steps run as one captured HIP graph (the host reads the
+    // activity flags between chunks).  Not while kernel timing is on: the
+    // timing records events around each launch.
+    const bool use_graph = !timing_on() && !getenv("BEM_NO_GRAPH");
+    hipGraphExec_t exec = nullptr;
+    struct ExecGuard {
+        hipGraphExec_t *e;
+        ~ExecGuard() {
+            if (*e) (void)hipGraphExecDestroy(*e);
+        }
+    } exec_guard{&exec};
+
     std::vector<double> nb(R), beta(R);
     std::vector<int> done(R, 0);
     HIP_CALL(hipMemset2DAsync(X, sizeof(double2) * ldx, 0, sizeof(double2) * N, R, s));
@@ -252,7 +286,6 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     }
     bool res_current = true;
     int total = 0, cycles = 0;
-    const double tol2 = tol * tol;
     std::vector<int> hact(R);
     for (int cyc = 0; cyc < max_cycles; cyc++) {
         if (!res_current) {
@@ -275,28 +308,33 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         kc_act_init<<<(R + 63) / 64, 64, 0, s>>>(R, w.g0, w.done, w.act);
         kc_start<<<gN, bs, 0, s>>>(N, w.s64, w.p, f32 ? w.p32 : nullptr, w.d, ld);
         HIP_CALL(hipGetLastError());
-        int cur = 0;
-        for (int it = 0; total < maxit; it++) {
-            total++;
-            double *gc = w.gam + (size_t)cur * R, *gn = w.gam + (size_t)(1 - cur) * R;
-            HOST_CALL(prod_n());
-            if (f32) kc_norm2<float2><<<R, 1024, 0, s>>>(w.q32, ld, M, w.del, nullptr, nullptr, 0.0);
-            else kc_norm2<double2><<<R, 1024, 0, s>>>(w.q64, ld, M, w.del, nullptr, nullptr, 0.0);
-            HIP_CALL(hipGetLastError());
-            HOST_CALL(prod_h());
-            if (f32) kc_upd_xs<float2><<<gN, bs, 0, s>>>(N, gc, w.del, w.act, w.p, w.d, w.w32, w.s64, ld);
-            else kc_upd_xs<double2><<<gN, bs, 0, s>>>(N, gc, w.del, w.act, w.p, w.d, w.w64, w.s64, ld);
-            kc_norm2<double2><<<R, 1024, 0, s>>>(w.s64, ld, N, gn, w.g0, w.act, tol2);
-            kc_upd_p<double2><<<gN, bs, 0, s>>>(N, gn, gc, w.act, w.s64, w.p, f32 ? w.p32 : nullptr, ld);
-            HIP_CALL(hipGetLastError());
-            cur = 1 - cur;
-            if ((it & 7) == 7) {
-                HIP_CALL(hipMemcpyAsync(hact.data(), w.act, sizeof(int) * R, hipMemcpyDeviceToHost, s));
-                HIP_CALL(hipStreamSynchronize(s));
-                bool live = false;
-                for (int q = 0; q < R; q++) live = live || hact[q];
-                if (!live) break;
+        for (;;) {
+            const int n = std::min(kChunk, maxit - total);
+            if (n <= 0) break;
+            if (n == kChunk && use_graph) {
+                if (!exec) {
+                    // the chunk is the same every time (pointers fixed for the
+                    // solve, scalars in device memory): capture it once
+                    hipGraph_t graph = nullptr;
+                    HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    const int rc = chunk(kChunk);
+                    hipError_t e = hipStreamEndCapture(s, &graph);
+                    HOST_CALL(rc);
+                    HIP_CALL(e);
+                    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    HIP_CALL(e);
+                }
+                HIP_CALL(hipGraphLaunch(exec, s));
+            } else {
+                HOST_CALL(chunk(n));
             }
+            total += n;
+            HIP_CALL(hipMemcpyAsync(hact.data(), w.act, sizeof(int) * R, hipMemcpyDeviceToHost, s));
+            HIP_CALL(hipStreamSynchronize(s));
+            bool live = false;
+            for (int q = 0; q < R; q++) live = live || hact[q];
+            if (!live) break;
         }
         kc_add<<<gN, bs, 0, s>>>(N, w.d, ld, X, ldx);
         HIP_CALL(hipGetLastError());

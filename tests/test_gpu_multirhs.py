@@ -164,3 +164,27 @@ def test_cgls_matches_oracle(R, prec):
     print("R %d prec %d stats %s worst err %.3g" % (R, prec, st, err.max()))
     assert st["converged"] == 1
     assert err.max() < (1e-5 if prec == bem.PREC_FP32 else 1e-8)
+
+
+def test_cgls_graph_replays_equal_eager(monkeypatch):
+    """The CG steps run as a captured HIP graph replayed every 8 steps, with
+    the host's flag reads and the FP64 residuals between replays.  Every
+    solve (three in a row, each replaying the graph many times) returns the
+    same bits as the same solve launched eagerly, and the oracle's solution."""
+    m = _mesh(3, 8)
+    k = 25.0
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    for seed in (1, 2, 3):
+        d = _dirs(12, seed=seed)
+        s.set_rhs(bem.RHS_PLANE, d, 1.0)
+        monkeypatch.delenv("BEM_NO_GRAPH", raising=False)
+        st = s.solve(bem.SOLVE_CGLS, tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=20)
+        xg = s.solution()
+        monkeypatch.setenv("BEM_NO_GRAPH", "1")
+        s.solve(bem.SOLVE_CGLS, tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=20)
+        xe = s.solution()
+        assert st["converged"] == 1 and st["iterations"] >= 16
+        assert np.array_equal(xg, xe), seed
+        xo = _oracle_lstsq(m, k, d)
+        assert _relcols(xg, xo).max() < 1e-5
