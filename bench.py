@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-large", action="store_true")
-    ap.add_argument("--only", choices=["batching", "large", "serving"],
+    ap.add_argument("--only", choices=["plain", "batching", "large", "serving"],
                     help="run one secondary workload alone and print its section (not the benchmark line)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -204,6 +204,54 @@ def bench_batching(bem, C, workers, fp64=True):
                           "unit": "TFLOP/s", "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                           "hbm_gbs": round(gbytes / (g_avg * 1e-3) / 1e9, 1), "flops_per_launch": flops,
                           "bytes_per_launch": gbytes},
+    }
+
+
+def bench_plain(bem, C, reps=20):
+    """Plain workload (the reference's own small case): rigid sphere r = 0.1 m
+    as icosphere s3 (642 nodes), 171 Hz, 8 CHIEF points, unit plane wave along
+    +z; FP32 assembly + QR least squares with FP64 refinement + the field at 64
+    seeded points on r = 1 m through the field operator.  Errors: surface vs
+    the rigid-sphere series, scattered field vs the exterior series."""
+    c, a = 343.0, 0.1
+    k = 2 * np.pi * 171.0 / c
+    m = bem.Mesh().icosphere(3, a)
+    m.set_bc(0, 1, 0)
+    m.set_chief(C.ball_points(101, 8, a / 2))
+    xs = C.sphere_points(101, 64, 1.0)
+    s = bem.Solver(m)
+    s.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+
+    def one():
+        s.assemble(k, bem.PREC_FP32)
+        st = s.solve(bem.SOLVE_QR, refine_tol=1e-10, max_refine=10)
+        s.field_matrix(xs)
+        return st, s.field_apply()[:, 0]
+
+    one()
+    bem.sync()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ts = time.perf_counter()
+        st, p = one()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    wall = time.perf_counter() - t0
+    x = s.solution()[:, 0]
+    pn = m.pnts().astype(np.float64)
+    ser = C.rigid_sphere_surface(a, k, pn[:, 2] / np.linalg.norm(pn, axis=1))
+    inc = np.exp(-1j * k * xs[:, 2].astype(np.float64))
+    fser = C.rigid_sphere_field(a, k, xs)
+    s.close()
+    return {
+        "workload": "plain: rigid sphere icosphere s3 (N=642), 171 Hz, 8 CHIEF, plane wave; FP32 assembly + QR with "
+                    "FP64 refinement + field at 64 points on r=1 m (field operator)",
+        "value": round(reps / wall, 2), "unit": "solves/s", "latency_ms": {
+            "p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "solve": {kk: st[kk] for kk in ("refinements", "converged", "residual")},
+        "rel_err_surface_vs_series": float(np.linalg.norm(x - ser) / np.linalg.norm(ser)),
+        "rel_err_scattered_field_vs_series": float(np.linalg.norm((p - inc) - (fser - inc)) /
+                                                   np.linalg.norm(fser - inc)),
     }
 
 
@@ -405,7 +453,9 @@ def main():
         # one secondary workload alone (for profiling); prints that section's JSON
         bem.quad_init(6)
         bem.set_medium(1.21, 343.0)
-        if args.only == "batching":
+        if args.only == "plain":
+            res = bench_plain(bem, C)
+        elif args.only == "batching":
             res = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
         elif args.only == "large":
             res = bench_large(bem, C, fp64=not args.no_fp64)
@@ -516,6 +566,8 @@ def main():
         cpu = cpu_baseline(mesh, k, rho, c, args.cpu_seconds, threads)
         log("cpu baseline", cpu)
 
+    plain = bench_plain(bem, C)
+    log("plain", plain)
     batching = None
     if not args.no_batching:
         batching = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
@@ -568,6 +620,7 @@ def main():
                           "traffic": measured_traffic("gemv_n_f32", args.config),
                           "launches": gn_n},
         "cpu_baseline": cpu and {kk: cpu[kk] for kk in ("value", "unit", "cores", "kind", "sample")},
+        "plain": plain,
         "batching": batching,
         "large": large,
         "serving": serving,

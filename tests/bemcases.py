@@ -105,3 +105,21 @@ def large_case(bem):
     m.set_chief(ball_points(1051, 32, 0.05))
     src = sources_outside(m, 1052, 32, 0.01)
     return m, src
+
+
+def rigid_sphere_field(a, k, x):
+    """Total pressure at exterior points x (P, 3) around a rigid sphere of
+    radius a for p_inc = exp(-i k z) (e^{+i omega t}):
+    p = sum_n (2n+1) (-i)^n [j_n(kr) - j_n'(ka) / h_n^(2)'(ka) h_n^(2)(kr)] P_n(cos t)."""
+    x = np.asarray(x, np.float64)
+    r = np.linalg.norm(x, axis=1)
+    ct = x[:, 2] / r
+    ka = k * a
+    nmax = int(max(ka, k * r.max()) + 4 * max(ka, 1.0) ** (1 / 3) + 20)
+    p = np.zeros(r.shape, complex)
+    for n in range(nmax + 1):
+        dh = special.spherical_jn(n, ka, derivative=True) - 1j * special.spherical_yn(n, ka, derivative=True)
+        dj = special.spherical_jn(n, ka, derivative=True)
+        h = special.spherical_jn(n, k * r) - 1j * special.spherical_yn(n, k * r)
+        p += (2 * n + 1) * (-1j) ** n * (special.spherical_jn(n, k * r) - dj / dh * h) * special.eval_legendre(n, ct)
+    return p

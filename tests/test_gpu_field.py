@@ -93,3 +93,28 @@ def test_field_operator_stale_wavenumber_refused():
     s.solve()
     with pytest.raises(bem.BemError):
         s.field_apply()
+
+
+def test_plain_config_field_matches_sphere_series():
+    """The small plain workload: rigid sphere r = 0.1 m (icosphere s3), 171 Hz,
+    8 CHIEF points, unit plane wave along +z, QR least squares; the field at 64
+    seeded points on r = 1 m against the exterior series.  The scattered part
+    (total minus incident) carries the comparison: <= 2 % of its size."""
+    a, k = 0.1, float(np.float32(2 * np.pi * 171.0 / SPEED))
+    m = bem.Mesh().icosphere(3, a)
+    m.set_bc(0, 1, 0)
+    m.set_chief(C.ball_points(101, 8, a / 2))
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+    st = s.solve(bem.SOLVE_QR, refine_tol=1e-10, max_refine=10)
+    assert st["converged"] == 1
+    xs = C.sphere_points(101, 64, 1.0)
+    s.field_matrix(xs)
+    p = s.field_apply()[:, 0]
+    inc = np.exp(-1j * k * xs[:, 2].astype(np.float64))
+    ser = C.rigid_sphere_field(a, k, xs)
+    scat, scat_ser = p - inc, ser - inc
+    err = np.linalg.norm(scat - scat_ser) / np.linalg.norm(scat_ser)
+    print("scattered-field error vs series %.3g" % err)
+    assert err < 0.02

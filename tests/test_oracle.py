@@ -233,3 +233,16 @@ def test_ray_inside():
     p, e, _ = _sphere(2)
     assert O.ray_inside([0.01, 0.02, -0.01], [0.3, 0.5, 0.81], p, e) == 1
     assert O.ray_inside([0.2, 0.0, 0.0], [0.3, 0.5, 0.81], p, e) == 0
+
+
+def test_rigid_sphere_field_series_reduces_to_surface_and_incident():
+    """The exterior series equals the surface series at r = a and tends to the
+    incident wave plus an outgoing scattered part: at ka -> 0 the scattered
+    field vanishes (Rayleigh), so p -> exp(-i k z) far from a tiny sphere."""
+    a, k = 0.1, 20.0
+    pts = C.sphere_points(3, 12, 1.0).astype(np.float64)
+    surf = C.rigid_sphere_field(a, k, pts * a)
+    ref = C.rigid_sphere_surface(a, k, pts[:, 2])
+    assert np.max(np.abs(surf - ref)) < 1e-6 * np.max(np.abs(ref))  # the two truncations differ
+    tiny = C.rigid_sphere_field(1e-4, 1.0, pts * 2.0)
+    assert np.max(np.abs(tiny - np.exp(-1j * 1.0 * 2.0 * pts[:, 2]))) < 1e-6
