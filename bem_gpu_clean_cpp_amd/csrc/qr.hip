@@ -37,6 +37,7 @@ struct QT<double> {
 };
 
 constexpr int kNB = 32;  // panel width
+constexpr int kCG = 8;   // panel columns updated together
 
 template <typename T>
 __device__ __forceinline__ T wsum(T v) {
@@ -63,12 +64,14 @@ __device__ __forceinline__ C2 vget(const C2 *A, long lda, int j0, int l, int i) 
 
 template <typename T>
 __global__ __launch_bounds__(1024) void qr_panel(typename QT<T>::c2 *A, long lda, int M, int j0, int nb,
-                                                 typename QT<T>::c2 *Tout /* nb x nb, ld kNB */) {
+                                                 typename QT<T>::c2 *Tout /* nb x nb, ld kNB */, int make_t) {
     using C2 = typename QT<T>::c2;
     __shared__ T red[2][16];
     __shared__ T s_tau[kNB];
     __shared__ C2 s_Y[kNB][kNB];  // v_l^H v_m for l < m
     __shared__ C2 s_T[kNB][kNB];
+    __shared__ T s_red[16][2 * kCG];
+    __shared__ T s_d[2 * kCG];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
     for (int jj = 0; jj < nb; jj++) {
         const int j = j0 + jj;
@@ -114,35 +117,78 @@ __global__ __launch_bounds__(1024) void qr_panel(typename QT<T>::c2 *A, long lda
             s_tau[jj] = tau;
         }
         __syncthreads();
-        // apply H^H = H (tau real) to the remaining panel columns: one wave per column
-        for (int c = j + 1 + wid; c < j0 + nb; c += nw) {
-            C2 *cc = A + (size_t)c * lda;
-            T dr = 0, di = 0;
-            for (int i = j + 1 + lane; i < M; i += 64) {
-                const C2 v = col[i], a = cc[i];
-                dr += v.x * a.x + v.y * a.y;
-                di += v.x * a.y - v.y * a.x;
+        // apply H^H = H (tau real) to the remaining panel columns, 8 at a time:
+        // every thread takes rows i > j for all 8 columns, one block reduction
+        // of the 8 dot products, then the rank-1 update of those rows
+        for (int c0 = j + 1; c0 < j0 + nb; c0 += kCG) {
+            const int ncg = min(kCG, j0 + nb - c0);
+            T dr[kCG], di[kCG];
+#pragma unroll
+            for (int c = 0; c < kCG; c++) {
+                dr[c] = 0;
+                di[c] = 0;
             }
-            dr = wsum(dr);
-            di = wsum(di);
-            const C2 aj = cc[j];
-            dr = (dr + aj.x) * tau;
-            di = (di + aj.y) * tau;
-            for (int i = j + 1 + lane; i < M; i += 64) {
+            for (int i = j + 1 + tid; i < M; i += blockDim.x) {
                 const C2 v = col[i];
-                C2 a = cc[i];
-                a.x -= dr * v.x - di * v.y;
-                a.y -= dr * v.y + di * v.x;
-                cc[i] = a;
+#pragma unroll
+                for (int c = 0; c < kCG; c++) {
+                    if (c < ncg) {
+                        const C2 a = A[(size_t)(c0 + c) * lda + i];
+                        dr[c] += v.x * a.x + v.y * a.y;
+                        di[c] += v.x * a.y - v.y * a.x;
+                    }
+                }
             }
-            if (lane == 0) {
-                C2 a = aj;
-                a.x -= dr;
-                a.y -= di;
-                cc[j] = a;
+#pragma unroll
+            for (int c = 0; c < kCG; c++) {
+                const T r = wsum(dr[c]), im = wsum(di[c]);
+                if (lane == 0) {
+                    s_red[wid][2 * c] = r;
+                    s_red[wid][2 * c + 1] = im;
+                }
             }
+            __syncthreads();
+            if (tid < 2 * ncg) {
+                T t = 0;
+                for (int q = 0; q < nw; q++) t += s_red[q][tid];
+                const int c = tid >> 1;
+                const C2 aj = A[(size_t)(c0 + c) * lda + j];  // row j: v_j = 1
+                s_d[tid] = (t + ((tid & 1) ? aj.y : aj.x)) * tau;
+            }
+            __syncthreads();
+            for (int i = j + 1 + tid; i < M; i += blockDim.x) {
+                const C2 v = col[i];
+#pragma unroll
+                for (int c = 0; c < kCG; c++) {
+                    if (c < ncg) {
+                        const T er = s_d[2 * c], ei = s_d[2 * c + 1];
+                        C2 *pa = A + (size_t)(c0 + c) * lda + i;
+                        C2 a = *pa;
+                        a.x -= er * v.x - ei * v.y;
+                        a.y -= er * v.y + ei * v.x;
+                        *pa = a;
+                    }
+                }
+            }
+            if (tid < ncg) {
+                C2 *pa = A + (size_t)(c0 + tid) * lda + j;
+                C2 a = *pa;
+                a.x -= s_d[2 * tid];
+                a.y -= s_d[2 * tid + 1];
+                *pa = a;
+            }
+            __syncthreads();
         }
-        __syncthreads();
+    }
+    if (!make_t) {  // T is formed by qr_make_t from an MFMA V^H V: pass the taus on its diagonal
+        for (int p = tid; p < kNB * kNB; p += blockDim.x) {
+            const int l = p / kNB, m = p % kNB;
+            C2 z;
+            z.x = (l == m && l < nb) ? s_tau[l] : (T)0;
+            z.y = 0;
+            Tout[p] = z;
+        }
+        return;
     }
     // Y[l][m] = v_l^H v_m for l < m (dot over rows >= j0+m)
     for (int p = wid; p < nb * nb; p += nw) {
@@ -174,6 +220,38 @@ __global__ __launch_bounds__(1024) void qr_panel(typename QT<T>::c2 *A, long lda
             T zr = 0, zi = 0;
             for (int q = tid; q < m; q++) {
                 const C2 t = s_T[tid][q], y = s_Y[q][m];
+                zr += t.x * y.x - t.y * y.y;
+                zi += t.x * y.y + t.y * y.x;
+            }
+            s_T[tid][m].x = -s_tau[m] * zr;
+            s_T[tid][m].y = -s_tau[m] * zi;
+        } else if (tid == m) {
+            s_T[m][m].x = s_tau[m];
+        }
+        __syncthreads();
+    }
+    for (int p = tid; p < kNB * kNB; p += blockDim.x) Tout[p] = s_T[p / kNB][p % kNB];
+}
+
+// T of the compact WY form from the taus (on Tout's diagonal) and
+// Y[m * ldy + l] = (V^H V)[l][m]: LAPACK larft, forward columnwise, as in qr_panel
+template <typename T>
+__global__ void qr_make_t(typename QT<T>::c2 *Tout, int nb, const typename QT<T>::c2 *Y, int ldy) {
+    using C2 = typename QT<T>::c2;
+    __shared__ C2 s_T[kNB][kNB];
+    __shared__ T s_tau[kNB];
+    const int tid = threadIdx.x;
+    if (tid < kNB) s_tau[tid] = Tout[tid * kNB + tid].x;
+    for (int p = tid; p < kNB * kNB; p += blockDim.x) {
+        s_T[p / kNB][p % kNB].x = 0;
+        s_T[p / kNB][p % kNB].y = 0;
+    }
+    __syncthreads();
+    for (int m = 0; m < nb; m++) {
+        if (tid < m) {
+            T zr = 0, zi = 0;
+            for (int q = tid; q < m; q++) {
+                const C2 t = s_T[tid][q], y = Y[(size_t)m * ldy + q];
                 zr += t.x * y.x - t.y * y.y;
                 zi += t.x * y.y + t.y * y.x;
             }
@@ -356,6 +434,7 @@ struct QRFactor {
     int wcols = 0;
     C2 *Vd = nullptr;  // explicit panel V (FP32 MFMA trailing update)
     C2 *Z = nullptr;   // (C^H V) of the trailing columns
+    C2 *Yz = nullptr;  // V^H V of the panel (kNB x kNB)
     char *gws = nullptr;
     long ldv = 0;
     hipStream_t st = nullptr;
@@ -364,6 +443,7 @@ struct QRFactor {
         sfree(W, st);
         sfree(Vd, st);
         sfree(Z, st);
+        sfree(Yz, st);
         sfree(gws, st);
     }
 };
@@ -400,9 +480,11 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
         f.ldv = (f.lda + 63) / 64 * 64;
         HOST_CALL(salloc(&f.Vd, (size_t)f.ldv * kNB, s));
         HOST_CALL(salloc(&f.Z, (size_t)kNB * f.N, s));
+        HOST_CALL(salloc(&f.Yz, (size_t)kNB * kNB, s));
         size_t wsz = 0;
         for (int p = 0; p < np; p++) {
             const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
+            wsz = std::max(wsz, ws_gemm_h_f32(f.M - j0, kNB, kNB));
             if (f.N - (j0 + nb) > 0) wsz = std::max(wsz, ws_gemm_h_f32(f.M - j0, f.N - (j0 + nb), kNB));
         }
         HOST_CALL(salloc(&f.gws, std::max<size_t>(wsz, 1), s));
@@ -411,8 +493,20 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
         const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
         TimedRegion tr;
         timing_begin(tr, sizeof(T) == 4 ? "qr_panel_f32" : "qr_panel_f64", s);
-        qr_panel<T><<<1, 1024, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Tb + (size_t)p * kNB * kNB);
+        C2 *Tp = f.Tb + (size_t)p * kNB * kNB;
+        const int Mp = f.M - j0;
+        qr_panel<T><<<1, 1024, 0, s>>>(f.A, f.lda, f.M, j0, nb, Tp, mfma ? 0 : 1);
         HIP_CALL(hipGetLastError());
+        if (mfma) {
+            // explicit V, then V^H V on MFMA and T from it
+            dim3 gv((unsigned)((Mp + 255) / 256), kNB);
+            k_build_v<T><<<gv, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Vd, f.ldv);
+            HIP_CALL(hipGetLastError());
+            HOST_CALL(gemm_h_f32((const float2 *)f.Vd, f.ldv, Mp, kNB, (const float2 *)f.Vd, f.ldv, (float2 *)f.Yz,
+                                 kNB, kNB, f.gws, s));
+            qr_make_t<T><<<1, 64, 0, s>>>(Tp, nb, f.Yz, kNB);
+            HIP_CALL(hipGetLastError());
+        }
         timing_end(tr, s);
         const int nc = f.N - (j0 + nb);
         if (nc > 0) {
@@ -420,11 +514,7 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
             TimedRegion tu;
             timing_begin(tu, sizeof(T) == 4 ? "qr_trailing_f32" : "qr_trailing_f64", s);
             if (mfma) {
-                const int Mp = f.M - j0;
                 float2 *Cb = (float2 *)(Ct + j0);  // rows j0.., columns j0+nb..
-                dim3 gv((unsigned)((Mp + 255) / 256), kNB);
-                k_build_v<T><<<gv, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Vd, f.ldv);
-                HIP_CALL(hipGetLastError());
                 HOST_CALL(gemm_h_f32(Cb, f.lda, Mp, nc, (const float2 *)f.Vd, f.ldv, (float2 *)f.Z, f.N, kNB, f.gws,
                                      s));
                 dim3 gw((unsigned)((nc + 255) / 256), kNB);
