@@ -233,6 +233,135 @@ __global__ __launch_bounds__(1024) void qr_panel(typename QT<T>::c2 *A, long lda
     for (int p = tid; p < kNB * kNB; p += blockDim.x) Tout[p] = s_T[p / kNB][p % kNB];
 }
 
+// Panel of exactly kNB columns and at most NT * RPT rows (rows j0..M-1) with
+// the panel in registers: thread t holds rows j0 + t + NT q (q < RPT) of all
+// kNB columns, so a column step costs three barriers and no memory traffic.
+// The column loop is unrolled so every register index is static.  Per column:
+// the norm below the diagonal (wave sums -> s_nrm), then the products v^H a_c
+// of the later columns (wave sums -> s_red, summed over the waves by 2 kNB
+// threads -> s_fin), then the update.  Each shared array is written only after
+// a barrier that follows its last read, so the three barriers suffice.  Writes
+// the Householder vectors and R back in place and the taus on Tout's diagonal
+// (T itself comes from qr_make_t).
+template <int NT, int RPT>
+__global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, int j0, float2 *Tout) {
+    constexpr int NW = NT / 64;
+    __shared__ float s_nrm[NW];
+    __shared__ float s_red[NW][2 * kNB];
+    __shared__ float2 s_fin[kNB];
+    __shared__ float s_bc[2];
+    __shared__ float s_tau[kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    float2 a[RPT][kNB];
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int r = j0 + tid + NT * q;
+#pragma unroll
+        for (int c = 0; c < kNB; c++) a[q][c] = r < M ? A[(size_t)(j0 + c) * lda + r] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int jj = 0; jj < kNB; jj++) {
+        const int j = j0 + jj;
+        // |x|^2 over rows > j, and alpha = A[j, j] to everyone
+        float sq = 0.f;
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int r = j0 + tid + NT * q;
+            if (r > j && r < M) sq += a[q][jj].x * a[q][jj].x + a[q][jj].y * a[q][jj].y;
+            if (r == j) {
+                s_bc[0] = a[q][jj].x;
+                s_bc[1] = a[q][jj].y;
+            }
+        }
+        sq = wsum(sq);
+        if (lane == 0) s_nrm[wid] = sq;
+        __syncthreads();
+        float tot = 0.f;
+#pragma unroll
+        for (int w = 0; w < NW; w++) tot += s_nrm[w];
+        const float2 alpha = make_float2(s_bc[0], s_bc[1]);
+        const float aa = sqrtf(alpha.x * alpha.x + alpha.y * alpha.y);
+        const float xnorm = sqrtf(aa * aa + tot);
+        float tau = 0.f;
+        float2 scale = make_float2(0.f, 0.f), beta = alpha;
+        if (tot > 0 || (xnorm > 0 && alpha.y != 0)) {
+            const float phr = aa > 0 ? alpha.x / aa : 1.f, phi = aa > 0 ? alpha.y / aa : 0.f;
+            beta = make_float2(-phr * xnorm, -phi * xnorm);
+            const float dr = alpha.x - beta.x, di = alpha.y - beta.y;
+            const float dd = dr * dr + di * di;
+            scale = make_float2(dr / dd, -di / dd);
+            tau = 2.f / (1.f + tot / dd);
+        }
+        if (tid == 0) s_tau[jj] = tau;
+        // v = x / (alpha - beta) below the diagonal (v_j = 1, zero above);
+        // R[j, j] = beta
+        float2 v[RPT];
+#pragma unroll
+        for (int q = 0; q < RPT; q++) {
+            const int r = j0 + tid + NT * q;
+            if (r > j && r < M) {
+                const float2 x = a[q][jj];
+                a[q][jj] = make_float2(x.x * scale.x - x.y * scale.y, x.x * scale.y + x.y * scale.x);
+                v[q] = a[q][jj];
+            } else {
+                if (r == j) a[q][jj] = beta;
+                v[q] = make_float2(r == j ? 1.f : 0.f, 0.f);
+            }
+        }
+        if (jj + 1 < kNB) {
+            // d_c = tau (v^H a_c) for the later columns
+#pragma unroll
+            for (int c = jj + 1; c < kNB; c++) {
+                float pr = 0.f, pi = 0.f;
+#pragma unroll
+                for (int q = 0; q < RPT; q++) {
+                    pr += v[q].x * a[q][c].x + v[q].y * a[q][c].y;
+                    pi += v[q].x * a[q][c].y - v[q].y * a[q][c].x;
+                }
+                pr = wsum(pr);
+                pi = wsum(pi);
+                if (lane == 0) {
+                    s_red[wid][2 * c] = pr;
+                    s_red[wid][2 * c + 1] = pi;
+                }
+                // one column's sums at a time: the live registers stay near the panel's
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            __syncthreads();
+            if (tid < 2 * kNB) {
+                float t = 0.f;
+#pragma unroll
+                for (int w = 0; w < NW; w++) t += s_red[w][tid];
+                ((float *)s_fin)[tid] = t * tau;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int c = jj + 1; c < kNB; c++) {
+                const float2 e = s_fin[c];
+#pragma unroll
+                for (int q = 0; q < RPT; q++) {
+                    a[q][c].x -= e.x * v[q].x - e.y * v[q].y;
+                    a[q][c].y -= e.x * v[q].y + e.y * v[q].x;
+                }
+                if ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+        const int r = j0 + tid + NT * q;
+        if (r < M) {
+#pragma unroll
+            for (int c = 0; c < kNB; c++) A[(size_t)(j0 + c) * lda + r] = a[q][c];
+        }
+    }
+    __syncthreads();
+    for (int p = tid; p < kNB * kNB; p += NT) {
+        const int l = p / kNB, m = p % kNB;
+        Tout[p] = make_float2(l == m ? s_tau[l] : 0.f, 0.f);
+    }
+}
+
 // T of the compact WY form from the taus (on Tout's diagonal) and
 // Y[m * ldy + l] = (V^H V)[l][m]: LAPACK larft, forward columnwise, as in qr_panel
 template <typename T>
@@ -495,7 +624,15 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
         timing_begin(tr, sizeof(T) == 4 ? "qr_panel_f32" : "qr_panel_f64", s);
         C2 *Tp = f.Tb + (size_t)p * kNB * kNB;
         const int Mp = f.M - j0;
-        qr_panel<T><<<1, 1024, 0, s>>>(f.A, f.lda, f.M, j0, nb, Tp, mfma ? 0 : 1);
+        if (mfma && nb == kNB && Mp <= 256 && !getenv("BEM_QR_GPANEL")) {
+            qr_panel_reg<256, 1><<<1, 256, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, (float2 *)Tp);
+        } else if (mfma && nb == kNB && Mp <= 512 && !getenv("BEM_QR_GPANEL")) {
+            qr_panel_reg<512, 1><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, (float2 *)Tp);
+        } else if (mfma && nb == kNB && Mp <= 1024 && !getenv("BEM_QR_GPANEL")) {
+            qr_panel_reg<512, 2><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, (float2 *)Tp);
+        } else {
+            qr_panel<T><<<1, 1024, 0, s>>>(f.A, f.lda, f.M, j0, nb, Tp, mfma ? 0 : 1);
+        }
         HIP_CALL(hipGetLastError());
         if (mfma) {
             // explicit V, then V^H V on MFMA and T from it

@@ -158,6 +158,26 @@ def test_qr_solve_matches_lapack():
     assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-5
 
 
+def test_qr_panel_variants_match_lapack(monkeypatch):
+    """1100 x 1040: the first panels (over 1024 rows) run the global-memory
+    panel kernel, then the register panels with 2 rows per thread (<= 1024
+    rows), 1 row per thread (<= 512, <= 256), and the last 16 columns the
+    global kernel again.  Against LAPACK, and against the global panel kernel
+    for every panel (BEM_QR_GPANEL)."""
+    rng = np.random.default_rng(7)
+    m, n = 1100, 1040
+    A = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+    A += 30 * np.eye(m, n, dtype=np.complex64)
+    B = (rng.standard_normal((m, 3)) + 1j * rng.standard_normal((m, 3))).astype(np.complex64)
+    X = bem.qr_solve(A, B)
+    Xo = np.linalg.lstsq(A.astype(np.complex128), B.astype(np.complex128), rcond=None)[0]
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 2e-5
+    monkeypatch.setenv("BEM_QR_GPANEL", "1")
+    Xg = bem.qr_solve(A, B)
+    assert np.linalg.norm(Xg - Xo) / np.linalg.norm(Xo) < 2e-5
+    assert np.linalg.norm(X - Xg) / np.linalg.norm(Xo) < 2e-5
+
+
 def test_lsq_solve_returns_r_and_q():
     rng = np.random.default_rng(2)
     m, n = 90, 70
