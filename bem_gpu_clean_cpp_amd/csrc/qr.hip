@@ -368,19 +368,22 @@ template <typename T>
 __global__ void qr_make_t(typename QT<T>::c2 *Tout, int nb, const typename QT<T>::c2 *Y, int ldy) {
     using C2 = typename QT<T>::c2;
     __shared__ C2 s_T[kNB][kNB];
+    __shared__ C2 s_Y[kNB][kNB + 1];
     __shared__ T s_tau[kNB];
     const int tid = threadIdx.x;
     if (tid < kNB) s_tau[tid] = Tout[tid * kNB + tid].x;
     for (int p = tid; p < kNB * kNB; p += blockDim.x) {
         s_T[p / kNB][p % kNB].x = 0;
         s_T[p / kNB][p % kNB].y = 0;
+        // the serial column loop below reads Y from shared memory only
+        s_Y[p / kNB][p % kNB] = Y[(size_t)(p / kNB) * ldy + p % kNB];
     }
     __syncthreads();
     for (int m = 0; m < nb; m++) {
         if (tid < m) {
             T zr = 0, zi = 0;
             for (int q = tid; q < m; q++) {
-                const C2 t = s_T[tid][q], y = Y[(size_t)m * ldy + q];
+                const C2 t = s_T[tid][q], y = s_Y[m][q];
                 zr += t.x * y.x - t.y * y.y;
                 zi += t.x * y.y + t.y * y.x;
             }
@@ -551,6 +554,161 @@ __global__ __launch_bounds__(1024) void trsv_upper(const typename QT<T>::c2 *Rm,
     }
 }
 
+// Sum of 2 kNB = 64 per-lane values over the wave, scattered: on return lane
+// l holds the wave total of slot l in x[0].  Each halving step sends one half
+// of the live slots to the partner lane and keeps the other half (the half
+// picked by the lane's bit H), so 32 + 16 + ... + 1 = 63 shuffles replace the
+// 64 x 6 of one butterfly sum per slot.
+template <int H, typename T>
+__device__ __forceinline__ void reduce_scatter_step(T (&x)[2 * kNB], int lane) {
+    const bool up = (lane & H) != 0;
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+        const T send = up ? x[i] : x[i + H];
+        const T keep = up ? x[i + H] : x[i];
+        x[i] = keep + __shfl_xor(send, H, 64);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void wave_reduce_scatter64(T (&x)[2 * kNB], int lane) {
+    reduce_scatter_step<32>(x, lane);
+    reduce_scatter_step<16>(x, lane);
+    reduce_scatter_step<8>(x, lane);
+    reduce_scatter_step<4>(x, lane);
+    reduce_scatter_step<2>(x, lane);
+    reduce_scatter_step<1>(x, lane);
+}
+
+// Whole least-squares solve of a small factored system in one workgroup per
+// right-hand side, b held in shared memory (M entries, dynamic): Q^H b panel
+// by panel (z = V^H b as one scattered reduction of the kNB products, w = T^H z,
+// b -= V w), then R x = b[0:N] by blocks of kNB columns from the bottom: one
+// wave solves the diagonal block with the block's rows of R in registers, then
+// the whole workgroup removes the block's unknowns from the rows above.  The
+// same arithmetic as qr_vhc + qr_update + trsv_upper without their 3 np + N
+// launches and barrier steps.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void qr_solve_small(const typename QT<T>::c2 *A, long lda, int M, int N,
+                                                       const typename QT<T>::c2 *Tb, typename QT<T>::c2 *B,
+                                                       long ldb) {
+    using C2 = typename QT<T>::c2;
+    extern __shared__ unsigned char qss_raw[];
+    C2 *sb = (C2 *)qss_raw;
+    __shared__ T s_red[16][2 * kNB];
+    __shared__ C2 s_z[kNB], s_w[kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+    C2 *b = B + (size_t)blockIdx.x * ldb;
+    for (int i = tid; i < M; i += blockDim.x) sb[i] = b[i];
+    __syncthreads();
+    const int np = (N + kNB - 1) / kNB;
+    for (int p = 0; p < np; p++) {
+        const int j0 = p * kNB, nb = min(kNB, N - j0);
+        T x[2 * kNB];
+#pragma unroll
+        for (int l = 0; l < 2 * kNB; l++) x[l] = 0;
+        for (int i = j0 + tid; i < M; i += blockDim.x) {
+            const C2 bi = sb[i];
+#pragma unroll
+            for (int l = 0; l < kNB; l++) {
+                if (l < nb) {
+                    const C2 v = vget(A, lda, j0, l, i);
+                    x[2 * l] += v.x * bi.x + v.y * bi.y;
+                    x[2 * l + 1] += v.x * bi.y - v.y * bi.x;
+                }
+            }
+        }
+        wave_reduce_scatter64(x, lane);
+        s_red[wid][lane] = x[0];
+        __syncthreads();
+        if (tid < 2 * kNB) {
+            T t = 0;
+            for (int w = 0; w < nw; w++) t += s_red[w][tid];
+            ((T *)s_z)[tid] = t;
+        }
+        __syncthreads();
+        if (tid < nb) {
+            // w = T^H z; T upper triangular
+            const C2 *Tm = Tb + (size_t)p * kNB * kNB;
+            T wr = 0, wi = 0;
+            for (int q = 0; q <= tid; q++) {
+                const C2 t = Tm[q * kNB + tid];
+                wr += t.x * s_z[q].x + t.y * s_z[q].y;
+                wi += t.x * s_z[q].y - t.y * s_z[q].x;
+            }
+            s_w[tid].x = wr;
+            s_w[tid].y = wi;
+        }
+        __syncthreads();
+        for (int i = j0 + tid; i < M; i += blockDim.x) {
+            T sr = 0, si = 0;
+            for (int l = 0; l < nb; l++) {
+                const C2 v = vget(A, lda, j0, l, i);
+                sr += v.x * s_w[l].x - v.y * s_w[l].y;
+                si += v.x * s_w[l].y + v.y * s_w[l].x;
+            }
+            sb[i].x -= sr;
+            sb[i].y -= si;
+        }
+        __syncthreads();
+    }
+    for (int p = np - 1; p >= 0; p--) {
+        const int c0 = p * kNB, nb = min(kNB, N - c0);
+        if (wid == 0) {
+            // lane l owns row c0 + l of the diagonal block
+            const int r = c0 + lane;
+            C2 rr[kNB];
+#pragma unroll
+            for (int m = 0; m < kNB; m++) {
+                if (lane < nb && m < nb && m >= lane) {
+                    rr[m] = A[(size_t)(c0 + m) * lda + r];
+                } else {
+                    rr[m].x = 0;
+                    rr[m].y = 0;
+                }
+            }
+            C2 bl;
+            if (lane < nb) {
+                bl = sb[r];
+            } else {
+                bl.x = 0;
+                bl.y = 0;
+            }
+#pragma unroll
+            for (int m = kNB - 1; m >= 0; m--) {
+                if (m < nb) {
+                    // lane m divides by the diagonal; everyone takes x_m from it
+                    C2 xm;
+                    const T d = rr[m].x * rr[m].x + rr[m].y * rr[m].y;
+                    xm.x = lane == m ? (bl.x * rr[m].x + bl.y * rr[m].y) / d : 0;
+                    xm.y = lane == m ? (bl.y * rr[m].x - bl.x * rr[m].y) / d : 0;
+                    xm.x = __shfl(xm.x, m, 64);
+                    xm.y = __shfl(xm.y, m, 64);
+                    if (lane < m) {
+                        bl.x -= rr[m].x * xm.x - rr[m].y * xm.y;
+                        bl.y -= rr[m].x * xm.y + rr[m].y * xm.x;
+                    } else if (lane == m) {
+                        bl = xm;
+                    }
+                }
+            }
+            if (lane < nb) sb[r] = bl;
+        }
+        __syncthreads();
+        for (int i = tid; i < c0; i += blockDim.x) {
+            T sr = 0, si = 0;
+            for (int m = 0; m < nb; m++) {
+                const C2 rv = A[(size_t)(c0 + m) * lda + i], xv = sb[c0 + m];
+                sr += rv.x * xv.x - rv.y * xv.y;
+                si += rv.x * xv.y + rv.y * xv.x;
+            }
+            sb[i].x -= sr;
+            sb[i].y -= si;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < N; i += blockDim.x) b[i] = sb[i];
+}
+
 // ------------------------------------------------------------------ host ---
 template <typename T>
 struct QRFactor {
@@ -674,9 +832,23 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
 }
 
 // Least squares solve with a factored A: B (M x R, ldb) <- solution in B[0:N].
+constexpr size_t kSmallSolveLds = 48 * 1024;  // b of up to 6144 FP32 / 3072 FP64 rows
+constexpr int kSmallSolveRhs = 64;
+
 template <typename T>
 static int qr_solve_factored(const QRFactor<T> &f, typename QT<T>::c2 *B, long ldb, int R, hipStream_t s) {
+    using C2 = typename QT<T>::c2;
     if (R <= 0) return EXIT_SUCCESS;
+    // small systems with few right-hand sides: the whole solve in one
+    // workgroup per right-hand side, b in shared memory
+    const size_t lds = sizeof(C2) * (size_t)f.M;
+    if (lds <= kSmallSolveLds && R <= kSmallSolveRhs && !getenv("BEM_QR_NO_FUSED_SOLVE")) {
+        // 64 FP64 partial sums per thread need the register room of 512 threads
+        constexpr int nt = sizeof(T) == 4 ? 1024 : 512;
+        qr_solve_small<T, nt><<<R, nt, lds, s>>>(f.A, f.lda, f.M, f.N, f.Tb, B, ldb);
+        HIP_CALL(hipGetLastError());
+        return EXIT_SUCCESS;
+    }
     HOST_CALL(qr_apply<T>(f, B, ldb, R, 1, s));
     trsv_upper<T><<<R, 1024, 0, s>>>(f.A, f.lda, f.N, B, ldb);
     HIP_CALL(hipGetLastError());
