@@ -233,25 +233,105 @@ __global__ __launch_bounds__(1024) void qr_panel(typename QT<T>::c2 *A, long lda
     for (int p = tid; p < kNB * kNB; p += blockDim.x) Tout[p] = s_T[p / kNB][p % kNB];
 }
 
+// Sum of 2 kNB = 64 per-lane values over the wave, scattered: on return lane
+// l holds the wave total of slot l in x[0].  Each halving step sends one half
+// of the live slots to the partner lane and keeps the other half (the half
+// picked by the lane's bit H), so 32 + 16 + ... + 1 = 63 shuffles replace the
+// 64 x 6 of one butterfly sum per slot.
+template <int H, typename T>
+__device__ __forceinline__ void reduce_scatter_step(T (&x)[2 * kNB], int lane) {
+    const bool up = (lane & H) != 0;
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+        const T send = up ? x[i] : x[i + H];
+        const T keep = up ? x[i + H] : x[i];
+        x[i] = keep + __shfl_xor(send, H, 64);
+    }
+}
+template <typename T>
+__device__ __forceinline__ void wave_reduce_scatter64(T (&x)[2 * kNB], int lane) {
+    reduce_scatter_step<32>(x, lane);
+    reduce_scatter_step<16>(x, lane);
+    reduce_scatter_step<8>(x, lane);
+    reduce_scatter_step<4>(x, lane);
+    reduce_scatter_step<2>(x, lane);
+    reduce_scatter_step<1>(x, lane);
+}
+
+// FP32: the two wide steps as lane-half swaps (v_permlane32_swap exchanges
+// lanes 32-63 of its first operand with lanes 0-31 of its second,
+// v_permlane16_swap the odd 16-lane rows of the first with the even rows of
+// the second), so the lane whose bit H is clear ends with its own and its
+// partner's x[i] and the other lane with both x[i + H]: one add, no select.
+// The narrow steps read the partner through DPP: row_ror:8 pairs l with l ^ 8
+// inside a row, row_half_mirror pairs l with 7 - l inside 8 lanes (opposite
+// bit 2, same higher bits, which is all the scatter needs), quad_perm
+// [2,3,0,1] and [1,0,3,2] pair across bits 1 and 0.
+template <int H>
+__device__ __forceinline__ void reduce_scatter_swap(float (&x)[2 * kNB]) {
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+        const unsigned a = __float_as_uint(x[i]), b = __float_as_uint(x[i + H]);
+        if constexpr (H == 32) {
+            const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+            x[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        } else {
+            const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+            x[i] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+        }
+    }
+}
+template <int H, int CTRL>
+__device__ __forceinline__ void reduce_scatter_dpp(float (&x)[2 * kNB], int lane) {
+    const bool up = (lane & H) != 0;
+#pragma unroll
+    for (int i = 0; i < H; i++) {
+        const float send = up ? x[i] : x[i + H];
+        const float keep = up ? x[i + H] : x[i];
+        x[i] = keep + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(send), CTRL, 0xF, 0xF, false));
+    }
+}
+template <>
+__device__ __forceinline__ void wave_reduce_scatter64<float>(float (&x)[2 * kNB], int lane) {
+    reduce_scatter_swap<32>(x);
+    reduce_scatter_swap<16>(x);
+    reduce_scatter_dpp<8, 0x128>(x, lane);  // row_ror:8
+    reduce_scatter_dpp<4, 0x141>(x, lane);  // row_half_mirror
+    reduce_scatter_dpp<2, 0x4E>(x, lane);   // quad_perm [2,3,0,1]
+    reduce_scatter_dpp<1, 0xB1>(x, lane);   // quad_perm [1,0,3,2]
+}
+
 // Panel of exactly kNB columns and at most NT * RPT rows (rows j0..M-1) with
 // the panel in registers: thread t holds rows j0 + t + NT q (q < RPT) of all
 // kNB columns, so a column step costs three barriers and no memory traffic.
-// The column loop is unrolled so every register index is static.  Per column:
-// the norm below the diagonal (wave sums -> s_nrm), then the products v^H a_c
-// of the later columns (wave sums -> s_red, summed over the waves by 2 kNB
-// threads -> s_fin), then the update.  Each shared array is written only after
-// a barrier that follows its last read, so the three barriers suffice.  Writes
-// the Householder vectors and R back in place and the taus on Tout's diagonal
-// (T itself comes from qr_make_t).
+// The column loop is unrolled so every register index is static.
+//
+// Column step jj (j = j0 + jj): the norm below the diagonal (wave sums ->
+// s_nrm) gives the reflector; then one scattered wave reduction forms
+// sum_r conj(v(r)) a_c(r) for every other column c (-> s_red, summed over the
+// waves by 2 kNB threads -> s_fin).  For c > jj that is v^H a_c, which with
+// tau updates a_c.  For c < jj the column holds v_c, exact on every row where
+// v is nonzero (r >= j > j0 + c), so the sum is conj(v_c^H v); those give
+// column jj of T by the forward recurrence (LAPACK larft; Schreiber & Van
+// Loan 1989)
+//   T[l][jj] = -tau_jj sum_{q=l}^{jj-1} T[l][q] (v_q^H v_jj),  T[jj][jj] = tau_jj
+// with H_1 ... H_kNB = I - V T V^H.  Each shared array is written only after a
+// barrier that follows its last read, so three barriers per column suffice.
+//
+// Writes the Householder vectors and R back in place, T to Tout (kNB x kNB,
+// row-major, zero below the diagonal) and the explicit V (unit diagonal, zero
+// above it) to Vd[l * ldv + (i - j0)] for rows i = j0..M-1.
 template <int NT, int RPT>
-__global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, int j0, float2 *Tout) {
+__global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, int j0, float2 *Tout, float2 *Vd,
+                                                   long ldv) {
     constexpr int NW = NT / 64;
     __shared__ float s_nrm[NW];
     __shared__ float s_red[NW][2 * kNB];
     __shared__ float2 s_fin[kNB];
+    __shared__ float2 s_T[kNB][kNB];
     __shared__ float s_bc[2];
-    __shared__ float s_tau[kNB];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int p = tid; p < kNB * kNB; p += NT) s_T[p / kNB][p % kNB] = make_float2(0.f, 0.f);
     float2 a[RPT][kNB];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
@@ -259,7 +339,7 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
 #pragma unroll
         for (int c = 0; c < kNB; c++) a[q][c] = r < M ? A[(size_t)(j0 + c) * lda + r] : make_float2(0.f, 0.f);
     }
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int jj = 0; jj < kNB; jj++) {
         const int j = j0 + jj;
         // |x|^2 over rows > j, and alpha = A[j, j] to everyone
@@ -292,7 +372,6 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
             scale = make_float2(dr / dd, -di / dd);
             tau = 2.f / (1.f + tot / dd);
         }
-        if (tid == 0) s_tau[jj] = tau;
         // v = x / (alpha - beta) below the diagonal (v_j = 1, zero above);
         // R[j, j] = beta
         float2 v[RPT];
@@ -308,31 +387,31 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
                 v[q] = make_float2(r == j ? 1.f : 0.f, 0.f);
             }
         }
-        if (jj + 1 < kNB) {
-            // d_c = tau (v^H a_c) for the later columns
+        {
+            // slots 2c, 2c + 1: sum_r conj(v(r)) a_c(r) for c != jj (slot pair
+            // jj zero), one scattered wave reduction for all of them
+            float x[2 * kNB];
 #pragma unroll
-            for (int c = jj + 1; c < kNB; c++) {
+            for (int c = 0; c < kNB; c++) {
                 float pr = 0.f, pi = 0.f;
+                if (c != jj) {
 #pragma unroll
-                for (int q = 0; q < RPT; q++) {
-                    pr += v[q].x * a[q][c].x + v[q].y * a[q][c].y;
-                    pi += v[q].x * a[q][c].y - v[q].y * a[q][c].x;
+                    for (int q = 0; q < RPT; q++) {
+                        pr += v[q].x * a[q][c].x + v[q].y * a[q][c].y;
+                        pi += v[q].x * a[q][c].y - v[q].y * a[q][c].x;
+                    }
                 }
-                pr = wsum(pr);
-                pi = wsum(pi);
-                if (lane == 0) {
-                    s_red[wid][2 * c] = pr;
-                    s_red[wid][2 * c + 1] = pi;
-                }
-                // one column's sums at a time: the live registers stay near the panel's
-                __builtin_amdgcn_sched_barrier(0);
+                x[2 * c] = pr;
+                x[2 * c + 1] = pi;
             }
+            wave_reduce_scatter64(x, lane);
+            s_red[wid][lane] = x[0];
             __syncthreads();
             if (tid < 2 * kNB) {
                 float t = 0.f;
 #pragma unroll
                 for (int w = 0; w < NW; w++) t += s_red[w][tid];
-                ((float *)s_fin)[tid] = t * tau;
+                ((float *)s_fin)[tid] = (tid >> 1) > jj ? t * tau : t;
             }
             __syncthreads();
 #pragma unroll
@@ -345,6 +424,18 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
                 }
                 if ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
             }
+            // column jj of T (v_q^H v = conj(s_fin[q]))
+            if (tid < jj) {
+                float tr = 0.f, ti = 0.f;
+                for (int q = tid; q < jj; q++) {
+                    const float2 t = s_T[tid][q], y = make_float2(s_fin[q].x, -s_fin[q].y);
+                    tr += t.x * y.x - t.y * y.y;
+                    ti += t.x * y.y + t.y * y.x;
+                }
+                s_T[tid][jj] = make_float2(-tau * tr, -tau * ti);
+            } else if (tid == jj) {
+                s_T[jj][jj] = make_float2(tau, 0.f);
+            }
         }
     }
 #pragma unroll
@@ -352,14 +443,15 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
         const int r = j0 + tid + NT * q;
         if (r < M) {
 #pragma unroll
-            for (int c = 0; c < kNB; c++) A[(size_t)(j0 + c) * lda + r] = a[q][c];
+            for (int c = 0; c < kNB; c++) {
+                A[(size_t)(j0 + c) * lda + r] = a[q][c];
+                const int d = j0 + c;
+                Vd[(size_t)c * ldv + (r - j0)] = r > d ? a[q][c] : make_float2(r == d ? 1.f : 0.f, 0.f);
+            }
         }
     }
     __syncthreads();
-    for (int p = tid; p < kNB * kNB; p += NT) {
-        const int l = p / kNB, m = p % kNB;
-        Tout[p] = make_float2(l == m ? s_tau[l] : 0.f, 0.f);
-    }
+    for (int p = tid; p < kNB * kNB; p += NT) Tout[p] = s_T[p / kNB][p % kNB];
 }
 
 // T of the compact WY form from the taus (on Tout's diagonal) and
@@ -552,31 +644,6 @@ __global__ __launch_bounds__(1024) void trsv_upper(const typename QT<T>::c2 *Rm,
         }
         __syncthreads();
     }
-}
-
-// Sum of 2 kNB = 64 per-lane values over the wave, scattered: on return lane
-// l holds the wave total of slot l in x[0].  Each halving step sends one half
-// of the live slots to the partner lane and keeps the other half (the half
-// picked by the lane's bit H), so 32 + 16 + ... + 1 = 63 shuffles replace the
-// 64 x 6 of one butterfly sum per slot.
-template <int H, typename T>
-__device__ __forceinline__ void reduce_scatter_step(T (&x)[2 * kNB], int lane) {
-    const bool up = (lane & H) != 0;
-#pragma unroll
-    for (int i = 0; i < H; i++) {
-        const T send = up ? x[i] : x[i + H];
-        const T keep = up ? x[i + H] : x[i];
-        x[i] = keep + __shfl_xor(send, H, 64);
-    }
-}
-template <typename T>
-__device__ __forceinline__ void wave_reduce_scatter64(T (&x)[2 * kNB], int lane) {
-    reduce_scatter_step<32>(x, lane);
-    reduce_scatter_step<16>(x, lane);
-    reduce_scatter_step<8>(x, lane);
-    reduce_scatter_step<4>(x, lane);
-    reduce_scatter_step<2>(x, lane);
-    reduce_scatter_step<1>(x, lane);
 }
 
 // Whole least-squares solve of a small factored system in one workgroup per
@@ -782,18 +849,23 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
         timing_begin(tr, sizeof(T) == 4 ? "qr_panel_f32" : "qr_panel_f64", s);
         C2 *Tp = f.Tb + (size_t)p * kNB * kNB;
         const int Mp = f.M - j0;
-        if (mfma && nb == kNB && Mp <= 256 && !getenv("BEM_QR_GPANEL")) {
-            qr_panel_reg<256, 1><<<1, 256, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, (float2 *)Tp);
-        } else if (mfma && nb == kNB && Mp <= 512 && !getenv("BEM_QR_GPANEL")) {
-            qr_panel_reg<512, 1><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, (float2 *)Tp);
-        } else if (mfma && nb == kNB && Mp <= 1024 && !getenv("BEM_QR_GPANEL")) {
-            qr_panel_reg<512, 2><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, (float2 *)Tp);
+        // panels of up to 1024 rows: the register kernel also forms T and the
+        // explicit V; taller ones (and a narrow last panel) run the global
+        // panel kernel, then V, V^H V on MFMA and T from it
+        const bool fit = mfma && nb == kNB && !getenv("BEM_QR_GPANEL");
+        const int reg = !fit ? 0 : Mp <= 256 ? 1 : Mp <= 512 ? 2 : Mp <= 1024 ? 3 : 0;
+        float2 *Tf = (float2 *)Tp, *Vf = (float2 *)f.Vd;
+        if (reg == 1) {
+            qr_panel_reg<256, 1><<<1, 256, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, Tf, Vf, f.ldv);
+        } else if (reg == 2) {
+            qr_panel_reg<512, 1><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, Tf, Vf, f.ldv);
+        } else if (reg == 3) {
+            qr_panel_reg<512, 2><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, Tf, Vf, f.ldv);
         } else {
             qr_panel<T><<<1, 1024, 0, s>>>(f.A, f.lda, f.M, j0, nb, Tp, mfma ? 0 : 1);
         }
         HIP_CALL(hipGetLastError());
-        if (mfma) {
-            // explicit V, then V^H V on MFMA and T from it
+        if (mfma && !reg) {
             dim3 gv((unsigned)((Mp + 255) / 256), kNB);
             k_build_v<T><<<gv, 256, 0, s>>>(f.A, f.lda, f.M, j0, nb, f.Vd, f.ldv);
             HIP_CALL(hipGetLastError());
