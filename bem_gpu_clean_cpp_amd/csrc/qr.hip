@@ -656,32 +656,43 @@ __global__ __launch_bounds__(1024) void trsv_upper(const typename QT<T>::c2 *Rm,
 // launches and barrier steps.
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void qr_solve_small(const typename QT<T>::c2 *A, long lda, int M, int N,
-                                                       const typename QT<T>::c2 *Tb, typename QT<T>::c2 *B,
-                                                       long ldb) {
+                                                     const typename QT<T>::c2 *Tb, typename QT<T>::c2 *B, long ldb) {
     using C2 = typename QT<T>::c2;
     extern __shared__ unsigned char qss_raw[];
     C2 *sb = (C2 *)qss_raw;
-    __shared__ T s_red[16][2 * kNB];
+    __shared__ T s_red[NT / 64][2 * kNB];
     __shared__ C2 s_z[kNB], s_w[kNB];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+    __shared__ C2 s_T[kNB * kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     C2 *b = B + (size_t)blockIdx.x * ldb;
-    for (int i = tid; i < M; i += blockDim.x) sb[i] = b[i];
-    __syncthreads();
+    for (int i = tid; i < M; i += NT) sb[i] = b[i];
+    // Every load below is unconditional from a clamped, valid address, with
+    // the unit diagonal, the zeros above it and columns past N selected
+    // afterwards, so the loads of a thread are all in flight together.
     const int np = (N + kNB - 1) / kNB;
     for (int p = 0; p < np; p++) {
         const int j0 = p * kNB, nb = min(kNB, N - j0);
+        for (int e = tid; e < kNB * kNB; e += NT) s_T[e] = Tb[(size_t)p * kNB * kNB + e];
+        __syncthreads();  // sb (first panel) and s_T
         T x[2 * kNB];
 #pragma unroll
         for (int l = 0; l < 2 * kNB; l++) x[l] = 0;
-        for (int i = j0 + tid; i < M; i += blockDim.x) {
+        for (int i = j0 + tid; i < M; i += NT) {
             const C2 bi = sb[i];
 #pragma unroll
-            for (int l = 0; l < kNB; l++) {
-                if (l < nb) {
-                    const C2 v = vget(A, lda, j0, l, i);
-                    x[2 * l] += v.x * bi.x + v.y * bi.y;
-                    x[2 * l + 1] += v.x * bi.y - v.y * bi.x;
+            for (int l0 = 0; l0 < kNB; l0 += kCG) {
+                C2 raw[kCG];
+#pragma unroll
+                for (int u = 0; u < kCG; u++) raw[u] = A[(size_t)min(j0 + l0 + u, N - 1) * lda + i];
+#pragma unroll
+                for (int u = 0; u < kCG; u++) {
+                    const int l = l0 + u;
+                    const bool below = i > j0 + l && l < nb, diag = i == j0 + l && l < nb;
+                    const T vx = below ? raw[u].x : (diag ? (T)1 : (T)0), vy = below ? raw[u].y : (T)0;
+                    x[2 * l] += vx * bi.x + vy * bi.y;
+                    x[2 * l + 1] += vx * bi.y - vy * bi.x;
                 }
+                __builtin_amdgcn_sched_barrier(0);  // one chunk's loads live at a time
             }
         }
         wave_reduce_scatter64(x, lane);
@@ -689,16 +700,16 @@ __global__ __launch_bounds__(NT) void qr_solve_small(const typename QT<T>::c2 *A
         __syncthreads();
         if (tid < 2 * kNB) {
             T t = 0;
-            for (int w = 0; w < nw; w++) t += s_red[w][tid];
+#pragma unroll
+            for (int w = 0; w < NT / 64; w++) t += s_red[w][tid];
             ((T *)s_z)[tid] = t;
         }
         __syncthreads();
         if (tid < nb) {
             // w = T^H z; T upper triangular
-            const C2 *Tm = Tb + (size_t)p * kNB * kNB;
             T wr = 0, wi = 0;
             for (int q = 0; q <= tid; q++) {
-                const C2 t = Tm[q * kNB + tid];
+                const C2 t = s_T[q * kNB + tid];
                 wr += t.x * s_z[q].x + t.y * s_z[q].y;
                 wi += t.x * s_z[q].y - t.y * s_z[q].x;
             }
@@ -706,37 +717,45 @@ __global__ __launch_bounds__(NT) void qr_solve_small(const typename QT<T>::c2 *A
             s_w[tid].y = wi;
         }
         __syncthreads();
-        for (int i = j0 + tid; i < M; i += blockDim.x) {
+        for (int i = j0 + tid; i < M; i += NT) {
             T sr = 0, si = 0;
-            for (int l = 0; l < nb; l++) {
-                const C2 v = vget(A, lda, j0, l, i);
-                sr += v.x * s_w[l].x - v.y * s_w[l].y;
-                si += v.x * s_w[l].y + v.y * s_w[l].x;
+#pragma unroll
+            for (int l0 = 0; l0 < kNB; l0 += kCG) {
+                C2 raw[kCG];
+#pragma unroll
+                for (int u = 0; u < kCG; u++) raw[u] = A[(size_t)min(j0 + l0 + u, N - 1) * lda + i];
+#pragma unroll
+                for (int u = 0; u < kCG; u++) {
+                    const int l = l0 + u;
+                    const bool below = i > j0 + l && l < nb, diag = i == j0 + l && l < nb;
+                    const T vx = below ? raw[u].x : (diag ? (T)1 : (T)0), vy = below ? raw[u].y : (T)0;
+                    sr += vx * s_w[l].x - vy * s_w[l].y;
+                    si += vx * s_w[l].y + vy * s_w[l].x;
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
             sb[i].x -= sr;
             sb[i].y -= si;
         }
-        __syncthreads();
+        __syncthreads();  // sb, and s_T / s_w before the next panel
     }
     for (int p = np - 1; p >= 0; p--) {
         const int c0 = p * kNB, nb = min(kNB, N - c0);
         if (wid == 0) {
             // lane l owns row c0 + l of the diagonal block
-            const int r = c0 + lane;
+            const int r = c0 + lane, rc = min(r, N - 1);
             C2 rr[kNB];
 #pragma unroll
+            for (int m = 0; m < kNB; m++) rr[m] = A[(size_t)min(c0 + m, N - 1) * lda + rc];
+#pragma unroll
             for (int m = 0; m < kNB; m++) {
-                if (lane < nb && m < nb && m >= lane) {
-                    rr[m] = A[(size_t)(c0 + m) * lda + r];
-                } else {
+                if (!(lane < nb && m < nb && m >= lane)) {
                     rr[m].x = 0;
                     rr[m].y = 0;
                 }
             }
-            C2 bl;
-            if (lane < nb) {
-                bl = sb[r];
-            } else {
+            C2 bl = sb[rc];
+            if (lane >= nb) {
                 bl.x = 0;
                 bl.y = 0;
             }
@@ -761,19 +780,28 @@ __global__ __launch_bounds__(NT) void qr_solve_small(const typename QT<T>::c2 *A
             if (lane < nb) sb[r] = bl;
         }
         __syncthreads();
-        for (int i = tid; i < c0; i += blockDim.x) {
+        for (int i = tid; i < c0; i += NT) {
             T sr = 0, si = 0;
-            for (int m = 0; m < nb; m++) {
-                const C2 rv = A[(size_t)(c0 + m) * lda + i], xv = sb[c0 + m];
-                sr += rv.x * xv.x - rv.y * xv.y;
-                si += rv.x * xv.y + rv.y * xv.x;
+#pragma unroll
+            for (int m0 = 0; m0 < kNB; m0 += kCG) {
+                C2 raw[kCG];
+#pragma unroll
+                for (int u = 0; u < kCG; u++) raw[u] = A[(size_t)min(c0 + m0 + u, N - 1) * lda + i];
+#pragma unroll
+                for (int u = 0; u < kCG; u++) {
+                    if (m0 + u < nb) {
+                        const C2 xv = sb[c0 + m0 + u];
+                        sr += raw[u].x * xv.x - raw[u].y * xv.y;
+                        si += raw[u].x * xv.y + raw[u].y * xv.x;
+                    }
+                }
             }
             sb[i].x -= sr;
             sb[i].y -= si;
         }
         __syncthreads();
     }
-    for (int i = tid; i < N; i += blockDim.x) b[i] = sb[i];
+    for (int i = tid; i < N; i += NT) b[i] = sb[i];
 }
 
 // ------------------------------------------------------------------ host ---
@@ -915,8 +943,9 @@ static int qr_solve_factored(const QRFactor<T> &f, typename QT<T>::c2 *B, long l
     // workgroup per right-hand side, b in shared memory
     const size_t lds = sizeof(C2) * (size_t)f.M;
     if (lds <= kSmallSolveLds && R <= kSmallSolveRhs && !getenv("BEM_QR_NO_FUSED_SOLVE")) {
-        // 64 FP64 partial sums per thread need the register room of 512 threads
-        constexpr int nt = sizeof(T) == 4 ? 1024 : 512;
+        // 2 kNB partial sums per thread plus a chunk of 8 loads: the register
+        // room of 512 (FP32) or 256 (FP64) threads
+        constexpr int nt = sizeof(T) == 4 ? 512 : 256;
         qr_solve_small<T, nt><<<R, nt, lds, s>>>(f.A, f.lda, f.M, f.N, f.Tb, B, ldb);
         HIP_CALL(hipGetLastError());
         return EXIT_SUCCESS;
