@@ -187,6 +187,13 @@ int field_apply_fp32(DeviceProblem &fdp, double k, const float2 *F, long ldF, co
 
 // ---------------------------------------------------------- linear algebra ---
 size_t ws_gemv_n(long M, long N, int R);
+// w = A^H (A v) in one pass over a row-major copy Ar of A (ldr = ne_ldr(N));
+// FP32 A and v, FP64 w; for N up to the limit of ne_fused_fits
+bool ne_fused_fits(long M, long N);
+long ne_ldr(long N);
+size_t ws_ne_apply(long M, long N);
+int transpose_rm_f32(const float2 *A, long lda, long M, long N, float2 *Ar, long ldr, hipStream_t s);
+int ne_apply_f32(const float2 *Ar, long ldr, long M, long N, const float2 *v, double2 *w, void *ws, hipStream_t s);
 int gemv_n_f32(const float2 *A, long lda, long M, long N, const float2 *x, long ldx, float2 *y, long ldy, int R,
                void *ws, hipStream_t s);
 int gemv_h_f32(const float2 *A, long lda, long M, long N, const float2 *x, long ldx, float2 *y, long ldy, int R,

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "bem_internal.hpp"
 
@@ -287,6 +288,251 @@ int gemv_h_f32a(const float2 *A, long lda, long M, long N, const double2 *x, lon
 int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, long ldx, double2 *y, long ldy, int R,
                hipStream_t s) {
     return gemv_h_impl<double, double, double, double>(A, lda, M, N, x, ldx, y, ldy, R, s, "gemv_h_f64");
+}
+
+// ------------------------------------------------------ normal equations ---
+// w = A^H (A v) in one pass over A, for the CHIEF normal equations with one
+// right-hand side.  (A^H A v) = sum_i conj(a_i) (a_i . v) over the rows a_i,
+// so with A stored row-major a workgroup can hold a row in registers, form
+// t_i = a_i . v with one workgroup reduction and add conj(a_i) t_i to its own
+// partial of w before the row leaves the registers.  Column-major A would need
+// all of A v before the first dot of A^H, i.e. two passes over A (the pair of
+// GEMVs above).  The row-major copy costs one transpose per matrix.
+//
+//   transpose_rm_f32  Ar[i * ldr + j] = A[j * lda + i] for j < N, zero for
+//                     N <= j < ldr (the padding is read by the pair loads);
+//   ne_apply_part     kNeBlocks workgroups of NT = 1024 (768 for rows of more
+//                     than 4096 columns: 12 waves leave 168 registers a lane)
+//                     threads, each over a contiguous range of rows.  Thread
+//                     t owns the column pairs t + NT k (16-byte loads, 1 KiB
+//                     per wave per row) and its partial of w in registers; v
+//                     sits in LDS.  Rows go in batches of RB: the batch's
+//                     loads are all in flight together, then one reduction
+//                     and one barrier (on LDS only) for all RB dots; the
+//                     reduction slots alternate by batch parity;
+//   ne_reduce         w = sum of the block partials in a fixed order, in FP64.
+constexpr int kNeBlocks = 256;
+constexpr int kNeMaxPairs = 8;  // column pairs per thread at 768 threads: N <= 12288
+
+__global__ __launch_bounds__(256) void k_transpose_rm(const float2 *__restrict__ A, long lda, int M, int N,
+                                                      float2 *__restrict__ Ar, long ldr) {
+    __shared__ float2 tile[64][65];
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int jj = ty; jj < 64; jj += 4) {
+        const int i = i0 + tx, j = j0 + jj;
+        tile[jj][tx] = (i < M && j < N) ? A[(size_t)j * lda + i] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int ii = ty; ii < 64; ii += 4) {
+        const int i = i0 + ii, j = j0 + tx;
+        if (i < M && j < ldr) Ar[(size_t)i * ldr + j] = tile[tx][ii];
+    }
+}
+
+__device__ __forceinline__ float ne_wsum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// Barrier that publishes this workgroup's LDS writes and nothing else: the
+// fences are restricted to LDS, so loads from global memory that are still in
+// flight (the next row) stay in flight across it.  (__syncthreads() waits for
+// every outstanding global load as well.)
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int NT, int PPT, int RB>
+__global__ __launch_bounds__(NT) void ne_apply_part(const float4 *__restrict__ Ar4, long ldr4, int M,
+                                                    int rows_per_blk, const float4 *__restrict__ v4, int N,
+                                                    float4 *__restrict__ part4, long ldp4) {
+    constexpr int NW = NT / 64;
+    // v is read by every row: it lives in LDS (NT * PPT pairs, zero past N),
+    // which leaves the registers to RB rows and the partial of w
+    __shared__ float4 s_v[NT * PPT];
+    __shared__ float s_red[2][NW][2 * RB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int r0 = blockIdx.x * rows_per_blk, r1 = min(M, r0 + rows_per_blk);
+    const int npairs = (N + 1) / 2;
+    for (int p = tid; p < NT * PPT; p += NT) {
+        float4 vv = p < npairs ? v4[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (2 * p + 1 >= N) {
+            vv.z = 0.f;
+            vv.w = 0.f;
+        }
+        s_v[p] = vv;
+    }
+    float4 w[PPT];
+    // byte offsets of the thread's pairs inside a row, the same for every row
+    unsigned off[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        off[k] = (unsigned)min(tid + NT * k, (int)ldr4 - 1) * 16u;
+        w[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    lds_barrier();  // s_v
+    int par = 0;
+    for (int i0 = r0; i0 < r1; i0 += RB, par ^= 1) {
+        // RB rows in flight at once; rows past the block's range are clamped
+        // to a valid row and get t = 0
+        float4 a[RB][PPT];
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            const char *row = (const char *)(Ar4 + (size_t)min(i0 + u, M - 1) * ldr4);
+#pragma unroll
+            for (int k = 0; k < PPT; k++) a[u][k] = *(const float4 *)(row + off[k]);
+        }
+        float pr[RB], pi[RB];
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            pr[u] = 0.f;
+            pi[u] = 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const float4 vr = s_v[tid + NT * k];
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                pr[u] += a[u][k].x * vr.x - a[u][k].y * vr.y + a[u][k].z * vr.z - a[u][k].w * vr.w;
+                pi[u] += a[u][k].x * vr.y + a[u][k].y * vr.x + a[u][k].z * vr.w + a[u][k].w * vr.z;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            pr[u] = ne_wsum(pr[u]);
+            pi[u] = ne_wsum(pi[u]);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int u = 0; u < RB; u++) {
+                s_red[par][wid][2 * u] = pr[u];
+                s_red[par][wid][2 * u + 1] = pi[u];
+            }
+        }
+        lds_barrier();
+#pragma unroll
+        for (int u = 0; u < RB; u++) {
+            float tr = 0.f, ti = 0.f;
+#pragma unroll
+            for (int q = 0; q < NW; q++) {
+                tr += s_red[par][q][2 * u];
+                ti += s_red[par][q][2 * u + 1];
+            }
+            if (i0 + u >= r1) {
+                tr = 0.f;
+                ti = 0.f;
+            }
+            // w += conj(a) t
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                w[k].x += a[u][k].x * tr + a[u][k].y * ti;
+                w[k].y += a[u][k].x * ti - a[u][k].y * tr;
+                w[k].z += a[u][k].z * tr + a[u][k].w * ti;
+                w[k].w += a[u][k].z * ti - a[u][k].w * tr;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int p = tid + NT * k;
+        if (p < npairs) part4[(size_t)blockIdx.x * ldp4 + p] = w[k];
+    }
+}
+
+// 64 columns per workgroup, 4 groups of the block partials per column, 8 loads
+// in flight per thread; the 4 group sums are combined in a fixed order
+__global__ __launch_bounds__(256) void ne_reduce(const float2 *__restrict__ part, long ldp, int nblk, int N,
+                                                 double2 *__restrict__ w) {
+    __shared__ double2 s_sum[4][64];
+    const int cl = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cl;
+    const int cc = min(c, N - 1);
+    double sr = 0, si = 0;
+    int b = g;
+    for (; b + 4 * 7 < nblk; b += 4 * 8) {
+        float2 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) q[u] = part[(size_t)(b + 4 * u) * ldp + cc];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            sr += q[u].x;
+            si += q[u].y;
+        }
+    }
+    for (; b < nblk; b += 4) {
+        const float2 q = part[(size_t)b * ldp + cc];
+        sr += q.x;
+        si += q.y;
+    }
+    s_sum[g][cl] = make_double2(sr, si);
+    __syncthreads();
+    if (g == 0 && c < N) {
+        const double2 a0 = s_sum[0][cl], a1 = s_sum[1][cl], a2 = s_sum[2][cl], a3 = s_sum[3][cl];
+        w[c] = make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
+    }
+}
+
+bool ne_fused_fits(long M, long N) { return M > 0 && N > 0 && (N + 1) / 2 <= 768L * kNeMaxPairs; }
+long ne_ldr(long N) { return (N + 63) / 64 * 64; }
+static void ne_shape(long M, int &rows_per_blk, int &nblk) {
+    rows_per_blk = (int)((M + kNeBlocks - 1) / kNeBlocks);
+    nblk = (int)((M + rows_per_blk - 1) / rows_per_blk);
+}
+size_t ws_ne_apply(long M, long N) {
+    int rpb, nblk;
+    ne_shape(M, rpb, nblk);
+    return sizeof(float2) * (size_t)nblk * (size_t)ne_ldr(N);
+}
+
+int transpose_rm_f32(const float2 *A, long lda, long M, long N, float2 *Ar, long ldr, hipStream_t s) {
+    if (ldr < N || ldr % 64) BEM_FAIL("row-major copy: ldr %ld must be a multiple of 64 and >= N = %ld", ldr, N);
+    dim3 g((unsigned)((M + 63) / 64), (unsigned)(ldr / 64));
+    k_transpose_rm<<<g, 256, 0, s>>>(A, lda, (int)M, (int)N, Ar, ldr);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
+int ne_apply_f32(const float2 *Ar, long ldr, long M, long N, const float2 *v, double2 *w, void *ws, hipStream_t s) {
+    if (!ne_fused_fits(M, N)) BEM_FAIL("fused normal-equation product: N = %ld is over its limit", N);
+    if (ldr != ne_ldr(N)) BEM_FAIL("fused normal-equation product: ldr %ld, expected %ld", ldr, ne_ldr(N));
+    int rpb, nblk;
+    ne_shape(M, rpb, nblk);
+    const long npairs = (N + 1) / 2, ldp = ldr;
+    TimedRegion tr, tq;
+    timing_begin(tr, "ne_apply_f32", s);
+    timing_begin(tq, "ne_apply_part", s);
+    const float4 *A4 = (const float4 *)Ar, *v4 = (const float4 *)v;
+    float4 *p4 = (float4 *)ws;
+    const long ldr4 = ldr / 2, ldp4 = ldp / 2;
+    // 1024 threads while 2 pairs per thread cover a row, then 512 threads
+    // (the register room for up to 12 pairs, with the next row in flight)
+    // two rows in flight per batch: more rows push the widest rows' registers
+    // into scratch (measured at N = 10242: 162 us with 2 rows, 184 with 3,
+    // 261 with 4)
+#define NE_LAUNCH(NT_, PPT_) \
+    ne_apply_part<NT_, PPT_, 2><<<nblk, NT_, 0, s>>>(A4, ldr4, (int)M, rpb, v4, (int)N, p4, ldp4)
+    if (npairs <= 1024) {
+        NE_LAUNCH(1024, 1);
+    } else if (npairs <= 2048) {
+        NE_LAUNCH(1024, 2);
+    } else if (npairs <= 768 * 6) {
+        NE_LAUNCH(768, 6);
+    } else if (npairs <= 768 * 7) {
+        NE_LAUNCH(768, 7);
+    } else {
+        NE_LAUNCH(768, 8);
+    }
+#undef NE_LAUNCH
+    HIP_CALL(hipGetLastError());
+    timing_end(tq, s);
+    ne_reduce<<<(unsigned)((N + 63) / 64), 256, 0, s>>>((const float2 *)ws, ldp, nblk, (int)N, w);
+    HIP_CALL(hipGetLastError());
+    timing_end(tr, s);
+    return EXIT_SUCCESS;
 }
 
 }  // namespace bem

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -160,10 +161,15 @@ struct KrylovWS {
     double2 *V = nullptr, *w64 = nullptr, *t64 = nullptr, *r = nullptr, *h = nullptr, *part = nullptr;
     float2 *v32 = nullptr, *w32 = nullptr, *t32 = nullptr;
     char *gws = nullptr;
+    // row-major copy of A for the one-pass normal-equation product (one
+    // right-hand side, FP32 A); null when the two GEMVs are used
+    float2 *Ar = nullptr;
+    long ldr = 0;
+    char *nws = nullptr;
     hipStream_t st = nullptr;
     ~KrylovWS() {
         sfree(V, st); sfree(w64, st); sfree(t64, st); sfree(r, st); sfree(h, st); sfree(part, st);
-        sfree(v32, st); sfree(w32, st); sfree(t32, st); sfree(gws, st);
+        sfree(v32, st); sfree(w32, st); sfree(t32, st); sfree(gws, st); sfree(Ar, st); sfree(nws, st);
     }
     int init(int n_, int m_, int R_, int restart_, hipStream_t s) {
         st = s;
@@ -223,6 +229,10 @@ struct Operator {
         const int bs = 256;
         const int R = ws.R;
         if (A32) {
+            if (R == 1 && normal() && ws.Ar) {
+                // A^H (A v) in one pass over the row-major copy, FP64 out
+                return ne_apply_f32(ws.Ar, ws.ldr, M, N, ws.v32, ws.w64, ws.nws, s);
+            }
             if (R == 1) {
                 if (!normal()) {
                     HOST_CALL(gemv_n_f32(A32, lda, M, N, ws.v32, 0, ws.w32, 0, 1, ws.gws, s));
@@ -324,6 +334,12 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     restart = std::max(1, std::min(restart, N));
     KrylovWS ws;
     HOST_CALL(ws.init(N, M, R, restart, s));
+    if (A32 && M > N && R == 1 && ne_fused_fits(M, N) && !getenv("BEM_NE_TWO_PASS")) {
+        ws.ldr = ne_ldr(N);
+        HOST_CALL(salloc(&ws.Ar, (size_t)M * ws.ldr, s));
+        HOST_CALL(salloc(&ws.nws, ws_ne_apply(M, N), s));
+        HOST_CALL(transpose_rm_f32(A32, lda, M, N, ws.Ar, ws.ldr, s));
+    }
     Operator op;
     op.A32 = A32;
     op.A64 = A64;

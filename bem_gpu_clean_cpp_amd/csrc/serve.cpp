@@ -85,6 +85,8 @@ double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int m
         b += (3 * sizeof(double2) + 3 * sizeof(float2)) * ldv * R;  // work vectors
         b += std::max(std::max(ws_gemv_n(M, N, 1), ws_gemm(M, N, nrhs)), ws_mres(M, N, nrhs));
         b += sizeof(double2) * (K + 2) * (2 + ((size_t)std::max(M, N) + 1023) / 1024) * R;
+        if (method == BEM_SOLVE_GMRES && precision != BEM_PREC_FP64 && nrhs == 1 && M > N && ne_fused_fits(M, N))
+            b += sizeof(float2) * (size_t)M * ne_ldr(N) + ws_ne_apply(M, N);  // row-major copy of A
     }
     b += sizeof(float2) * R * ld;  // right-hand-side partials
     return (double)b * 1.05 + (1 << 20);

@@ -150,14 +150,18 @@ def test_python_binding_covers_every_declared_symbol():
 
 def test_request_footprint_bounds_the_allocations():
     """The HBM footprint used for admission covers A and grows with the
-    right-hand sides and the Krylov basis; QR keeps a factored copy of A."""
+    right-hand sides and the Krylov basis; QR keeps a factored copy of A, and
+    GMRES on the normal equations with one right-hand side a row-major one."""
     M, N, E = 10258, 10242, 20480
     ld = (M + 63) // 64 * 64
     f1 = bem.request_footprint(M, N, E, 1)
+    f2 = bem.request_footprint(M, N, E, 2)
     f32 = bem.request_footprint(M, N, E, 32)
-    assert f1 >= 8.0 * ld * N
-    assert f32 > f1 + 16.0 * ld * 31 * 31  # 31 more FP64 bases of 31 vectors
+    assert f1 >= 16.0 * ld * N  # A and its row-major copy
+    assert f2 >= 8.0 * ld * N
+    assert f32 > f2 + 16.0 * ld * 30 * 31  # 30 more FP64 bases of 31 vectors
     assert bem.request_footprint(M, N, E, 1, restart=100) > f1
+    assert bem.request_footprint(M, N, E, 1, method=bem.SOLVE_CGLS) < f1 - 8.0 * M * N
     assert bem.request_footprint(M, N, E, 1, method=bem.SOLVE_QR) >= 16.0 * ld * N
     assert bem.request_footprint(M, N, E, 1, precision=bem.PREC_FP64) >= 16.0 * ld * N
     assert bem.request_footprint(N - 1, N, E, 1) < 0  # underdetermined: refused
