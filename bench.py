@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--only", choices=["plain", "batching", "large", "serving"],
                     help="run one secondary workload alone and print its section (not the benchmark line)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
@@ -568,8 +569,10 @@ def main():
         cpu = cpu_baseline(mesh, k, rho, c, args.cpu_seconds, threads)
         log("cpu baseline", cpu)
 
-    plain = bench_plain(bem, C)
-    log("plain", plain)
+    plain = None
+    if not args.no_plain:
+        plain = bench_plain(bem, C)
+        log("plain", plain)
     batching = None
     if not args.no_batching:
         batching = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
