@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-subdiv", type=int, default=3, help="icosphere subdivision of the plain workload")
+    ap.add_argument("--plain-reps", type=int, default=20)
     ap.add_argument("--only", choices=["plain", "batching", "large", "serving"],
                     help="run one secondary workload alone and print its section (not the benchmark line)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
@@ -208,15 +210,18 @@ def bench_batching(bem, C, workers, fp64=True):
     }
 
 
-def bench_plain(bem, C, reps=20):
+def bench_plain(bem, C, reps=20, subdiv=3):
     """Plain workload (the reference's own small case): rigid sphere r = 0.1 m
     as icosphere s3 (642 nodes), 171 Hz, 8 CHIEF points, unit plane wave along
     +z; FP32 assembly + QR least squares with FP64 refinement + the field at 64
     seeded points on r = 1 m through the field operator.  Errors: surface vs
-    the rigid-sphere series, scattered field vs the exterior series."""
+    the rigid-sphere series, scattered field vs the exterior series.  A larger
+    subdivision runs the same path at that size (for profiling the QR); one
+    extra solve with kernel timing on gives the QR's panel and trailing-update
+    times and the trailing updates' MFMA rate."""
     c, a = 343.0, 0.1
     k = 2 * np.pi * 171.0 / c
-    m = bem.Mesh().icosphere(3, a)
+    m = bem.Mesh().icosphere(subdiv, a)
     m.set_bc(0, 1, 0)
     m.set_chief(C.ball_points(101, 8, a / 2))
     xs = C.sphere_points(101, 64, 1.0)
@@ -238,6 +243,16 @@ def bench_plain(bem, C, reps=20):
         st, p = one()
         lat.append((time.perf_counter() - ts) * 1e3)
     wall = time.perf_counter() - t0
+    bem.timing_reset()
+    bem.timing_enable(True)
+    s.solve(bem.SOLVE_QR, refine_tol=1e-10, max_refine=10)
+    bem.sync()
+    bem.timing_enable(False)
+    p_ms, p_n = bem.timing_get("qr_panel_f32")
+    t_ms, t_n = bem.timing_get("qr_trailing_f32")
+    bem.timing_reset()
+    Mq, Nq = m.num_pnts + m.num_chief, m.num_pnts
+    qr_flop = 8.0 * Mq * Nq * Nq - 8.0 * Nq ** 3 / 3
     x = s.solution()[:, 0]
     pn = m.pnts().astype(np.float64)
     ser = C.rigid_sphere_surface(a, k, pn[:, 2] / np.linalg.norm(pn, axis=1))
@@ -245,14 +260,17 @@ def bench_plain(bem, C, reps=20):
     fser = C.rigid_sphere_field(a, k, xs)
     s.close()
     return {
-        "workload": "plain: rigid sphere icosphere s3 (N=642), 171 Hz, 8 CHIEF, plane wave; FP32 assembly + QR with "
-                    "FP64 refinement + field at 64 points on r=1 m (field operator)",
+        "workload": "plain: rigid sphere icosphere s%d (N=%d), 171 Hz, 8 CHIEF, plane wave; FP32 assembly + QR with "
+                    "FP64 refinement + field at 64 points on r=1 m (field operator)" % (subdiv, Nq),
         "value": round(reps / wall, 2), "unit": "solves/s", "latency_ms": {
             "p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
         "solve": {kk: st[kk] for kk in ("refinements", "converged", "residual")},
         "rel_err_surface_vs_series": float(np.linalg.norm(x - ser) / np.linalg.norm(ser)),
         "rel_err_scattered_field_vs_series": float(np.linalg.norm((p - inc) - (fser - inc)) /
                                                    np.linalg.norm(fser - inc)),
+        "qr_factor": {"panels": p_n, "panel_ms": round(p_ms, 3), "trailing_ms": round(t_ms, 3),
+                      "trailing_tflops_mfma": round(qr_flop / (t_ms * 1e-3) / 1e12, 2) if t_ms else None,
+                      "flop": qr_flop},
     }
 
 
@@ -455,7 +473,7 @@ def main():
         bem.quad_init(6)
         bem.set_medium(1.21, 343.0)
         if args.only == "plain":
-            res = bench_plain(bem, C)
+            res = bench_plain(bem, C, args.plain_reps, args.plain_subdiv)
         elif args.only == "batching":
             res = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
         elif args.only == "large":
@@ -571,7 +589,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        plain = bench_plain(bem, C)
+        plain = bench_plain(bem, C, args.plain_reps, args.plain_subdiv)
         log("plain", plain)
     batching = None
     if not args.no_batching:
