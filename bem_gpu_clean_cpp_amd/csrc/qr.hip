@@ -804,6 +804,177 @@ __global__ __launch_bounds__(NT) void qr_solve_small(const typename QT<T>::c2 *A
     for (int i = tid; i < N; i += NT) b[i] = sb[i];
 }
 
+// Q^H B for systems past the one-workgroup solve: one launch per panel, the
+// rows split over workgroups of NT threads (one row per thread, one
+// right-hand side per blockIdx.y).  Launch p finishes panel p on the block's
+// rows (z = sum of the partials of V_p^H b in a fixed order, w = T_p^H z,
+// b -= V_p w) and then forms the block's partial of V_{p+1}^H b over the same
+// rows, whose values are final for panel p+1.  p = -1 only forms the partials
+// of panel 0.  Partials alternate between two buffers by panel parity.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void qr_apply_step(const typename QT<T>::c2 *A, long lda, int M, int N,
+                                                    const typename QT<T>::c2 *Tb, typename QT<T>::c2 *B, long ldb,
+                                                    int p, const T *zin, T *zout, int nblk) {
+    using C2 = typename QT<T>::c2;
+    __shared__ T s_red[NT / 64][2 * kNB];
+    __shared__ C2 s_z[kNB], s_w[kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, col = blockIdx.y;
+    const int i = blockIdx.x * NT + tid;
+    const bool live = i < M;
+    C2 *b = B + (size_t)col * ldb;
+    C2 bi;
+    bi.x = 0;
+    bi.y = 0;
+    if (live) bi = b[i];
+    const int ic = min(i, M - 1);
+    const int np = (N + kNB - 1) / kNB;
+    if (p >= 0) {
+        const int j0 = p * kNB, nb = min(kNB, N - j0);
+        if (tid < 2 * kNB) {
+            T t = 0;
+            for (int q = 0; q < nblk; q++) t += zin[((size_t)col * nblk + q) * 2 * kNB + tid];
+            ((T *)s_z)[tid] = t;
+        }
+        __syncthreads();
+        if (tid < nb) {
+            // w = T^H z; T upper triangular
+            const C2 *Tm = Tb + (size_t)p * kNB * kNB;
+            T wr = 0, wi = 0;
+            for (int q = 0; q <= tid; q++) {
+                const C2 t = Tm[q * kNB + tid];
+                wr += t.x * s_z[q].x + t.y * s_z[q].y;
+                wi += t.x * s_z[q].y - t.y * s_z[q].x;
+            }
+            s_w[tid].x = wr;
+            s_w[tid].y = wi;
+        }
+        __syncthreads();
+        T sr = 0, si = 0;
+#pragma unroll
+        for (int l0 = 0; l0 < kNB; l0 += kCG) {
+            C2 raw[kCG];
+#pragma unroll
+            for (int u = 0; u < kCG; u++) raw[u] = A[(size_t)min(j0 + l0 + u, N - 1) * lda + ic];
+#pragma unroll
+            for (int u = 0; u < kCG; u++) {
+                const int l = l0 + u;
+                const bool below = i > j0 + l && l < nb, diag = i == j0 + l && l < nb;
+                const T vx = below ? raw[u].x : (diag ? (T)1 : (T)0), vy = below ? raw[u].y : (T)0;
+                sr += vx * s_w[l].x - vy * s_w[l].y;
+                si += vx * s_w[l].y + vy * s_w[l].x;
+            }
+        }
+        if (live && i >= j0) {
+            bi.x -= sr;
+            bi.y -= si;
+            b[i] = bi;
+        }
+    }
+    if (p + 1 < np) {
+        const int j0 = (p + 1) * kNB, nb = min(kNB, N - j0);
+        T x[2 * kNB];
+#pragma unroll
+        for (int l = 0; l < 2 * kNB; l++) x[l] = 0;
+#pragma unroll
+        for (int l0 = 0; l0 < kNB; l0 += kCG) {
+            C2 raw[kCG];
+#pragma unroll
+            for (int u = 0; u < kCG; u++) raw[u] = A[(size_t)min(j0 + l0 + u, N - 1) * lda + ic];
+#pragma unroll
+            for (int u = 0; u < kCG; u++) {
+                const int l = l0 + u;
+                const bool below = live && i > j0 + l && l < nb, diag = i == j0 + l && l < nb;
+                const T vx = below ? raw[u].x : (diag ? (T)1 : (T)0), vy = below ? raw[u].y : (T)0;
+                x[2 * l] = vx * bi.x + vy * bi.y;
+                x[2 * l + 1] = vx * bi.y - vy * bi.x;
+            }
+        }
+        wave_reduce_scatter64(x, lane);
+        s_red[wid][lane] = x[0];
+        __syncthreads();
+        if (tid < 2 * kNB) {
+            T t = 0;
+#pragma unroll
+            for (int w = 0; w < NT / 64; w++) t += s_red[w][tid];
+            zout[((size_t)col * nblk + blockIdx.x) * 2 * kNB + tid] = t;
+        }
+    }
+}
+
+// One block column of R x = b for systems past the one-workgroup solve: every
+// workgroup solves the nb x nb diagonal block at c0 redundantly (one wave,
+// the block's rows of R in registers; b[c0..c0+nb) is not written in this
+// step), workgroup 0 stores the unknowns in X, and every workgroup removes
+// them from its rows below c0 of the working right-hand side B.
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void trsv_block_step(const typename QT<T>::c2 *A, long lda, int N,
+                                                      typename QT<T>::c2 *B, long ldb, typename QT<T>::c2 *X,
+                                                      long ldx, int c0, int nb) {
+    using C2 = typename QT<T>::c2;
+    __shared__ C2 s_x[kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, col = blockIdx.y;
+    C2 *b = B + (size_t)col * ldb;
+    if (wid == 0) {
+        const int r = c0 + lane, rc = min(r, N - 1);
+        C2 rr[kNB];
+#pragma unroll
+        for (int m = 0; m < kNB; m++) rr[m] = A[(size_t)min(c0 + m, N - 1) * lda + rc];
+#pragma unroll
+        for (int m = 0; m < kNB; m++) {
+            if (!(lane < nb && m < nb && m >= lane)) {
+                rr[m].x = 0;
+                rr[m].y = 0;
+            }
+        }
+        C2 bl = b[rc];
+        if (lane >= nb) {
+            bl.x = 0;
+            bl.y = 0;
+        }
+#pragma unroll
+        for (int m = kNB - 1; m >= 0; m--) {
+            if (m < nb) {
+                C2 xm;
+                const T d = rr[m].x * rr[m].x + rr[m].y * rr[m].y;
+                xm.x = lane == m ? (bl.x * rr[m].x + bl.y * rr[m].y) / d : 0;
+                xm.y = lane == m ? (bl.y * rr[m].x - bl.x * rr[m].y) / d : 0;
+                xm.x = __shfl(xm.x, m, 64);
+                xm.y = __shfl(xm.y, m, 64);
+                if (lane < m) {
+                    bl.x -= rr[m].x * xm.x - rr[m].y * xm.y;
+                    bl.y -= rr[m].x * xm.y + rr[m].y * xm.x;
+                } else if (lane == m) {
+                    bl = xm;
+                }
+            }
+        }
+        if (lane < nb) {
+            s_x[lane] = bl;
+            if (blockIdx.x == 0) X[(size_t)col * ldx + r] = bl;
+        }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * NT + tid;
+    if (i < c0) {
+        T sr = 0, si = 0;
+#pragma unroll
+        for (int m0 = 0; m0 < kNB; m0 += kCG) {
+            C2 raw[kCG];
+#pragma unroll
+            for (int u = 0; u < kCG; u++) raw[u] = A[(size_t)min(c0 + m0 + u, N - 1) * lda + i];
+#pragma unroll
+            for (int u = 0; u < kCG; u++) {
+                if (m0 + u < nb) {
+                    sr += raw[u].x * s_x[m0 + u].x - raw[u].y * s_x[m0 + u].y;
+                    si += raw[u].x * s_x[m0 + u].y + raw[u].y * s_x[m0 + u].x;
+                }
+            }
+        }
+        b[i].x -= sr;
+        b[i].y -= si;
+    }
+}
+
 // ------------------------------------------------------------------ host ---
 template <typename T>
 struct QRFactor {
@@ -950,9 +1121,41 @@ static int qr_solve_factored(const QRFactor<T> &f, typename QT<T>::c2 *B, long l
         HIP_CALL(hipGetLastError());
         return EXIT_SUCCESS;
     }
-    HOST_CALL(qr_apply<T>(f, B, ldb, R, 1, s));
-    trsv_upper<T><<<R, 1024, 0, s>>>(f.A, f.lda, f.N, B, ldb);
-    HIP_CALL(hipGetLastError());
+    if (getenv("BEM_QR_PER_COLUMN_SOLVE")) {
+        HOST_CALL(qr_apply<T>(f, B, ldb, R, 1, s));
+        trsv_upper<T><<<R, 1024, 0, s>>>(f.A, f.lda, f.N, B, ldb);
+        HIP_CALL(hipGetLastError());
+        return EXIT_SUCCESS;
+    }
+    // larger systems: one launch per panel for Q^H B and one per block column
+    // for the back substitution, the rows spread over workgroups
+    constexpr int nt = 256;
+    const int np = (f.N + kNB - 1) / kNB;
+    const int nblk = (f.M + nt - 1) / nt;
+    T *zb = nullptr;
+    C2 *X = nullptr;
+    HOST_CALL(salloc(&zb, (size_t)2 * R * nblk * 2 * kNB, s));
+    HOST_CALL(salloc(&X, (size_t)f.N * R, s));
+    int rc = EXIT_SUCCESS;
+    for (int p = -1; p < np && rc == EXIT_SUCCESS; p++) {
+        const T *zin = zb + (size_t)((p + 2) & 1) * R * nblk * 2 * kNB;
+        T *zout = zb + (size_t)((p + 1) & 1) * R * nblk * 2 * kNB;
+        qr_apply_step<T, nt><<<dim3(nblk, R), nt, 0, s>>>(f.A, f.lda, f.M, f.N, f.Tb, B, ldb, p, zin, zout, nblk);
+        if (hipGetLastError() != hipSuccess) rc = EXIT_FAILURE;
+    }
+    for (int p = np - 1; p >= 0 && rc == EXIT_SUCCESS; p--) {
+        const int c0 = p * kNB, nb = std::min(kNB, f.N - c0);
+        const int g = std::max(1, (c0 + nt - 1) / nt);
+        trsv_block_step<T, nt><<<dim3(g, R), nt, 0, s>>>(f.A, f.lda, f.N, B, ldb, X, f.N, c0, nb);
+        if (hipGetLastError() != hipSuccess) rc = EXIT_FAILURE;
+    }
+    if (rc == EXIT_SUCCESS &&
+        hipMemcpy2DAsync(B, sizeof(C2) * ldb, X, sizeof(C2) * f.N, sizeof(C2) * f.N, R, hipMemcpyDeviceToDevice,
+                         s) != hipSuccess)
+        rc = EXIT_FAILURE;
+    sfree(zb, s);
+    sfree(X, s);
+    if (rc != EXIT_SUCCESS) BEM_FAIL("QR solve launches failed");
     return EXIT_SUCCESS;
 }
 

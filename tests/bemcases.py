@@ -123,3 +123,14 @@ def rigid_sphere_field(a, k, x):
         h = special.spherical_jn(n, k * r) - 1j * special.spherical_yn(n, k * r)
         p += (2 * n + 1) * (-1j) ** n * (special.spherical_jn(n, k * r) - dj / dh * h) * special.eval_legendre(n, ct)
     return p
+
+
+def two_spheres(bem, subdiv, radius, gap=0.5, bc=(0, 1, 0)):
+    """Two icospheres of the given radius centred at x = -/+ (1 + gap/2) radius,
+    so their surfaces are gap * radius apart: one mesh of 2 (10 4^s + 2) nodes."""
+    a = bem.Mesh().icosphere(subdiv, radius)
+    p, e = a.pnts(), a.elems()["nodes"]
+    d = (1.0 + gap / 2) * radius
+    pts = np.vstack([p - [d, 0, 0], p + [d, 0, 0]]).astype(np.float32)
+    els = np.vstack([e, e + p.shape[0]]).astype(np.int32)
+    return bem.Mesh().set(pts, els, bc)

@@ -29,18 +29,9 @@ def _setup():
     bem.set_medium(1.2041, 343.21)
 
 
-def _two_spheres(s, r):
-    a = bem.Mesh().icosphere(s, r)
-    p, e = a.pnts(), a.elems()["nodes"]
-    pts = np.vstack([p - [2.5 * r, 0, 0], p + [2.5 * r, 0, 0]]).astype(np.float32)
-    els = np.vstack([e, e + p.shape[0]]).astype(np.int32)
-    m = bem.Mesh().set(pts, els, (0, 1, 0))
-    return m
-
-
 def _mesh(case):
     if case == "two_s4":
-        m = _two_spheres(4, 0.1)
+        m = C.two_spheres(bem, 4, 0.1, gap=3.0)
         m.set_chief(C.ball_points(23, 16, 0.05) - [0.25, 0, 0])
     else:
         m = bem.Mesh().icosphere(int(case[1]), 0.1)

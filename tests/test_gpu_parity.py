@@ -200,6 +200,61 @@ def test_qr_fp64_fused_solve_matches_launch_per_panel(monkeypatch):
     assert np.linalg.norm(x - xn) / np.linalg.norm(xn) < 1e-12
 
 
+def test_qr_launch_per_panel_solve_many_rhs(monkeypatch):
+    """More than 64 right-hand sides take the multi-workgroup solve (one launch
+    per panel for Q^H B, one per block column for R X = Q^H B): against LAPACK
+    and against the per-column kernels (BEM_QR_PER_COLUMN_SOLVE)."""
+    rng = np.random.default_rng(11)
+    m, n, r = 300, 261, 70
+    A = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+    A += 20 * np.eye(m, n, dtype=np.complex64)
+    B = (rng.standard_normal((m, r)) + 1j * rng.standard_normal((m, r))).astype(np.complex64)
+    X = bem.qr_solve(A, B)
+    Xo = np.linalg.lstsq(A.astype(np.complex128), B.astype(np.complex128), rcond=None)[0]
+    assert np.linalg.norm(X - Xo) / np.linalg.norm(Xo) < 1e-5
+    monkeypatch.setenv("BEM_QR_PER_COLUMN_SOLVE", "1")
+    Xc = bem.qr_solve(A, B)
+    assert np.linalg.norm(X - Xc) / np.linalg.norm(Xo) < 1e-5
+
+
+def test_qr_launch_per_panel_solve_tall(monkeypatch):
+    """6400 x 6300 (past the one-workgroup solve's 6144 rows), 2 right-hand
+    sides: the normal-equation residual A^H (b - A x) against the right-hand
+    side's A^H b, and the per-column kernels' solution."""
+    rng = np.random.default_rng(12)
+    m, n = 6400, 6300
+    A = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+    A += 80 * np.eye(m, n, dtype=np.complex64)
+    B = (rng.standard_normal((m, 2)) + 1j * rng.standard_normal((m, 2))).astype(np.complex64)
+    print("case: QR least squares 6400 x 6300, 2 right-hand sides", flush=True)
+    X = bem.qr_solve(A, B)
+    A64, B64, X64 = A.astype(np.complex128), B.astype(np.complex128), X.astype(np.complex128)
+    res = A64.conj().T @ (B64 - A64 @ X64)
+    ref = A64.conj().T @ B64
+    assert np.linalg.norm(res) / np.linalg.norm(ref) < 1e-5
+    monkeypatch.setenv("BEM_QR_PER_COLUMN_SOLVE", "1")
+    Xc = bem.qr_solve(A, B)
+    assert np.linalg.norm(X - Xc) / np.linalg.norm(Xc) < 1e-5
+
+
+def test_qr_fp64_multi_workgroup_solve_matches_per_column(monkeypatch):
+    """FP64 QR of a two-sphere CHIEF system with M = 5140 rows (past the
+    one-workgroup FP64 solve's 3072) and 3 right-hand sides: the
+    multi-workgroup solve against the per-column kernels, to FP64 accuracy."""
+    m = C.two_spheres(bem, 4, 0.1)
+    m.set_chief(C.ball_points(29, 16, 0.05) - [0.125, 0, 0])
+    print("case: FP64 QR, two s4 spheres, N %d M %d" % (m.num_pnts, m.num_pnts + m.num_chief), flush=True)
+    s = bem.Solver(m)
+    s.assemble(9.0, bem.PREC_FP64)
+    s.set_rhs(bem.RHS_PLANE, [[0, 0, 1], [0.6, 0.8, 0], [1, 0, 0]], 1.0)
+    s.solve(bem.SOLVE_QR)
+    x = s.solution()
+    monkeypatch.setenv("BEM_QR_PER_COLUMN_SOLVE", "1")
+    s.solve(bem.SOLVE_QR)
+    xc = s.solution()
+    assert np.linalg.norm(x - xc) / np.linalg.norm(xc) < 1e-12
+
+
 def test_lsq_solve_returns_r_and_q():
     rng = np.random.default_rng(2)
     m, n = 90, 70
