@@ -38,6 +38,7 @@ struct QT<double> {
 
 constexpr int kNB = 32;  // panel width
 constexpr int kCG = 8;   // panel columns updated together
+constexpr int kColThreads = 256;  // rows per workgroup of the tall-panel column launches
 
 template <typename T>
 __device__ __forceinline__ T wsum(T v) {
@@ -452,6 +453,132 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
     }
     __syncthreads();
     for (int p = tid; p < kNB * kNB; p += NT) Tout[p] = s_T[p / kNB][p % kNB];
+}
+
+// Tall FP32 panels (more rows than the register panel holds): one launch per
+// column, the rows split over workgroups (one row per thread).  Launch jj
+// applies the reflector of column jj to the workgroup's rows and then forms
+// the partials of column jj + 1 from the updated rows: |x|^2 below its
+// diagonal and sum conj(x_r) a_c(r) for the later columns c, x being the
+// column below the diagonal before scaling.  With v = x / (alpha - beta) below
+// the diagonal and v_j = 1,
+//   v^H a_c = a_c(j) + conj(scale) sum_{r > j} conj(x_r) a_c(r),
+// so the reflector's products need no pass of their own after its norm.  The
+// partials are summed in a fixed order by every workgroup of the next launch.
+// Row j of the panel (alpha and the a_c(j)) travels in its own buffer,
+// written by the workgroup holding the row, because that workgroup overwrites
+// the row during the launch that reads it.  jj = -1 only forms the partials of
+// column 0.  Column indices are runtime values here, so the row is walked with
+// compile-time column loops and compares, never by a runtime register index.
+// Leaves the reflectors and R in place and tau_jj on Tout's diagonal.
+template <int NT>
+__global__ __launch_bounds__(NT) void qr_panel_col(float2 *A, long lda, int M, int j0, int jj, const float *zin,
+                                                   const float2 *rin, float *zout, float2 *rout, int nblk,
+                                                   float2 *Tout) {
+    __shared__ float s_red[NT / 64][2 * kNB];
+    __shared__ float s_z[2 * kNB];
+    __shared__ float2 s_d[kNB];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int i = j0 + blockIdx.x * NT + tid;
+    const bool live = i < M;
+    const int ic = min(i, M - 1);
+    float2 a[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; c++) a[c] = A[(size_t)(j0 + c) * lda + ic];
+    if (jj >= 0) {
+        const int j = j0 + jj;
+        if (tid < 2 * kNB) {
+            float t = 0.f;
+            for (int q = 0; q < nblk; q++) t += zin[(size_t)q * 2 * kNB + tid];
+            s_z[tid] = t;
+        }
+        __syncthreads();
+        const float tot = s_z[2 * jj];
+        const float2 alpha = rin[jj];
+        const float aa = sqrtf(alpha.x * alpha.x + alpha.y * alpha.y);
+        const float xnorm = sqrtf(aa * aa + tot);
+        float tau = 0.f;
+        float2 scale = make_float2(0.f, 0.f), beta = alpha;
+        if (tot > 0 || (xnorm > 0 && alpha.y != 0)) {
+            const float phr = aa > 0 ? alpha.x / aa : 1.f, phi = aa > 0 ? alpha.y / aa : 0.f;
+            beta = make_float2(-phr * xnorm, -phi * xnorm);
+            const float dr = alpha.x - beta.x, di = alpha.y - beta.y;
+            const float dd = dr * dr + di * di;
+            scale = make_float2(dr / dd, -di / dd);
+            tau = 2.f / (1.f + tot / dd);
+        }
+        if (tid < kNB) {
+            const int c = tid;
+            float2 d = make_float2(0.f, 0.f);
+            if (c > jj) {
+                // tau (a_c(j) + conj(scale) S_c)
+                const float sx = s_z[2 * c], sy = s_z[2 * c + 1];
+                const float2 ac = rin[c];
+                d.x = tau * (ac.x + scale.x * sx + scale.y * sy);
+                d.y = tau * (ac.y + scale.x * sy - scale.y * sx);
+            }
+            s_d[c] = d;
+        }
+        if (blockIdx.x == 0 && tid == 0) Tout[jj * kNB + jj] = make_float2(tau, 0.f);
+        __syncthreads();
+        if (live && i >= j) {
+            float2 x = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int c = 0; c < kNB; c++)
+                if (c == jj) x = a[c];
+            const bool below = i > j;
+            const float2 v = below ? make_float2(x.x * scale.x - x.y * scale.y, x.x * scale.y + x.y * scale.x)
+                                   : make_float2(1.f, 0.f);
+#pragma unroll
+            for (int c = 0; c < kNB; c++) {
+                if (c == jj) {
+                    a[c] = below ? v : beta;
+                } else if (c > jj) {
+                    const float2 d = s_d[c];
+                    a[c].x -= d.x * v.x - d.y * v.y;
+                    a[c].y -= d.x * v.y + d.y * v.x;
+                }
+                if (c >= jj) A[(size_t)(j0 + c) * lda + i] = a[c];
+            }
+        }
+    }
+    if (jj + 1 < kNB) {
+        const int k = jj + 1, j1 = j0 + k;
+        float2 x = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int c = 0; c < kNB; c++)
+            if (c == k) x = a[c];
+        const bool below = live && i > j1;
+        float z[2 * kNB];
+#pragma unroll
+        for (int c = 0; c < kNB; c++) {
+            float pr = 0.f, pi = 0.f;
+            if (below) {
+                if (c == k) {
+                    pr = x.x * x.x + x.y * x.y;
+                } else if (c > k) {
+                    pr = x.x * a[c].x + x.y * a[c].y;
+                    pi = x.x * a[c].y - x.y * a[c].x;
+                }
+            }
+            z[2 * c] = pr;
+            z[2 * c + 1] = pi;
+        }
+        if (live && i == j1) {
+#pragma unroll
+            for (int c = 0; c < kNB; c++)
+                if (c >= k) rout[c] = a[c];
+        }
+        wave_reduce_scatter64(z, lane);
+        s_red[wid][lane] = z[0];
+        __syncthreads();
+        if (tid < 2 * kNB) {
+            float t = 0.f;
+#pragma unroll
+            for (int w = 0; w < NT / 64; w++) t += s_red[w][tid];
+            zout[(size_t)blockIdx.x * 2 * kNB + tid] = t;
+        }
+    }
 }
 
 // T of the compact WY form from the taus (on Tout's diagonal) and
@@ -990,6 +1117,10 @@ struct QRFactor {
     C2 *Yz = nullptr;  // V^H V of the panel (kNB x kNB)
     char *gws = nullptr;
     long ldv = 0;
+    // partials and panel rows of the one-launch-per-column tall panel
+    float *zcol = nullptr;
+    long zcol_stride = 0;
+    float2 *rcol = nullptr;
     hipStream_t st = nullptr;
     ~QRFactor() {
         sfree(Tb, st);
@@ -998,6 +1129,8 @@ struct QRFactor {
         sfree(Z, st);
         sfree(Yz, st);
         sfree(gws, st);
+        sfree(zcol, st);
+        sfree(rcol, st);
     }
 };
 
@@ -1041,6 +1174,9 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
             if (f.N - (j0 + nb) > 0) wsz = std::max(wsz, ws_gemm_h_f32(f.M - j0, f.N - (j0 + nb), kNB));
         }
         HOST_CALL(salloc(&f.gws, std::max<size_t>(wsz, 1), s));
+        f.zcol_stride = (long)((f.M + kColThreads - 1) / kColThreads) * 2 * kNB;
+        HOST_CALL(salloc(&f.zcol, (size_t)2 * f.zcol_stride, s));
+        HOST_CALL(salloc(&f.rcol, (size_t)2 * kNB, s));
     }
     for (int p = 0; p < np; p++) {
         const int j0 = p * kNB, nb = std::min(kNB, f.N - j0);
@@ -1060,6 +1196,17 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
             qr_panel_reg<512, 1><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, Tf, Vf, f.ldv);
         } else if (reg == 3) {
             qr_panel_reg<512, 2><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, Tf, Vf, f.ldv);
+        } else if (fit && !getenv("BEM_QR_ONE_WG_PANEL")) {
+            // tall panel: one launch per column over kColThreads-row workgroups
+            const int nblk = (Mp + kColThreads - 1) / kColThreads;
+            for (int jj = -1; jj < kNB; jj++) {
+                const int bi = jj & 1, bo = (jj + 1) & 1;
+                qr_panel_col<kColThreads><<<nblk, kColThreads, 0, s>>>(
+                    (float2 *)f.A, f.lda, f.M, j0, jj, f.zcol + (size_t)bi * f.zcol_stride,
+                    (const float2 *)f.rcol + (size_t)bi * kNB, f.zcol + (size_t)bo * f.zcol_stride,
+                    (float2 *)f.rcol + (size_t)bo * kNB, nblk, Tf);
+                HIP_CALL(hipGetLastError());
+            }
         } else {
             qr_panel<T><<<1, 1024, 0, s>>>(f.A, f.lda, f.M, j0, nb, Tp, mfma ? 0 : 1);
         }

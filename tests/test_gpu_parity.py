@@ -159,11 +159,11 @@ def test_qr_solve_matches_lapack():
 
 
 def test_qr_panel_variants_match_lapack(monkeypatch):
-    """1100 x 1040: the first panels (over 1024 rows) run the global-memory
-    panel kernel, then the register panels with 2 rows per thread (<= 1024
-    rows), 1 row per thread (<= 512, <= 256), and the last 16 columns the
-    global kernel again.  Against LAPACK, and against the global panel kernel
-    for every panel (BEM_QR_GPANEL)."""
+    """1100 x 1040: the first panels (over 1024 rows) run one launch per
+    column over row-split workgroups, then the register panels with 2 rows per
+    thread (<= 1024 rows), 1 row per thread (<= 512, <= 256), and the last 16
+    columns the one-workgroup global-memory panel.  Against LAPACK, and
+    against the one-workgroup panel for every panel (BEM_QR_GPANEL)."""
     rng = np.random.default_rng(7)
     m, n = 1100, 1040
     A = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
