@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void kb_mdot_part(const double2 *__restrict__ 
 
 // out[r * ostride + l] = sum_blocks part; with acc: acc[r * ostride + l] += the sum
 __global__ void kb_mdot_final(const double2 *__restrict__ part, long pstride, int nblk, int nv,
-                              double2 *__restrict__ out, long ostride, double2 *__restrict__ acc) {
+                              double2 *__restrict__ out, long ostride, double2 *__restrict__ acc, long astride) {
     const int r = blockIdx.y, l = threadIdx.x;
     if (l >= nv) return;
     part += (size_t)r * pstride;
@@ -87,10 +87,10 @@ __global__ void kb_mdot_final(const double2 *__restrict__ part, long pstride, in
     }
     out[(size_t)r * ostride + l] = make_double2(sr, si);
     if (acc) {
-        double2 a = acc[(size_t)r * ostride + l];
+        double2 a = acc[(size_t)r * astride + l];
         a.x += sr;
         a.y += si;
-        acc[(size_t)r * ostride + l] = a;
+        acc[(size_t)r * astride + l] = a;
     }
 }
 
@@ -155,6 +155,8 @@ __global__ void kb_cvt(const float2 *__restrict__ in, long istride, double2 *__r
 }
 
 // ---------------------------------------------------------------- driver ---
+constexpr int kMaxChunk = 8;  // Arnoldi steps between reads of the Hessenberg columns
+
 struct KrylovWS {
     int n = 0, m = 0, R = 0, restart = 0, nblk = 0;
     long ldv = 0, vstride = 0, hstride = 0, pstride = 0;
@@ -166,10 +168,12 @@ struct KrylovWS {
     float2 *Ar = nullptr;
     long ldr = 0;
     char *nws = nullptr;
+    double2 *hch = nullptr;  // Hessenberg columns of an Arnoldi chunk
     hipStream_t st = nullptr;
     ~KrylovWS() {
         sfree(V, st); sfree(w64, st); sfree(t64, st); sfree(r, st); sfree(h, st); sfree(part, st);
         sfree(v32, st); sfree(w32, st); sfree(t32, st); sfree(gws, st); sfree(Ar, st); sfree(nws, st);
+        sfree(hch, st);
     }
     int init(int n_, int m_, int R_, int restart_, hipStream_t s) {
         st = s;
@@ -199,13 +203,14 @@ struct KrylovWS {
 
 // batched dots <V_r[:, 0..nv), w_r> -> out[r * ostride + l] (and acc += out)
 static int bdot(KrylovWS &ws, const double2 *V, long vstride, int nv, const double2 *w, long wstride, int n,
-                double2 *out, long ostride, hipStream_t s, double2 *acc = nullptr) {
+                double2 *out, long ostride, hipStream_t s, double2 *acc = nullptr, long astride = 0) {
     const int nblk = (n + 1023) / 1024;
     dim3 g1(nblk, ws.R, (nv + 3) / 4);
     kb_mdot_part<<<g1, 256, 0, s>>>(V, ws.ldv, vstride, nv, w, wstride, n, ws.part, ws.pstride);
     HIP_CALL(hipGetLastError());
     dim3 g2(1, ws.R);
-    kb_mdot_final<<<g2, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc);
+    kb_mdot_final<<<g2, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc,
+                                                        astride ? astride : ostride);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
@@ -367,6 +372,14 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             HIP_CALL(hipMemsetAsync(X + (size_t)q * ldx, 0, sizeof(double2) * N, s));
         }
     const int K = restart;
+    // Hessenberg column slots of a chunk: per system kMaxChunk slots of K + 2
+    // entries (the products at 0..j, |w|^2 at K + 1)
+    const long hs = K + 2, hcs = hs * kMaxChunk;
+    HOST_CALL(salloc(&ws.hch, (size_t)hcs * R, s));
+    std::vector<double2> hch((size_t)hcs * R);
+    double rate = 0.0;
+    std::vector<double> g_start(R), g_end(R);
+    std::vector<int> steps(R);
     std::vector<zc> H((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K), g((size_t)R * (K + 1));
     std::vector<int> tlen(R, 0);
     std::vector<char> live(R, 0);
@@ -404,59 +417,103 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             g[(size_t)q * (K + 1)] = beta[q];
             if (done[q] && op.A32) HIP_CALL(hipMemsetAsync(ws.v32 + (size_t)q * ldv, 0, sizeof(float2) * ldv, s));
         }
+        // Arnoldi steps in chunks: the steps of a chunk are launched back to
+        // back and write their Hessenberg columns to slots of hch; one copy to
+        // the host then gives all of them, and the Givens rotations and the
+        // convergence test run over the chunk's steps in order (steps past a
+        // system's convergence are ignored).  The chunk length is the number
+        // of steps the slowest live system still needs at its observed rate
+        // of decrease of |g| (4 for the first chunk of the first cycle), at
+        // most kMaxChunk, so few steps run past convergence.
         int j = 0;
-        for (; j < K && total < maxit; j++) {
-            total++;
-            HOST_CALL(op.apply(ws, j, s));
-            double2 *h1 = ws.h, *h2 = ws.h + (K + 2);
-            dim3 gg((N + bs - 1) / bs, R);
-            HOST_CALL(bdot(ws, ws.V, ws.vstride, j + 1, ws.w64, ldv, N, h1, ws.hstride, s));
-            kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, j + 1, h1, ws.hstride, -1.0, ws.w64, ldv, N);
-            HIP_CALL(hipGetLastError());
-            HOST_CALL(bdot(ws, ws.V, ws.vstride, j + 1, ws.w64, ldv, N, h2, ws.hstride, s, h1));  // h1 += h2
-            kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, j + 1, h2, ws.hstride, -1.0, ws.w64, ldv, N);
-            HIP_CALL(hipGetLastError());
-            HOST_CALL(bnorm2(ws, ws.w64, ldv, N, h2, ws.hstride, s));
-            kb_normalize<<<gg, bs, 0, s>>>(ws.w64, ldv, h2, ws.hstride, ws.V + (size_t)(j + 1) * ldv, ws.vstride,
-                                           op.A32 ? ws.v32 : nullptr, ldv, N);
-            HIP_CALL(hipGetLastError());
-            HIP_CALL(hipMemcpyAsync(hbuf.data(), ws.h, sizeof(double2) * ws.hstride * R, hipMemcpyDeviceToHost, s));
+        while (j < K && total < maxit) {
+            int c = std::min({kMaxChunk, K - j, maxit - total});
+            if (rate < 1.0 && rate > 0.0) {
+                int need = 1;
+                for (int q = 0; q < R; q++) {
+                    if (!live[q]) continue;
+                    const double gj = std::abs(g[(size_t)q * (K + 1) + j]);
+                    if (gj <= tol * beta[q]) continue;
+                    need = std::max(need, (int)std::ceil(std::log(tol * beta[q] / gj) / std::log(rate)));
+                }
+                c = std::max(1, std::min(c, need));
+            } else if (j == 0 && cycles == 1) {
+                c = std::min(c, 4);
+            }
+            for (int t = 0; t < c; t++) {
+                const int jj = j + t;
+                HOST_CALL(op.apply(ws, jj, s));
+                double2 *h1 = ws.hch + (size_t)t * hs, *h2 = ws.h + (K + 2);
+                dim3 gg((N + bs - 1) / bs, R);
+                HOST_CALL(bdot(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, h1, hcs, s));
+                kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, jj + 1, h1, hcs, -1.0, ws.w64, ldv, N);
+                HIP_CALL(hipGetLastError());
+                HOST_CALL(bdot(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, h2, ws.hstride, s, h1, hcs));  // h1 += h2
+                kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, jj + 1, h2, ws.hstride, -1.0, ws.w64, ldv, N);
+                HIP_CALL(hipGetLastError());
+                HOST_CALL(bnorm2(ws, ws.w64, ldv, N, h1 + (K + 1), hcs, s));
+                kb_normalize<<<gg, bs, 0, s>>>(ws.w64, ldv, h1 + (K + 1), hcs, ws.V + (size_t)(jj + 1) * ldv,
+                                               ws.vstride, op.A32 ? ws.v32 : nullptr, ldv, N);
+                HIP_CALL(hipGetLastError());
+            }
+            HIP_CALL(hipMemcpyAsync(hch.data(), ws.hch, sizeof(double2) * hcs * R, hipMemcpyDeviceToHost, s));
             HIP_CALL(hipStreamSynchronize(s));
-            bool anylive = false;
             for (int q = 0; q < R; q++) {
-                if (!live[q]) continue;
-                const double2 *hq = &hbuf[(size_t)q * ws.hstride];
-                zc *hc = &H[((size_t)q * K + j) * (K + 1)];
-                zc *csq = &cs[(size_t)q * K], *snq = &sn[(size_t)q * K], *gq = &g[(size_t)q * (K + 1)];
-                for (int l = 0; l <= j; l++) hc[l] = zc(hq[l].x, hq[l].y);
-                const double hn = std::sqrt(std::max(0.0, hq[K + 2].x));
-                hc[j + 1] = hn;
-                // previous rotations, then a new one (GMRES.cu:58-88 / 44-56)
-                for (int l = 0; l < j; l++) {
-                    const zc t0 = csq[l] * hc[l] + snq[l] * hc[l + 1];
-                    const zc t1 = -std::conj(snq[l]) * hc[l] + std::conj(csq[l]) * hc[l + 1];
-                    hc[l] = t0;
-                    hc[l + 1] = t1;
-                }
-                const double mag = std::sqrt(std::norm(hc[j]) + std::norm(hc[j + 1]));
-                tlen[q] = j + 1;
-                if (mag == 0) {
-                    live[q] = 0;
-                    continue;
-                }
-                csq[j] = std::conj(hc[j]) / mag;
-                snq[j] = std::conj(hc[j + 1]) / mag;
-                hc[j] = csq[j] * hc[j] + snq[j] * hc[j + 1];
-                hc[j + 1] = 0;
-                gq[j + 1] = -std::conj(snq[j]) * gq[j];
-                gq[j] = csq[j] * gq[j];
-                if (std::abs(gq[j + 1]) <= tol * beta[q] || hn == 0) live[q] = 0;
-                anylive = anylive || live[q];
+                g_start[q] = std::abs(g[(size_t)q * (K + 1) + j]);
+                steps[q] = 0;
             }
-            if (!anylive) {
-                j++;
-                break;
+            bool anylive = false;
+            for (int t = 0; t < c; t++) {
+                const int jj = j + t;
+                total++;
+                anylive = false;
+                for (int q = 0; q < R; q++) {
+                    if (!live[q]) continue;
+                    const double2 *hq = &hch[(size_t)q * hcs + (size_t)t * hs];
+                    zc *hc = &H[((size_t)q * K + jj) * (K + 1)];
+                    zc *csq = &cs[(size_t)q * K], *snq = &sn[(size_t)q * K], *gq = &g[(size_t)q * (K + 1)];
+                    for (int l = 0; l <= jj; l++) hc[l] = zc(hq[l].x, hq[l].y);
+                    const double hn = std::sqrt(std::max(0.0, hq[K + 1].x));
+                    hc[jj + 1] = hn;
+                    // previous rotations, then a new one (GMRES.cu:58-88 / 44-56)
+                    for (int l = 0; l < jj; l++) {
+                        const zc t0 = csq[l] * hc[l] + snq[l] * hc[l + 1];
+                        const zc t1 = -std::conj(snq[l]) * hc[l] + std::conj(csq[l]) * hc[l + 1];
+                        hc[l] = t0;
+                        hc[l + 1] = t1;
+                    }
+                    const double mag = std::sqrt(std::norm(hc[jj]) + std::norm(hc[jj + 1]));
+                    tlen[q] = jj + 1;
+                    if (mag == 0) {
+                        live[q] = 0;
+                        continue;
+                    }
+                    csq[jj] = std::conj(hc[jj]) / mag;
+                    snq[jj] = std::conj(hc[jj + 1]) / mag;
+                    hc[jj] = csq[jj] * hc[jj] + snq[jj] * hc[jj + 1];
+                    hc[jj + 1] = 0;
+                    gq[jj + 1] = -std::conj(snq[jj]) * gq[jj];
+                    gq[jj] = csq[jj] * gq[jj];
+                    const double gnew = std::abs(gq[jj + 1]);
+                    steps[q]++;
+                    g_end[q] = gnew;
+                    if (gnew <= tol * beta[q] || hn == 0) live[q] = 0;
+                    anylive = anylive || live[q];
+                }
+                if (!anylive) {
+                    // later steps of this chunk ran past convergence
+                    j = jj + 1;
+                    break;
+                }
             }
+            // geometric-mean rate over the chunk; the slowest system sets the next length
+            double rate_obs = 0.0;
+            for (int q = 0; q < R; q++)
+                if (steps[q] > 0 && g_start[q] > 0 && g_end[q] > 0)
+                    rate_obs = std::max(rate_obs, std::pow(g_end[q] / g_start[q], 1.0 / steps[q]));
+            if (rate_obs > 0) rate = rate_obs;
+            if (!anylive) break;
+            j += c;
         }
         // back substitution per system (Rsolver, GMRES.cu:26-42), x_r += V_r y_r
         int tmax = 0;
