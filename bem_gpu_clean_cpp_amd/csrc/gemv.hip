@@ -535,4 +535,165 @@ int ne_apply_f32(const float2 *Ar, long ldr, long M, long N, const float2 *v, do
     return EXIT_SUCCESS;
 }
 
+// r = A^H (b - A x) with FP64 accumulation in one pass over the row-major copy
+// (the FP64 refinement residual of the normal equations; x = null gives
+// A^H b).  Same layout as ne_apply_part, one row at a time.  x is read in
+// FP64 for every row: at 16 bytes a column it stays in L2, and reading it
+// leaves the registers to the row and the FP64 partial of r (x4 is a copy of x
+// padded with zeros to the row length, so every pair load is valid).  a_i . x is
+// formed in FP64 (FP32 entries widened), t_i = b_i - a_i . x, and the
+// partial of r += conj(a_i) t_i in FP64.  The partials are reduced in a fixed
+// order in FP64.
+__device__ __forceinline__ double ne_wsum_d(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+template <int NT, int PPT>
+__global__ __launch_bounds__(NT) void ne_resid_part(const float4 *__restrict__ Ar4, long ldr4, int M, int rows_per_blk,
+                                                    const double4 *__restrict__ x4, const double2 *__restrict__ b,
+                                                    int N, double4 *__restrict__ part4, long ldp4) {
+    constexpr int NW = NT / 64;
+    __shared__ double s_red[2][NW][2];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int r0 = blockIdx.x * rows_per_blk, r1 = min(M, r0 + rows_per_blk);
+    const int npairs = (N + 1) / 2;
+    double4 w[PPT];
+    unsigned off[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        off[k] = (unsigned)min(tid + NT * k, npairs - 1);
+        w[k] = make_double4(0, 0, 0, 0);
+    }
+    for (int i = r0; i < r1; i++) {
+        const float4 *row = Ar4 + (size_t)i * ldr4;
+        float4 a[PPT];
+#pragma unroll
+        for (int k = 0; k < PPT; k++) a[k] = row[off[k]];
+        double dr = 0, di = 0;
+        if (x4) {
+#pragma unroll
+            for (int k = 0; k < PPT; k++) {
+                const int p = tid + NT * k;
+                if (p < npairs) {
+                    const double4 xv = x4[p];
+                    dr = fma((double)a[k].x, xv.x, dr);
+                    dr = fma(-(double)a[k].y, xv.y, dr);
+                    dr = fma((double)a[k].z, xv.z, dr);
+                    dr = fma(-(double)a[k].w, xv.w, dr);
+                    di = fma((double)a[k].x, xv.y, di);
+                    di = fma((double)a[k].y, xv.x, di);
+                    di = fma((double)a[k].z, xv.w, di);
+                    di = fma((double)a[k].w, xv.z, di);
+                }
+            }
+        }
+        dr = ne_wsum_d(dr);
+        di = ne_wsum_d(di);
+        const int par = i & 1;
+        if (lane == 0) {
+            s_red[par][wid][0] = dr;
+            s_red[par][wid][1] = di;
+        }
+        lds_barrier();
+        double tr = 0, ti = 0;
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            tr += s_red[par][q][0];
+            ti += s_red[par][q][1];
+        }
+        const double2 bi = b[i];
+        tr = bi.x - tr;
+        ti = bi.y - ti;
+        // r += conj(a) t
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            w[k].x = fma((double)a[k].x, tr, fma((double)a[k].y, ti, w[k].x));
+            w[k].y = fma((double)a[k].x, ti, fma(-(double)a[k].y, tr, w[k].y));
+            w[k].z = fma((double)a[k].z, tr, fma((double)a[k].w, ti, w[k].z));
+            w[k].w = fma((double)a[k].z, ti, fma(-(double)a[k].w, tr, w[k].w));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const int p = tid + NT * k;
+        if (p < npairs) part4[(size_t)blockIdx.x * ldp4 + p] = w[k];
+    }
+}
+
+// 64 columns per workgroup, 4 groups of the block partials per column; FP64
+__global__ __launch_bounds__(256) void ne_reduce_d(const double2 *__restrict__ part, long ldp, int nblk, int N,
+                                                   double2 *__restrict__ w) {
+    __shared__ double2 s_sum[4][64];
+    const int cl = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + cl;
+    const int cc = min(c, N - 1);
+    double sr = 0, si = 0;
+    int b = g;
+    for (; b + 4 * 7 < nblk; b += 4 * 8) {
+        double2 q[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) q[u] = part[(size_t)(b + 4 * u) * ldp + cc];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            sr += q[u].x;
+            si += q[u].y;
+        }
+    }
+    for (; b < nblk; b += 4) {
+        const double2 q = part[(size_t)b * ldp + cc];
+        sr += q.x;
+        si += q.y;
+    }
+    s_sum[g][cl] = make_double2(sr, si);
+    __syncthreads();
+    if (g == 0 && c < N) {
+        const double2 a0 = s_sum[0][cl], a1 = s_sum[1][cl], a2 = s_sum[2][cl], a3 = s_sum[3][cl];
+        w[c] = make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
+    }
+}
+
+size_t ws_ne_resid(long M, long N) { return 2 * ws_ne_apply(M, N) + sizeof(double2) * (size_t)ne_ldr(N); }
+
+int ne_resid_f32a(const float2 *Ar, long ldr, long M, long N, const double2 *x, const double2 *b, double2 *r, void *ws,
+                  hipStream_t s) {
+    if (!ne_fused_fits(M, N)) BEM_FAIL("fused normal-equation residual: N = %ld is over its limit", N);
+    if (ldr != ne_ldr(N)) BEM_FAIL("fused normal-equation residual: ldr %ld, expected %ld", ldr, ne_ldr(N));
+    int rpb, nblk;
+    ne_shape(M, rpb, nblk);
+    const long npairs = (N + 1) / 2, ldp = ldr;  // part: nblk rows of ldp double2
+    TimedRegion tr;
+    timing_begin(tr, "ne_resid_f32a", s);
+    const float4 *A4 = (const float4 *)Ar;
+    double4 *p4 = (double4 *)ws;
+    const long ldr4 = ldr / 2, ldp4 = ldp / 2;
+    // x padded with zeros to ldr after the partials
+    double2 *xpad = nullptr;
+    if (x) {
+        xpad = (double2 *)((char *)ws + 2 * ws_ne_apply(M, N));
+        HIP_CALL(hipMemcpyAsync(xpad, x, sizeof(double2) * N, hipMemcpyDeviceToDevice, s));
+        if (ldr > N) HIP_CALL(hipMemsetAsync(xpad + N, 0, sizeof(double2) * (ldr - N), s));
+    }
+#define NE_LAUNCH(NT_, PPT_) \
+    ne_resid_part<NT_, PPT_><<<nblk, NT_, 0, s>>>(A4, ldr4, (int)M, rpb, (const double4 *)xpad, b, (int)N, p4, ldp4)
+    // FP64 partials of r take 8 registers a pair: past 2 pairs per thread at
+    // 1024 threads the rows go to 512-thread workgroups (256 registers a lane)
+    if (npairs <= 1024) {
+        NE_LAUNCH(1024, 1);
+    } else if (npairs <= 2048) {
+        NE_LAUNCH(1024, 2);
+    } else if (npairs <= 512 * 8) {
+        NE_LAUNCH(512, 8);
+    } else {
+        NE_LAUNCH(512, 12);
+    }
+#undef NE_LAUNCH
+    HIP_CALL(hipGetLastError());
+    ne_reduce_d<<<(unsigned)((N + 63) / 64), 256, 0, s>>>((const double2 *)ws, ldp, nblk, (int)N, r);
+    HIP_CALL(hipGetLastError());
+    timing_end(tr, s);
+    return EXIT_SUCCESS;
+}
+
 }  // namespace bem

@@ -290,6 +290,10 @@ struct Operator {
     int residual(KrylovWS &ws, const double2 *B, long ldb, const double2 *X, long ldx, double2 *r,
                  hipStream_t s) const {
         const int R = ws.R;
+        if (A32 && R == 1 && normal() && ws.Ar && !getenv("BEM_NE_RESID_TWO_PASS")) {
+            // A^H (b - A x) in one pass over the row-major copy, FP64 accumulation
+            return ne_resid_f32a(ws.Ar, ws.ldr, M, N, X, B, r, ws.nws, s);
+        }
         if (A32) {
             if (R == 1) {
                 if (!normal()) {
@@ -342,7 +346,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     if (A32 && M > N && R == 1 && ne_fused_fits(M, N) && !getenv("BEM_NE_TWO_PASS")) {
         ws.ldr = ne_ldr(N);
         HOST_CALL(salloc(&ws.Ar, (size_t)M * ws.ldr, s));
-        HOST_CALL(salloc(&ws.nws, ws_ne_apply(M, N), s));
+        HOST_CALL(salloc(&ws.nws, std::max(ws_ne_apply(M, N), ws_ne_resid(M, N)), s));
         HOST_CALL(transpose_rm_f32(A32, lda, M, N, ws.Ar, ws.ldr, s));
     }
     Operator op;
