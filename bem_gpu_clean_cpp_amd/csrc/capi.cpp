@@ -1,6 +1,5 @@
 // extern "C" boundary of libbem (declared in include/bem.h).
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -424,6 +423,8 @@ void bem_solver_destroy(bem_solver *s) {
         (void)hipStreamSynchronize(s->stream);
         solver_free_mats(s);
         device_problem_free(s->dp);
+        for (hipEvent_t e : {s->ae0, s->ae1, s->se0, s->se1})
+            if (e) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(s->stream);
     }
     delete s;
@@ -446,6 +447,17 @@ int bem_solver_release(bem_solver *sv) {
     return EXIT_SUCCESS;
 }
 
+// the last assembly's time into the stats, once its events have completed
+static int resolve_assembly_time(const bem_solver *sv) {
+    if (!sv->asm_pending) return EXIT_SUCCESS;
+    HIP_CALL(hipEventSynchronize(sv->ae1));
+    float ms = 0;
+    HIP_CALL(hipEventElapsedTime(&ms, sv->ae0, sv->ae1));
+    sv->stats.assemble_ms = ms;
+    sv->asm_pending = false;
+    return EXIT_SUCCESS;
+}
+
 int bem_solver_assemble(bem_solver *sv, float k, int precision) {
     if (!sv) BEM_FAIL("null solver");
     if (precision != BEM_PREC_FP32 && precision != BEM_PREC_FP64) BEM_FAIL("precision must be 32 or 64");
@@ -453,10 +465,9 @@ int bem_solver_assemble(bem_solver *sv, float k, int precision) {
     hipStream_t s = sv->stream;
     DeviceProblem &dp = sv->dp;
     if (dp.M == 0) BEM_FAIL("solver has no mesh (released)");
-    hipEvent_t e0, e1;
-    HIP_CALL(hipEventCreate(&e0));
-    HIP_CALL(hipEventCreate(&e1));
-    HIP_CALL(hipEventRecord(e0, s));
+    if (!sv->ae0) HIP_CALL(hipEventCreate(&sv->ae0));
+    if (!sv->ae1) HIP_CALL(hipEventCreate(&sv->ae1));
+    HIP_CALL(hipEventRecord(sv->ae0, s));
     const size_t need = (size_t)dp.ldA * dp.N;
     if (precision == BEM_PREC_FP32) {
         if (sv->A64) {
@@ -481,13 +492,9 @@ int bem_solver_assemble(bem_solver *sv, float k, int precision) {
         }
         HOST_CALL(assemble_fp64(dp, (double)k, sv->A64, s));
     }
-    HIP_CALL(hipEventRecord(e1, s));
-    HIP_CALL(hipEventSynchronize(e1));
-    float ms = 0;
-    HIP_CALL(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    sv->stats.assemble_ms = ms;
+    // no wait here: the assembly's time is read from the events later
+    HIP_CALL(hipEventRecord(sv->ae1, s));
+    sv->asm_pending = true;
     sv->k = k;
     sv->prec = precision;
     sv->xr = 0;
@@ -529,7 +536,9 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
         sv->x_cap = (size_t)ld * R;
     }
     sv->xr = 0;
-    const auto t0 = std::chrono::steady_clock::now();
+    if (!sv->se0) HIP_CALL(hipEventCreate(&sv->se0));
+    if (!sv->se1) HIP_CALL(hipEventCreate(&sv->se1));
+    HIP_CALL(hipEventRecord(sv->se0, s));
     HOST_CALL(rhs_fp64(dp, (double)sv->k, sv->B64, ld, sv->rhs, s));
     bem_solve_stats st{};
     st.method = method;
@@ -562,10 +571,15 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
             st.converged = 1;
         }
     }
+    HIP_CALL(hipEventRecord(sv->se1, s));
     HIP_CALL(hipStreamSynchronize(s));
     sv->xr = R;
-    const auto t1 = std::chrono::steady_clock::now();
-    st.solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    // device time from the solve's first work to its last (an assembly still
+    // running when the solve was called is not counted)
+    float sms = 0;
+    HIP_CALL(hipEventElapsedTime(&sms, sv->se0, sv->se1));
+    st.solve_ms = sms;
+    HOST_CALL(resolve_assembly_time(sv));
     st.assemble_ms = sv->stats.assemble_ms;
     sv->stats = st;
     if (stats) *stats = st;
@@ -777,6 +791,8 @@ int bem_solver_dims(const bem_solver *sv, int *M, int *N, int *nrhs) {
 
 int bem_solver_stats(const bem_solver *sv, bem_solve_stats *st) {
     if (!sv || !st) BEM_FAIL("null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    HOST_CALL(resolve_assembly_time(sv));
     *st = sv->stats;
     return EXIT_SUCCESS;
 }

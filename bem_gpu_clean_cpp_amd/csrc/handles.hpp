@@ -26,7 +26,13 @@ struct bem_solver {
     size_t a_cap = 0;  // elements allocated for A32 / A64
     size_t x_cap = 0;  // elements allocated for each of B64, X64, X32
     int xr = 0;        // columns of the last solution in X64
-    bem_solve_stats stats{};
+    mutable bem_solve_stats stats{};
+    // events around the last assembly: its time is read once it has finished
+    // (when the stats are read or a solve synchronises), so assembling does
+    // not wait for the device
+    hipEvent_t ae0 = nullptr, ae1 = nullptr;
+    mutable bool asm_pending = false;
+    hipEvent_t se0 = nullptr, se1 = nullptr;  // around the last solve
     // exterior field as a matrix: the mesh with the field points as extra rows
     bem::Mesh fmesh;
     bem::DeviceProblem fdp;
