@@ -168,14 +168,16 @@ typedef struct bem_solve_stats {
     int converged;        /* 1 when the requested tolerance was reached */
     double residual;      /* final relative FP64 residual of the (normal) equations */
     double assemble_ms;   /* last assembly, device time */
-    double solve_ms;      /* last solve, wall time */
+    double solve_ms;      /* last solve, device time from its first to its last work */
 } bem_solve_stats;
 
 /* Create a solver for the mesh (nodes, elements, CHIEF points are copied to HBM). */
 bem_solver *bem_solver_create(const bem_mesh *m);
 void bem_solver_destroy(bem_solver *s);
 /* Assemble A and the boundary-condition part of B at wavenumber k in FP32
- * (complex64 A) or FP64 (complex128 A). */
+ * (complex64 A) or FP64 (complex128 A).  Returns once the work is queued on
+ * the solver's stream; every later call on the solver is ordered after it,
+ * and assemble_ms is filled in when the stats are read or a solve ends. */
 int bem_solver_assemble(bem_solver *s, float k, int precision);
 /* Right-hand sides: kind BEM_RHS_POINT (params = x,y,z per source, incident
  * q*G(x, src)) or BEM_RHS_PLANE (params = unit direction per wave, incident
