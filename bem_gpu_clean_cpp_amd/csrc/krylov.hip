@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cmath>
 #include <complex>
@@ -382,6 +383,14 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     HOST_CALL(salloc(&ws.hch, (size_t)hcs * R, s));
     std::vector<double2> hch((size_t)hcs * R);
     double rate = 0.0;
+    // Classical Gram-Schmidt, one pass while it is safe: the host sees every
+    // step's products h and the new norm hn, and |w before|^2 = hn^2 + sum |h|^2,
+    // so a step whose w lost more than 99 % of its squared norm to the
+    // projection (where one pass leaves w measurably non-orthogonal) is seen;
+    // from then on every step of the solve takes the second pass (CGS2,
+    // "twice is enough").  BEM_CGS2 forces two passes throughout.
+    bool cgs2 = getenv("BEM_CGS2") != nullptr;
+    constexpr double kCancel = 1e-2;
     std::vector<double> g_start(R), g_end(R);
     std::vector<int> steps(R);
     std::vector<zc> H((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K), g((size_t)R * (K + 1));
@@ -452,9 +461,11 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                 HOST_CALL(bdot(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, h1, hcs, s));
                 kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, jj + 1, h1, hcs, -1.0, ws.w64, ldv, N);
                 HIP_CALL(hipGetLastError());
-                HOST_CALL(bdot(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, h2, ws.hstride, s, h1, hcs));  // h1 += h2
-                kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, jj + 1, h2, ws.hstride, -1.0, ws.w64, ldv, N);
-                HIP_CALL(hipGetLastError());
+                if (cgs2) {
+                    HOST_CALL(bdot(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, h2, ws.hstride, s, h1, hcs));  // h1 += h2
+                    kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, jj + 1, h2, ws.hstride, -1.0, ws.w64, ldv, N);
+                    HIP_CALL(hipGetLastError());
+                }
                 HOST_CALL(bnorm2(ws, ws.w64, ldv, N, h1 + (K + 1), hcs, s));
                 kb_normalize<<<gg, bs, 0, s>>>(ws.w64, ldv, h1 + (K + 1), hcs, ws.V + (size_t)(jj + 1) * ldv,
                                                ws.vstride, op.A32 ? ws.v32 : nullptr, ldv, N);
@@ -476,9 +487,20 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                     const double2 *hq = &hch[(size_t)q * hcs + (size_t)t * hs];
                     zc *hc = &H[((size_t)q * K + jj) * (K + 1)];
                     zc *csq = &cs[(size_t)q * K], *snq = &sn[(size_t)q * K], *gq = &g[(size_t)q * (K + 1)];
-                    for (int l = 0; l <= jj; l++) hc[l] = zc(hq[l].x, hq[l].y);
-                    const double hn = std::sqrt(std::max(0.0, hq[K + 1].x));
+                    double proj2 = 0;
+                    for (int l = 0; l <= jj; l++) {
+                        hc[l] = zc(hq[l].x, hq[l].y);
+                        proj2 += std::norm(hc[l]);
+                    }
+                    const double hn2 = std::max(0.0, hq[K + 1].x);
+                    const double hn = std::sqrt(hn2);
                     hc[jj + 1] = hn;
+                    if (!cgs2 && hn2 < kCancel * (hn2 + proj2)) {
+                        cgs2 = true;
+                        if (getenv("BEM_TRACE_SOLVE"))
+                            fprintf(stderr, "gmres: step %d, system %d kept %.2e of |w|^2: second Gram-Schmidt pass from here\n",
+                                    jj, q, hn2 / (hn2 + proj2));
+                    }
                     // previous rotations, then a new one (GMRES.cu:58-88 / 44-56)
                     for (int l = 0; l < jj; l++) {
                         const zc t0 = csq[l] * hc[l] + snq[l] * hc[l + 1];
