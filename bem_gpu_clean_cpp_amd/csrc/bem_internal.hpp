@@ -192,10 +192,13 @@ size_t ws_gemv_n(long M, long N, int R);
 bool ne_fused_fits(long M, long N);
 long ne_ldr(long N);
 size_t ws_ne_apply(long M, long N);
-int transpose_rm_f32(const float2 *A, long lda, long M, long N, float2 *Ar, long ldr, hipStream_t s);
 int ne_apply_f32(const float2 *Ar, long ldr, long M, long N, const float2 *v, double2 *w, void *ws, hipStream_t s);
 // r = A^H (b - A x) (x = null: A^H b) with FP64 accumulation, one pass over the row-major copy
 size_t ws_ne_resid(long M, long N);
+// the row-major copy and r = A^H b (FP64) in the same pass over A
+size_t ws_ne_rhs(long M, long N);
+int transpose_rm_ahb_f32(const float2 *A, long lda, long M, long N, float2 *Ar, long ldr, const double2 *b,
+                         double2 *r, void *ws, hipStream_t s);
 int ne_resid_f32a(const float2 *Ar, long ldr, long M, long N, const double2 *x, const double2 *b, double2 *r, void *ws,
                   hipStream_t s);
 int gemv_n_f32(const float2 *A, long lda, long M, long N, const float2 *x, long ldx, float2 *y, long ldy, int R,

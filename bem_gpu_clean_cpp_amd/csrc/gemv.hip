@@ -299,8 +299,9 @@ int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 // all of A v before the first dot of A^H, i.e. two passes over A (the pair of
 // GEMVs above).  The row-major copy costs one transpose per matrix.
 //
-//   transpose_rm_f32  Ar[i * ldr + j] = A[j * lda + i] for j < N, zero for
-//                     N <= j < ldr (the padding is read by the pair loads);
+//   transpose_rm_ahb_f32  Ar[i * ldr + j] = A[j * lda + i] for j < N, zero
+//                     for N <= j < ldr (the padding is read by the pair
+//                     loads), and A^H b from the same pass;
 //   ne_apply_part     kNeBlocks workgroups of NT = 1024 (768 for rows of more
 //                     than 4096 columns: 12 waves leave 168 registers a lane)
 //                     threads, each over a contiguous range of rows.  Thread
@@ -314,19 +315,42 @@ int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 constexpr int kNeBlocks = 256;
 constexpr int kNeMaxPairs = 8;  // column pairs per thread at 768 threads: N <= 12288
 
-__global__ __launch_bounds__(256) void k_transpose_rm(const float2 *__restrict__ A, long lda, int M, int N,
-                                                      float2 *__restrict__ Ar, long ldr) {
+// The row-major copy in 64 x 64 LDS tiles (Ar[i * ldr + j] = A[j * lda + i],
+// zero for N <= j < ldr, the padding the pair loads read), also forming the
+// partial of A^H b over the tile's 64 rows for its 64 columns (FP64): P[blockIdx.x * ldr + j] = sum over the
+// tile's rows i of conj(A[i, j]) b[i].  Thread (tx, ty) takes column j0 + tx
+// and the rows ty, ty + 4, ... of the tile; the four row groups are added in
+// a fixed order.  The pass over A that the first residual of the normal
+// equations (A^H b) needs is the one the copy makes anyway.
+__global__ __launch_bounds__(256) void k_transpose_rm_ahb(const float2 *__restrict__ A, long lda, int M, int N,
+                                                          float2 *__restrict__ Ar, long ldr,
+                                                          const double2 *__restrict__ b,
+                                                          double2 *__restrict__ P) {
     __shared__ float2 tile[64][65];
+    __shared__ double2 s_b[64];
+    __shared__ double2 s_p[4][64];
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for (int jj = ty; jj < 64; jj += 4) {
         const int i = i0 + tx, j = j0 + jj;
         tile[jj][tx] = (i < M && j < N) ? A[(size_t)j * lda + i] : make_float2(0.f, 0.f);
     }
+    if (ty == 0) s_b[tx] = i0 + tx < M ? b[i0 + tx] : make_double2(0, 0);
     __syncthreads();
+    double pr = 0, pi = 0;
     for (int ii = ty; ii < 64; ii += 4) {
         const int i = i0 + ii, j = j0 + tx;
-        if (i < M && j < ldr) Ar[(size_t)i * ldr + j] = tile[tx][ii];
+        const float2 a = tile[tx][ii];
+        if (i < M && j < ldr) Ar[(size_t)i * ldr + j] = a;
+        const double2 bb = s_b[ii];
+        pr = fma((double)a.x, bb.x, fma((double)a.y, bb.y, pr));
+        pi = fma((double)a.x, bb.y, fma(-(double)a.y, bb.x, pi));
+    }
+    s_p[ty][tx] = make_double2(pr, pi);
+    __syncthreads();
+    if (ty == 0) {
+        const double2 p0 = s_p[0][tx], p1 = s_p[1][tx], p2 = s_p[2][tx], p3 = s_p[3][tx];
+        P[(size_t)blockIdx.x * ldr + j0 + tx] = make_double2((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
     }
 }
 
@@ -488,13 +512,6 @@ size_t ws_ne_apply(long M, long N) {
     return sizeof(float2) * (size_t)nblk * (size_t)ne_ldr(N);
 }
 
-int transpose_rm_f32(const float2 *A, long lda, long M, long N, float2 *Ar, long ldr, hipStream_t s) {
-    if (ldr < N || ldr % 64) BEM_FAIL("row-major copy: ldr %ld must be a multiple of 64 and >= N = %ld", ldr, N);
-    dim3 g((unsigned)((M + 63) / 64), (unsigned)(ldr / 64));
-    k_transpose_rm<<<g, 256, 0, s>>>(A, lda, (int)M, (int)N, Ar, ldr);
-    HIP_CALL(hipGetLastError());
-    return EXIT_SUCCESS;
-}
 
 int ne_apply_f32(const float2 *Ar, long ldr, long M, long N, const float2 *v, double2 *w, void *ws, hipStream_t s) {
     if (!ne_fused_fits(M, N)) BEM_FAIL("fused normal-equation product: N = %ld is over its limit", N);
@@ -655,6 +672,21 @@ __global__ __launch_bounds__(256) void ne_reduce_d(const double2 *__restrict__ p
 }
 
 size_t ws_ne_resid(long M, long N) { return 2 * ws_ne_apply(M, N) + sizeof(double2) * (size_t)ne_ldr(N); }
+size_t ws_ne_rhs(long M, long N) { return sizeof(double2) * (size_t)((M + 63) / 64) * (size_t)ne_ldr(N); }
+
+// Ar = the row-major copy of A and r = A^H b (FP64) in one pass over A
+int transpose_rm_ahb_f32(const float2 *A, long lda, long M, long N, float2 *Ar, long ldr, const double2 *b,
+                         double2 *r, void *ws, hipStream_t s) {
+    if (ldr != ne_ldr(N)) BEM_FAIL("row-major copy: ldr %ld, expected %ld", ldr, ne_ldr(N));
+    const int nbx = (int)((M + 63) / 64);
+    dim3 g((unsigned)nbx, (unsigned)(ldr / 64));
+    double2 *P = (double2 *)ws;
+    k_transpose_rm_ahb<<<g, 256, 0, s>>>(A, lda, (int)M, (int)N, Ar, ldr, b, P);
+    HIP_CALL(hipGetLastError());
+    ne_reduce_d<<<(unsigned)((N + 63) / 64), 256, 0, s>>>(P, ldr, nbx, (int)N, r);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
 
 int ne_resid_f32a(const float2 *Ar, long ldr, long M, long N, const double2 *x, const double2 *b, double2 *r, void *ws,
                   hipStream_t s) {

@@ -347,8 +347,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     if (A32 && M > N && R == 1 && ne_fused_fits(M, N) && !getenv("BEM_NE_TWO_PASS")) {
         ws.ldr = ne_ldr(N);
         HOST_CALL(salloc(&ws.Ar, (size_t)M * ws.ldr, s));
-        HOST_CALL(salloc(&ws.nws, std::max(ws_ne_apply(M, N), ws_ne_resid(M, N)), s));
-        HOST_CALL(transpose_rm_f32(A32, lda, M, N, ws.Ar, ws.ldr, s));
+        HOST_CALL(salloc(&ws.nws, std::max({ws_ne_apply(M, N), ws_ne_resid(M, N), ws_ne_rhs(M, N)}), s));
     }
     Operator op;
     op.A32 = A32;
@@ -358,8 +357,10 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     op.N = N;
     const int bs = 256;
     const long ldv = ws.ldv;
-    // ||rhs|| per system: the residual at x = 0
-    HOST_CALL(op.rhs(ws, B, ldb, ws.r, s));
+    // ||rhs|| per system: the residual at x = 0.  With the row-major copy, the
+    // copy and A^H b come from one pass over A.
+    if (ws.Ar) HOST_CALL(transpose_rm_ahb_f32(A32, lda, M, N, ws.Ar, ws.ldr, B, ws.r, ws.nws, s));
+    else HOST_CALL(op.rhs(ws, B, ldb, ws.r, s));
     std::vector<double2> hbuf((size_t)ws.hstride * R);
     auto norms = [&](const double2 *w, std::vector<double> &out) -> int {
         HOST_CALL(bnorm2(ws, w, ldv, N, ws.h, ws.hstride, s));
