@@ -1257,10 +1257,11 @@ template <typename T>
 static int qr_solve_factored(const QRFactor<T> &f, typename QT<T>::c2 *B, long ldb, int R, hipStream_t s) {
     using C2 = typename QT<T>::c2;
     if (R <= 0) return EXIT_SUCCESS;
-    // small systems with few right-hand sides: the whole solve in one
-    // workgroup per right-hand side, b in shared memory
+    // the whole solve in one workgroup per right-hand side, b in shared memory
+    // (on request; the row-split launches below measured faster at every size
+    // from 129 to 4096 rows)
     const size_t lds = sizeof(C2) * (size_t)f.M;
-    if (lds <= kSmallSolveLds && R <= kSmallSolveRhs && !getenv("BEM_QR_NO_FUSED_SOLVE")) {
+    if (lds <= kSmallSolveLds && R <= kSmallSolveRhs && getenv("BEM_QR_ONE_WG_SOLVE")) {
         // 2 kNB partial sums per thread plus a chunk of 8 loads: the register
         // room of 512 (FP32) or 256 (FP64) threads
         constexpr int nt = sizeof(T) == 4 ? 512 : 256;
@@ -1274,8 +1275,8 @@ static int qr_solve_factored(const QRFactor<T> &f, typename QT<T>::c2 *B, long l
         HIP_CALL(hipGetLastError());
         return EXIT_SUCCESS;
     }
-    // larger systems: one launch per panel for Q^H B and one per block column
-    // for the back substitution, the rows spread over workgroups
+    // one launch per panel for Q^H B and one per block column for the back
+    // substitution, the rows spread over workgroups
     constexpr int nt = 256;
     const int np = (f.N + kNB - 1) / kNB;
     const int nblk = (f.M + nt - 1) / nt;

@@ -177,16 +177,16 @@ def test_qr_panel_variants_match_lapack(monkeypatch):
     assert np.linalg.norm(Xg - Xo) / np.linalg.norm(Xo) < 2e-5
     assert np.linalg.norm(X - Xg) / np.linalg.norm(Xo) < 2e-5
     # the one-workgroup solve (Q^H b and back substitution) against the
-    # per-panel launches
-    monkeypatch.setenv("BEM_QR_NO_FUSED_SOLVE", "1")
+    # row-split launches
+    monkeypatch.setenv("BEM_QR_ONE_WG_SOLVE", "1")
     Xn = bem.qr_solve(A, B)
     assert np.linalg.norm(Xn - Xo) / np.linalg.norm(Xo) < 2e-5
     assert np.linalg.norm(X - Xn) / np.linalg.norm(Xo) < 2e-5
 
 
-def test_qr_fp64_fused_solve_matches_launch_per_panel(monkeypatch):
+def test_qr_fp64_one_workgroup_solve_matches_row_split(monkeypatch):
     """FP64 QR of a CHIEF system with a ragged last panel (N = 162): the
-    one-workgroup solve against the per-panel launches, to FP64 accuracy."""
+    one-workgroup solve against the row-split launches, to FP64 accuracy."""
     m = _icosphere(2, nchief=6)
     k = 17.0
     s = bem.Solver(m)
@@ -194,7 +194,7 @@ def test_qr_fp64_fused_solve_matches_launch_per_panel(monkeypatch):
     s.set_rhs(bem.RHS_PLANE, [[0, 0, 1], [0.6, 0.8, 0], [0, 1, 0]], 1.0)
     s.solve(bem.SOLVE_QR)
     x = s.solution()
-    monkeypatch.setenv("BEM_QR_NO_FUSED_SOLVE", "1")
+    monkeypatch.setenv("BEM_QR_ONE_WG_SOLVE", "1")
     s.solve(bem.SOLVE_QR)
     xn = s.solution()
     assert np.linalg.norm(x - xn) / np.linalg.norm(xn) < 1e-12
