@@ -313,10 +313,9 @@ __device__ __forceinline__ void wave_reduce_scatter64<float>(float (&x)[2 * kNB]
 // waves by 2 kNB threads -> s_fin).  For c > jj that is v^H a_c, which with
 // tau updates a_c.  For c < jj the column holds v_c, exact on every row where
 // v is nonzero (r >= j > j0 + c), so the sum is conj(v_c^H v); those give
-// column jj of T by the forward recurrence (LAPACK larft; Schreiber & Van
-// Loan 1989)
-//   T[l][jj] = -tau_jj sum_{q=l}^{jj-1} T[l][q] (v_q^H v_jj),  T[jj][jj] = tau_jj
-// with H_1 ... H_kNB = I - V T V^H.  Each shared array is written only after a
+// column jj of U = diag(1/tau) + the strictly upper part of V^H V, and after
+// the last column T = U^-1 (H_1 ... H_kNB = I - V T V^H; Schreiber & Van Loan
+// 1989, Joffrain et al. 2006).  Each shared array is written only after a
 // barrier that follows its last read, so three barriers per column suffice.
 //
 // Writes the Householder vectors and R back in place, T to Tout (kNB x kNB,
@@ -329,10 +328,10 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
     __shared__ float s_nrm[NW];
     __shared__ float s_red[NW][2 * kNB];
     __shared__ float2 s_fin[kNB];
-    __shared__ float2 s_T[kNB][kNB];
+    __shared__ float2 s_U[kNB][kNB];  // strictly upper part of V^H V (row q, column m)
+    __shared__ float s_tau[kNB];
     __shared__ float s_bc[2];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    for (int p = tid; p < kNB * kNB; p += NT) s_T[p / kNB][p % kNB] = make_float2(0.f, 0.f);
     float2 a[RPT][kNB];
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
@@ -425,17 +424,11 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
                 }
                 if ((c & 3) == 3) __builtin_amdgcn_sched_barrier(0);
             }
-            // column jj of T (v_q^H v = conj(s_fin[q]))
+            // column jj of U: v_q^H v = conj(s_fin[q]) for q < jj
             if (tid < jj) {
-                float tr = 0.f, ti = 0.f;
-                for (int q = tid; q < jj; q++) {
-                    const float2 t = s_T[tid][q], y = make_float2(s_fin[q].x, -s_fin[q].y);
-                    tr += t.x * y.x - t.y * y.y;
-                    ti += t.x * y.y + t.y * y.x;
-                }
-                s_T[tid][jj] = make_float2(-tau * tr, -tau * ti);
+                s_U[tid][jj] = make_float2(s_fin[tid].x, -s_fin[tid].y);
             } else if (tid == jj) {
-                s_T[jj][jj] = make_float2(tau, 0.f);
+                s_tau[jj] = tau;
             }
         }
     }
@@ -452,7 +445,38 @@ __global__ __launch_bounds__(NT) void qr_panel_reg(float2 *A, long lda, int M, i
         }
     }
     __syncthreads();
-    for (int p = tid; p < kNB * kNB; p += NT) Tout[p] = s_T[p / kNB][p % kNB];
+    // T = U^-1 with U = diag(1/tau) + the strictly upper part of V^H V (the
+    // inverse of larft's T; Joffrain et al. 2006).  Lane j of wave 0 back-
+    // substitutes column j: t_j = tau_j, t_i = -tau_i sum_{k=i+1..j} U[i][k] t_k,
+    // written with tau_i rather than 1 / U[i][i], so a tau of zero gives the zero
+    // row and column larft gives.  The per-column recurrence of larft inside the
+    // column loop was a serial chain of up to 31 dependent shared-memory reads
+    // per column; this is one triangular solve per panel.
+    // t_k is zero above row j until written, so the sums need no bound on k,
+    // the loads of a row of U need no condition, and only t_j is kept by a
+    // select
+    if (tid < kNB) {
+        const int j = tid;
+        float2 t[kNB];
+#pragma unroll
+        for (int i = 0; i < kNB; i++) t[i] = make_float2(i == j ? s_tau[i] : 0.f, 0.f);
+#pragma unroll
+        for (int i = kNB - 2; i >= 0; i--) {
+            float2 u[kNB];
+#pragma unroll
+            for (int k = i + 1; k < kNB; k++) u[k] = s_U[i][k];
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int k = i + 1; k < kNB; k++) {
+                sr += u[k].x * t[k].x - u[k].y * t[k].y;
+                si += u[k].x * t[k].y + u[k].y * t[k].x;
+            }
+            const float ti = s_tau[i];
+            if (i != j) t[i] = make_float2(-ti * sr, -ti * si);
+        }
+#pragma unroll
+        for (int i = 0; i < kNB; i++) Tout[i * kNB + j] = t[i];
+    }
 }
 
 // Tall FP32 panels (more rows than the register panel holds): one launch per
