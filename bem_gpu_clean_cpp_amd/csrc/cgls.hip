@@ -22,8 +22,9 @@
 //
 // Accuracy comes from FP64 residual refinement, as for GMRES: each cycle
 // computes r = b - A x and s = A^H r with FP64 accumulation, runs CG on the
-// correction until |s| fell by `tol`, and adds it to x; the solve stops when
-// every system's |A^H (b - A x)| <= outer_tol |A^H b|.
+// correction until |s| fell by `tol` (or by only as much as reaching
+// 0.1 outer_tol takes, when that is less), and adds it to x; the solve stops
+// when every system's |A^H (b - A x)| <= outer_tol |A^H b|.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,10 +43,10 @@ int resid_n_f64(const double2 *A, long lda, long M, long N, const double2 *x, lo
                 double2 *r, long ldr, int R, void *ws, hipStream_t s);
 
 // out[r] = sum_i |v_r[i]|^2 (FP64, fixed-order tree); with act: the system
-// stays active while out[r] > tol2 * g0[r]
+// stays active while out[r] > thr[r]
 template <typename C2>
 __global__ __launch_bounds__(1024) void kc_norm2(const C2 *__restrict__ v, long ld, int n, double *__restrict__ out,
-                                                 const double *__restrict__ g0, int *__restrict__ act, double tol2) {
+                                                 const double *__restrict__ thr, int *__restrict__ act) {
     __shared__ double red[1024];
     const int r = blockIdx.x;
     const C2 *vr = v + (size_t)r * ld;
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(1024) void kc_norm2(const C2 *__restrict__ v, long 
     }
     if (threadIdx.x == 0) {
         out[r] = red[0];
-        if (act) act[r] = act[r] && red[0] > tol2 * g0[r];
+        if (act) act[r] = act[r] && red[0] > thr[r];
     }
 }
 
@@ -146,13 +147,13 @@ struct CglsWS {
     hipStream_t st = nullptr;
     double2 *rr = nullptr, *s64 = nullptr, *d = nullptr, *p = nullptr, *q64 = nullptr, *w64 = nullptr;
     float2 *p32 = nullptr, *q32 = nullptr, *w32 = nullptr;
-    double *gam = nullptr, *del = nullptr, *g0 = nullptr;
+    double *gam = nullptr, *del = nullptr, *g0 = nullptr, *thr = nullptr;
     int *act = nullptr, *done = nullptr;
     char *gws = nullptr;
     ~CglsWS() {
         sfree(rr, st); sfree(s64, st); sfree(d, st); sfree(p, st); sfree(q64, st); sfree(w64, st);
         sfree(p32, st); sfree(q32, st); sfree(w32, st);
-        sfree(gam, st); sfree(del, st); sfree(g0, st); sfree(act, st); sfree(done, st); sfree(gws, st);
+        sfree(gam, st); sfree(del, st); sfree(g0, st); sfree(thr, st); sfree(act, st); sfree(done, st); sfree(gws, st);
     }
 };
 
@@ -184,6 +185,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     HOST_CALL(salloc(&w.gam, 2 * (size_t)R, s));
     HOST_CALL(salloc(&w.del, R, s));
     HOST_CALL(salloc(&w.g0, R, s));
+    HOST_CALL(salloc(&w.thr, R, s));
     HOST_CALL(salloc(&w.act, R, s));
     HOST_CALL(salloc(&w.done, R, s));
     HOST_CALL(salloc(&w.gws, std::max(std::max(ws_gemv_n(M, N, 1), ws_gemm(M, N, R)), ws_mres(M, N, R)), s));
@@ -213,7 +215,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     };
     std::vector<double> hg(R);
     auto norms = [&](std::vector<double> &out) -> int {
-        kc_norm2<double2><<<R, 1024, 0, s>>>(w.s64, ld, N, w.g0, nullptr, nullptr, 0.0);
+        kc_norm2<double2><<<R, 1024, 0, s>>>(w.s64, ld, N, w.g0, nullptr, nullptr);
         HIP_CALL(hipGetLastError());
         HIP_CALL(hipMemcpyAsync(hg.data(), w.g0, sizeof(double) * R, hipMemcpyDeviceToHost, s));
         HIP_CALL(hipStreamSynchronize(s));
@@ -244,20 +246,19 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         return EXIT_SUCCESS;
     };
 
-    const double tol2 = tol * tol;
     // n CG steps (n even, so every chunk starts on gamma slot 0)
     auto chunk = [&](int n) -> int {
         for (int it = 0; it < n; it++) {
             const int cur = it & 1;
             double *gc = w.gam + (size_t)cur * R, *gn = w.gam + (size_t)(1 - cur) * R;
             HOST_CALL(prod_n());
-            if (f32) kc_norm2<float2><<<R, 1024, 0, s>>>(w.q32, ld, M, w.del, nullptr, nullptr, 0.0);
-            else kc_norm2<double2><<<R, 1024, 0, s>>>(w.q64, ld, M, w.del, nullptr, nullptr, 0.0);
+            if (f32) kc_norm2<float2><<<R, 1024, 0, s>>>(w.q32, ld, M, w.del, nullptr, nullptr);
+            else kc_norm2<double2><<<R, 1024, 0, s>>>(w.q64, ld, M, w.del, nullptr, nullptr);
             HIP_CALL(hipGetLastError());
             HOST_CALL(prod_h());
             if (f32) kc_upd_xs<float2><<<gN, bs, 0, s>>>(N, gc, w.del, w.act, w.p, w.d, w.w32, w.s64, ld);
             else kc_upd_xs<double2><<<gN, bs, 0, s>>>(N, gc, w.del, w.act, w.p, w.d, w.w64, w.s64, ld);
-            kc_norm2<double2><<<R, 1024, 0, s>>>(w.s64, ld, N, gn, w.g0, w.act, tol2);
+            kc_norm2<double2><<<R, 1024, 0, s>>>(w.s64, ld, N, gn, w.thr, w.act);
             kc_upd_p<double2><<<gN, bs, 0, s>>>(N, gn, gc, w.act, w.s64, w.p, f32 ? w.p32 : nullptr, ld);
             HIP_CALL(hipGetLastError());
         }
@@ -277,6 +278,12 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
 
     std::vector<double> nb(R), beta(R);
     std::vector<int> done(R, 0);
+    // Tolerance of a cycle relative to its start: `tol`, or looser when the
+    // cycle starts within a factor 1/tol of outer_tol (margin 0.1, so a
+    // further cycle is unlikely); the threshold on |s|^2 goes to the device
+    constexpr double kCycleMargin = 0.1;
+    const bool fixed_ctol = getenv("BEM_FIXED_CYCLE_TOL") != nullptr;
+    std::vector<double> hthr(R);
     HIP_CALL(hipMemset2DAsync(X, sizeof(double2) * ldx, 0, sizeof(double2) * N, R, s));
     HOST_CALL(outer_residual(true));  // from X = 0
     HOST_CALL(norms(nb));
@@ -299,9 +306,14 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         for (int q = 0; q < R; q++) {
             if (nb[q] > 0 && beta[q] <= outer_tol * nb[q]) done[q] = 1;
             any = any || !done[q];
+            double ctol = tol;
+            if (outer_tol > 0 && nb[q] > 0 && beta[q] > 0 && !fixed_ctol)
+                ctol = std::max(tol, std::min(0.5, kCycleMargin * outer_tol * nb[q] / beta[q]));
+            hthr[q] = ctol * ctol * hg[q];
         }
         if (!any || total >= maxit) break;
         cycles++;
+        HIP_CALL(hipMemcpyAsync(w.thr, hthr.data(), sizeof(double) * R, hipMemcpyHostToDevice, s));
         // inner CG on the correction: p = s = s64, gamma = g0 = |s|^2 (still in w.g0)
         HIP_CALL(hipMemcpyAsync(w.done, done.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
         HIP_CALL(hipMemcpyAsync(w.gam, w.g0, sizeof(double) * R, hipMemcpyDeviceToDevice, s));

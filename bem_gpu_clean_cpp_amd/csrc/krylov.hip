@@ -333,7 +333,8 @@ struct Operator {
 // applied to all of them at once (one MFMA product per Arnoldi step when
 // R > 1).  Each cycle starts from the true FP64 residual and runs up to
 // `restart` steps; a system leaves the cycle when its estimated residual fell by
-// `tol` relative to the cycle's start, and the solve stops when every system's
+// `tol` relative to the cycle's start (or by only as much as reaching
+// 0.1 outer_tol takes, when that is less), and the solve stops when every system's
 // FP64 residual is below `outer_tol` relative to ||rhs|| (or maxit / max_cycles).
 // B: M x R (ldb), X: N x R (ldx, initial guess on entry).
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
@@ -370,6 +371,14 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
         return EXIT_SUCCESS;
     };
     std::vector<double> nb(R), beta(R), relres(R, 0.0);
+    // Tolerance of a cycle relative to its start: `tol`, or looser when the
+    // cycle starts within a factor 1/tol of outer_tol; the margin 0.1 makes a
+    // further cycle unlikely (the FP32 operator's error on the correction is
+    // far below it)
+    constexpr double kCycleMargin = 0.1;
+    std::vector<double> ctol(R, tol);
+    const bool fixed_ctol = getenv("BEM_FIXED_CYCLE_TOL") != nullptr;
+    const bool trace = getenv("BEM_TRACE_SOLVE") != nullptr;
     HOST_CALL(norms(ws.r, nb));
     std::vector<char> done(R, 0);
     for (int q = 0; q < R; q++)
@@ -409,12 +418,20 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             res_current = true;
         }
         bool any = false;
+        double worst = 0.0;
         for (int q = 0; q < R; q++) {
             if (nb[q] == 0) continue;
             relres[q] = beta[q] / nb[q];
             if (relres[q] <= outer_tol) done[q] = 1;
             any = any || !done[q];
+            // a cycle that starts near outer_tol only has to close the gap
+            ctol[q] = tol;
+            if (outer_tol > 0 && relres[q] > 0 && !fixed_ctol)
+                ctol[q] = std::max(tol, std::min(0.5, kCycleMargin * outer_tol / relres[q]));
+            worst = std::max(worst, relres[q]);
         }
+        if (trace)
+            fprintf(stderr, "gmres: cycle %d, worst relative residual %.3e, %d steps so far\n", cycles, worst, total);
         if (!any || total >= maxit) break;
         cycles++;
         // V_r[:, 0] = r_r / beta_r (norms still in ws.h[.., 0])
@@ -447,8 +464,8 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                 for (int q = 0; q < R; q++) {
                     if (!live[q]) continue;
                     const double gj = std::abs(g[(size_t)q * (K + 1) + j]);
-                    if (gj <= tol * beta[q]) continue;
-                    need = std::max(need, (int)std::ceil(std::log(tol * beta[q] / gj) / std::log(rate)));
+                    if (gj <= ctol[q] * beta[q]) continue;
+                    need = std::max(need, (int)std::ceil(std::log(ctol[q] * beta[q] / gj) / std::log(rate)));
                 }
                 c = std::max(1, std::min(c, need));
             } else if (j == 0 && cycles == 1) {
@@ -498,7 +515,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                     hc[jj + 1] = hn;
                     if (!cgs2 && hn2 < kCancel * (hn2 + proj2)) {
                         cgs2 = true;
-                        if (getenv("BEM_TRACE_SOLVE"))
+                        if (trace)
                             fprintf(stderr, "gmres: step %d, system %d kept %.2e of |w|^2: second Gram-Schmidt pass from here\n",
                                     jj, q, hn2 / (hn2 + proj2));
                     }
@@ -524,7 +541,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                     const double gnew = std::abs(gq[jj + 1]);
                     steps[q]++;
                     g_end[q] = gnew;
-                    if (gnew <= tol * beta[q] || hn == 0) live[q] = 0;
+                    if (gnew <= ctol[q] * beta[q] || hn == 0) live[q] = 0;
                     anylive = anylive || live[q];
                 }
                 if (!anylive) {

@@ -186,7 +186,8 @@ int bem_solver_set_rhs(bem_solver *s, int kind, const float *params, int nrhs, f
 /* Solve for all right-hand sides.  method BEM_SOLVE_GMRES: restarted GMRES
  * (restart, tol relative) on the square system, or on the normal equations
  * A^H A x = A^H b when CHIEF rows make it overdetermined; the FP32 path is
- * wrapped in FP64 residual refinement until refine_tol (<= 0: one pass).
+ * wrapped in FP64 residual refinement until refine_tol (<= 0: one pass; a
+ * refinement cycle stops at 0.1 refine_tol when that is looser than tol).
  * method BEM_SOLVE_QR: blocked Householder least squares (FP32 path also
  * refined in FP64).  method BEM_SOLVE_CGLS: conjugate gradients on the
  * normal equations (restart unused), for the overdetermined CHIEF systems,
