@@ -238,10 +238,18 @@ int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ld
 int mh_f64(const double2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
            hipStream_t s);
 
-// X holds the initial guess (x_zero: known to be zero, which saves one residual)
+// GMRES workspace (Krylov basis, row-major copy of A, partials) kept by a
+// caller between solves of the same shape: allocating and freeing the
+// row-major copy costs about 1 ms of host time per solve at N = 10242
+struct GmresCache;
+void gmres_cache_free(GmresCache *&c);
+// X holds the initial guess (x_zero: known to be zero, which saves one residual).
+// cache: null to allocate the workspace for this call only; else *cache is
+// reused when its shape matches and replaced when it does not (free it with
+// gmres_cache_free after the stream's work on it has been ordered)
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
-                 int max_cycles, bem_solve_stats *st, hipStream_t s);
+                 int max_cycles, bem_solve_stats *st, hipStream_t s, GmresCache **cache = nullptr);
 // CGLS (conjugate gradients on A^H A x = A^H b) for R systems in lockstep,
 // FP64 residual refinement; X is overwritten (the solve starts from zero)
 int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,

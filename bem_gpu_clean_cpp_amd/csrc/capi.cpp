@@ -409,6 +409,7 @@ static void solver_free_mats(bem_solver *s) {
     sfree(s->X32, s->stream);
     sfree(s->F, s->stream);
     sfree(s->fbterm, s->stream);
+    gmres_cache_free(s->gcache);
     device_problem_free(s->fdp);
     s->a_cap = s->x_cap = s->f_cap = 0;
     s->fP = 0;
@@ -554,8 +555,7 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
         const int cycles = max_refine > 0 ? max_refine : (1 << 20);
         if (maxit <= 0) maxit = 100000;
         HOST_CALL(gmres_device(sv->A32, sv->A64, ld, dp.M, dp.N, sv->B64, ld, sv->X64, ld, true, R, restart,
-                               (double)tol,
-                               maxit, outer, cycles, &st, s));
+                               (double)tol, maxit, outer, cycles, &st, s, &sv->gcache));
     } else {
         if (sv->A32) {
             HOST_CALL(qr_refined_f32(sv->A32, ld, dp.M, dp.N, sv->B64, ld, sv->X64, ld, R,

@@ -33,6 +33,7 @@ struct bem_solver {
     hipEvent_t ae0 = nullptr, ae1 = nullptr;
     mutable bool asm_pending = false;
     hipEvent_t se0 = nullptr, se1 = nullptr;  // around the last solve
+    bem::GmresCache *gcache = nullptr;         // GMRES workspace of the last solve's shape
     // exterior field as a matrix: the mesh with the field points as extra rows
     bem::Mesh fmesh;
     bem::DeviceProblem fdp;

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -337,19 +338,54 @@ struct Operator {
 // 0.1 outer_tol takes, when that is less), and the solve stops when every system's
 // FP64 residual is below `outer_tol` relative to ||rhs|| (or maxit / max_cycles).
 // B: M x R (ldb), X: N x R (ldx, initial guess on entry).
+struct GmresCache {
+    KrylovWS ws;
+    bool ne = false;
+    bool matches(int N, int M, int R, int restart, bool ne_, hipStream_t s) const {
+        return ws.n == N && ws.m == M && ws.R == R && ws.restart == restart && ne == ne_ && ws.st == s;
+    }
+};
+
+void gmres_cache_free(GmresCache *&c) {
+    delete c;
+    c = nullptr;
+}
+
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
-                 int max_cycles, bem_solve_stats *st, hipStream_t s) {
+                 int max_cycles, bem_solve_stats *st, hipStream_t s, GmresCache **cache) {
     if (M < N) BEM_FAIL("GMRES needs M >= N (got %d x %d)", M, N);
     if (R <= 0) return EXIT_SUCCESS;
     restart = std::max(1, std::min(restart, N));
-    KrylovWS ws;
-    HOST_CALL(ws.init(N, M, R, restart, s));
-    if (A32 && M > N && R == 1 && ne_fused_fits(M, N) && !getenv("BEM_NE_TWO_PASS")) {
-        ws.ldr = ne_ldr(N);
-        HOST_CALL(salloc(&ws.Ar, (size_t)M * ws.ldr, s));
-        HOST_CALL(salloc(&ws.nws, std::max({ws_ne_apply(M, N), ws_ne_resid(M, N), ws_ne_rhs(M, N)}), s));
+    const int K = restart;
+    // Hessenberg column slots of a chunk: per system kMaxChunk slots of K + 2
+    // entries (the products at 0..j, |w|^2 at K + 1)
+    const long hs = K + 2, hcs = hs * kMaxChunk;
+    const bool ne = A32 && M > N && R == 1 && ne_fused_fits(M, N) && !getenv("BEM_NE_TWO_PASS");
+    std::unique_ptr<GmresCache> local;
+    GmresCache *gc = nullptr;
+    if (cache && *cache && (*cache)->matches(N, M, R, restart, ne, s)) {
+        gc = *cache;
+    } else {
+        if (cache) gmres_cache_free(*cache);
+        local.reset(new GmresCache);
+        local->ne = ne;
+        KrylovWS &w = local->ws;
+        HOST_CALL(w.init(N, M, R, restart, s));
+        if (ne) {
+            w.ldr = ne_ldr(N);
+            HOST_CALL(salloc(&w.Ar, (size_t)M * w.ldr, s));
+            HOST_CALL(salloc(&w.nws, std::max({ws_ne_apply(M, N), ws_ne_resid(M, N), ws_ne_rhs(M, N)}), s));
+        }
+        HOST_CALL(salloc(&w.hch, (size_t)hcs * R, s));
+        if (cache) {
+            *cache = local.release();
+            gc = *cache;
+        } else {
+            gc = local.get();
+        }
     }
+    KrylovWS &ws = gc->ws;
     Operator op;
     op.A32 = A32;
     op.A64 = A64;
@@ -386,11 +422,6 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             done[q] = 1;
             HIP_CALL(hipMemsetAsync(X + (size_t)q * ldx, 0, sizeof(double2) * N, s));
         }
-    const int K = restart;
-    // Hessenberg column slots of a chunk: per system kMaxChunk slots of K + 2
-    // entries (the products at 0..j, |w|^2 at K + 1)
-    const long hs = K + 2, hcs = hs * kMaxChunk;
-    HOST_CALL(salloc(&ws.hch, (size_t)hcs * R, s));
     std::vector<double2> hch((size_t)hcs * R);
     double rate = 0.0;
     // Classical Gram-Schmidt, one pass while it is safe: the host sees every

@@ -84,3 +84,42 @@ def test_one_pass_product_matches_oracle():
     print("error vs oracle %.3g, %s" % (err, st))
     assert st["converged"] == 1
     assert err < 1e-5
+
+
+def test_workspace_kept_between_solves_matches_fresh_solver(monkeypatch):
+    """A solver keeps its GMRES workspace (basis, row-major copy of A) between
+    solves of one shape and replaces it when the shape changes: a repeated
+    solve, a solve after reassembly at another wavenumber and a solve with
+    another restart length must each equal a fresh solver's, bit for bit."""
+    monkeypatch.delenv("BEM_NE_TWO_PASS", raising=False)
+    m = _mesh("s3")
+    k1 = 2 * np.pi * 1000.0 / SPEED
+    k2 = 2 * np.pi * 1500.0 / SPEED
+
+    def fresh(k, restart):
+        s = bem.Solver(m)
+        s.assemble(k, bem.PREC_FP32)
+        s.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+        s.solve(bem.SOLVE_GMRES, restart=restart, tol=1e-6, refine_tol=1e-10, max_refine=50)
+        x = s.solution()
+        s.close()
+        return x
+
+    s = bem.Solver(m)
+    s.assemble(k1, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+    kw = dict(tol=1e-6, refine_tol=1e-10, max_refine=50)
+    s.solve(bem.SOLVE_GMRES, restart=30, **kw)
+    xa = s.solution()
+    s.solve(bem.SOLVE_GMRES, restart=30, **kw)
+    xb = s.solution()
+    s.assemble(k2, bem.PREC_FP32)
+    s.solve(bem.SOLVE_GMRES, restart=30, **kw)
+    xc = s.solution()
+    s.solve(bem.SOLVE_GMRES, restart=20, **kw)
+    xd = s.solution()
+    s.close()
+    np.testing.assert_array_equal(xa, fresh(k1, 30))
+    np.testing.assert_array_equal(xb, xa)
+    np.testing.assert_array_equal(xc, fresh(k2, 30))
+    np.testing.assert_array_equal(xd, fresh(k2, 20))
