@@ -140,6 +140,7 @@ struct DeviceProblem {
     int P = 0;                    // column tiles
     int qg = 1;                   // lanes per row in the quadrature kernel
     int impedance = 0;            // some element has ca != 0
+    double diam = 0;              // bounding-box diagonal of the M points: bounds every |x - y|
     int *tile_ptr = nullptr;
     int *tile_entries = nullptr;
     int n_isolated = 0;

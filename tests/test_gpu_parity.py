@@ -67,6 +67,28 @@ def test_assembly_fp32_matches_oracle(qg, k, monkeypatch):
     assert err < 2e-5
 
 
+@pytest.mark.parametrize("far", [False, True])
+def test_assembly_fp32_phase_range(far, monkeypatch):
+    """The fast kernel hands k r / (2 pi) unreduced to v_sin_f32 / v_cos_f32
+    while the bounding-box diagonal of the points keeps it within their
+    256-revolution range, and reduces it first otherwise.  A CHIEF point 500 m
+    away (k = 36.6: about 2900 revolutions) selects the reduced form; both
+    forms must match the oracle."""
+    monkeypatch.setenv("BEM_QG", "1")
+    m = _icosphere(2)
+    ch = C.ball_points(101, 3, 0.05)
+    if far:
+        ch = np.vstack([ch, [[500.0, 0.0, 0.0]]])
+    m.set_chief(ch)
+    k = 36.6
+    print("case: icosphere s2 + %d CHIEF (far point: %s), k=%g" % (len(ch), far, k), flush=True)
+    A, B = bem.bem_system(m, k)
+    Ao, Bo = _oracle_system(m, k)
+    err = _rel_max(A, Ao)
+    print("FP32 assembly max rel err %.3g" % err)
+    assert err < 2e-5
+
+
 def test_assembly_fp64_matches_oracle():
     m = _icosphere(2, nchief=4)
     s = bem.Solver(m)
