@@ -101,6 +101,16 @@ __device__ __forceinline__ int xcd_remap(int g, int n) {
 }
 
 // --------------------------------------------------------- element prep ---
+// 6-point Gauss-Legendre nodes and weights on [-1, 1]
+struct GL6 {
+    static constexpr double eta[6] = {-0.9324695142031520278123016, -0.6612093864662645136613996,
+                                      -0.2386191860831969086305017, 0.2386191860831969086305017,
+                                      0.6612093864662645136613996,  0.9324695142031520278123016};
+    static constexpr double w[6] = {0.1713244923791703450402961, 0.3607615730481386075698335,
+                                    0.4679139345726910473898703, 0.4679139345726910473898703,
+                                    0.3607615730481386075698335, 0.1713244923791703450402961};
+};
+
 template <typename R>
 __global__ void prep_elems(const double *__restrict__ pts64, const int *__restrict__ elems,
                            const float2 *__restrict__ bc, int E, ElemRec<R> *__restrict__ out) {
@@ -136,6 +146,17 @@ __global__ void prep_elems(const double *__restrict__ pts64, const int *__restri
     r.nodes[0] = n0;
     r.nodes[1] = n1;
     r.nodes[2] = n2;
+    double f[3];
+    for (int d = 0; d < 3; d++) {
+        f[d] = e2[d] - e1[d];
+        r.f[d] = (R)f[d];
+    }
+    for (int t = 0; t < 6; t++) {
+        const double th = 0.5 + 0.5 * GL6::eta[t];
+        double vv = 0;
+        for (int d = 0; d < 3; d++) vv += (e1[d] + th * f[d]) * (e1[d] + th * f[d]);
+        r.vv[t] = (R)vv;
+    }
     for (int i = 0; i < (int)(sizeof(r.pad) / sizeof(int)); i++) r.pad[i] = 0;
     out[e] = r;
 }
@@ -413,14 +434,6 @@ __global__ __launch_bounds__(256) void asm_tile(AsmArgs<R> a) {
 // vertex p3; the rows where r -> 0 (x a vertex of e) take the FP64 singular
 // rule, and on icosphere s5 at 2 and 8 kHz the entries are as close to FP64
 // as with the direct form (relative Frobenius error 9.3e-8 and 1.06e-7).
-struct GL6 {
-    static constexpr double eta[6] = {-0.9324695142031520278123016, -0.6612093864662645136613996,
-                                      -0.2386191860831969086305017, 0.2386191860831969086305017,
-                                      0.6612093864662645136613996,  0.9324695142031520278123016};
-    static constexpr double w[6] = {0.1713244923791703450402961, 0.3607615730481386075698335,
-                                    0.4679139345726910473898703, 0.4679139345726910473898703,
-                                    0.3607615730481386075698335, 0.1713244923791703450402961};
-};
 __device__ __forceinline__ constexpr float gl_theta(int n) { return (float)(0.5 + 0.5 * GL6::eta[n]); }
 __device__ __forceinline__ constexpr float gl_rho(int m) { return (float)(0.5 + 0.5 * GL6::eta[m]); }
 __device__ __forceinline__ constexpr float gl_W(int n, int m) {
@@ -452,18 +465,18 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
         const ElemRec<float> &rec = a.recs[e];
         const float ax = rec.p3[0] - xp.x, ay = rec.p3[1] - xp.y, az = rec.p3[2] - xp.z;
         const float dn = rec.n[0] * ax + rec.n[1] * ay + rec.n[2] * az;
-        const float e1x = rec.e1[0], e1y = rec.e1[1], e1z = rec.e1[2];
-        const float fx = rec.e2[0] - e1x, fy = rec.e2[1] - e1y, fz = rec.e2[2] - e1z;
         const float aa = __builtin_fmaf(ax, ax, __builtin_fmaf(ay, ay, az * az));
+        // 2 a.v_n = 2 a.e1 + 2 theta_n a.f
+        const float ae1 = __builtin_fmaf(ax, rec.e1[0], __builtin_fmaf(ay, rec.e1[1], az * rec.e1[2]));
+        const float af = __builtin_fmaf(ax, rec.f[0], __builtin_fmaf(ay, rec.f[1], az * rec.f[2]));
+        const float ae1x2 = ae1 + ae1;
         float h1r = 0, h1i = 0, h2r = 0, h2i = 0, htr = 0, hti = 0, gr = 0, gi = 0, cc = 0;
         float g1r = 0, g1i = 0, g2r = 0, g2i = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
             const float th = gl_theta(n);
-            const float vx = __builtin_fmaf(th, fx, e1x), vy = __builtin_fmaf(th, fy, e1y),
-                        vz = __builtin_fmaf(th, fz, e1z);
-            const float tav = 2.0f * __builtin_fmaf(ax, vx, __builtin_fmaf(ay, vy, az * vz));
-            const float vv = __builtin_fmaf(vx, vx, __builtin_fmaf(vy, vy, vz * vz));
+            const float tav = __builtin_fmaf(2.0f * th, af, ae1x2);
+            const float vv = rec.vv[n];
             float Anr = 0, Ani = 0, Bnr = 0, Bni = 0, Cnr = 0, Cni = 0;
 #pragma unroll
             for (int m = 0; m < 6; m++) {

@@ -109,7 +109,9 @@ void build_tile_plan(const Mesh &m, int P, TilePlan &plan);
 
 // Element record read by the assembly kernels with scalar loads (128 B):
 //   p3, e1 = p1-p3, e2 = p2-p3, n (unit normal of e1 x e2), J = |e1 x e2|,
-//   beta = ca/cb, gamma = cc/cb, nodes.
+//   beta = ca/cb, gamma = cc/cb, nodes, and for the separable FP32 rule
+//   f = e2 - e1 and vv[n] = |e1 + theta_n f|^2 at the six Gauss-Legendre
+//   theta_n (element-only terms of the squared distance).
 template <typename R>
 struct alignas(16) ElemRec {
     R p3[3];
@@ -120,7 +122,9 @@ struct alignas(16) ElemRec {
     R beta[2];
     R gamma[2];
     int nodes[3];
-    int pad[(sizeof(R) == 4) ? 12 : 1];
+    R f[3];
+    R vv[6];
+    int pad[(sizeof(R) == 4) ? 3 : 1];
 };
 static_assert(sizeof(ElemRec<float>) == 128, "record size");
 
