@@ -436,6 +436,10 @@ __global__ __launch_bounds__(256) void asm_tile(AsmArgs<R> a) {
 // as with the direct form (relative Frobenius error 9.3e-8 and 1.06e-7).
 __device__ __forceinline__ constexpr float gl_theta(int n) { return (float)(0.5 + 0.5 * GL6::eta[n]); }
 __device__ __forceinline__ constexpr float gl_rho(int m) { return (float)(0.5 + 0.5 * GL6::eta[m]); }
+// theta_n - theta_{n+1} for n < 5, theta_5 for n = 5
+__device__ __forceinline__ constexpr float gl_dtheta(int n) {
+    return n < 5 ? (float)(0.5 * (GL6::eta[n] - GL6::eta[n < 5 ? n + 1 : n])) : (float)(0.5 + 0.5 * GL6::eta[5]);
+}
 __device__ __forceinline__ constexpr float gl_W(int n, int m) {
     return (float)(0.25 * GL6::w[n] * GL6::w[m] * (0.5 + 0.5 * GL6::eta[m]));
 }
@@ -470,14 +474,18 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
         const float ae1 = __builtin_fmaf(ax, rec.e1[0], __builtin_fmaf(ay, rec.e1[1], az * rec.e1[2]));
         const float af = __builtin_fmaf(ax, rec.f[0], __builtin_fmaf(ay, rec.f[1], az * rec.f[2]));
         const float ae1x2 = ae1 + ae1;
-        float h1r = 0, h1i = 0, h2r = 0, h2i = 0, htr = 0, hti = 0, gr = 0, gi = 0, cc = 0;
-        float g1r = 0, g1i = 0, g2r = 0, g2i = 0;
+        // S = sum W rho H~ and ht = sum W H~ over all 36 points; the theta
+        // moment h2 = sum_n theta_n A_n of the per-row sums A_n comes from the
+        // running totals S_n of S after each row,
+        //   h2 = sum_{n<5} (theta_n - theta_{n+1}) S_n + theta_5 S_5,
+        // and h1 = S - h2 (likewise g1, g2 from SC = sum W rho G~)
+        float Sr = 0, Si = 0, h2r = 0, h2i = 0, htr = 0, hti = 0, gr = 0, gi = 0, cc = 0;
+        float SCr = 0, SCi = 0, g2r = 0, g2i = 0;
 #pragma unroll
         for (int n = 0; n < 6; n++) {
             const float th = gl_theta(n);
             const float tav = __builtin_fmaf(2.0f * th, af, ae1x2);
             const float vv = rec.vv[n];
-            float Anr = 0, Ani = 0, Bnr = 0, Bni = 0, Cnr = 0, Cni = 0;
 #pragma unroll
             for (int m = 0; m < 6; m++) {
                 const float rho = gl_rho(m), W = gl_W(n, m), Wr = gl_Wrho(n, m);
@@ -499,17 +507,17 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
                 // 4 pi r dG/dn / (-dn) = (cos kr - i sin kr)(1/r^3 + i k/r^2)
                 const float Hr = __builtin_fmaf(cs, q3, sn * q2k);
                 const float Hi = __builtin_fmaf(cs, q2k, -sn * q3);
-                Anr = __builtin_fmaf(Wr, Hr, Anr);
-                Ani = __builtin_fmaf(Wr, Hi, Ani);
-                Bnr = __builtin_fmaf(W, Hr, Bnr);
-                Bni = __builtin_fmaf(W, Hi, Bni);
+                Sr = __builtin_fmaf(Wr, Hr, Sr);
+                Si = __builtin_fmaf(Wr, Hi, Si);
+                htr = __builtin_fmaf(W, Hr, htr);
+                hti = __builtin_fmaf(W, Hi, hti);
                 // 4 pi G = (cos kr - i sin kr) / r
                 if (IMP) {
                     const float Gr = cs * rinv, Gi = -sn * rinv;
                     gr = __builtin_fmaf(W, Gr, gr);
                     gi = __builtin_fmaf(W, Gi, gi);
-                    Cnr = __builtin_fmaf(Wr, Gr, Cnr);
-                    Cni = __builtin_fmaf(Wr, Gi, Cni);
+                    SCr = __builtin_fmaf(Wr, Gr, SCr);
+                    SCi = __builtin_fmaf(Wr, Gi, SCi);
                 } else {
                     const float Wg = W * rinv;
                     gr = __builtin_fmaf(Wg, cs, gr);
@@ -517,25 +525,22 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
                 }
                 cc = __builtin_fmaf(W, q3, cc);
             }
-            const float om = 1.0f - th;
-            h1r = __builtin_fmaf(om, Anr, h1r);
-            h1i = __builtin_fmaf(om, Ani, h1i);
-            h2r = __builtin_fmaf(th, Anr, h2r);
-            h2i = __builtin_fmaf(th, Ani, h2i);
-            htr += Bnr;
-            hti += Bni;
+            // theta moment of the rho-weighted sums from their running totals
+            const float dth = gl_dtheta(n);
+            h2r = __builtin_fmaf(dth, Sr, h2r);
+            h2i = __builtin_fmaf(dth, Si, h2i);
             if (IMP) {
-                g1r = __builtin_fmaf(om, Cnr, g1r);
-                g1i = __builtin_fmaf(om, Cni, g1i);
-                g2r = __builtin_fmaf(th, Cnr, g2r);
-                g2i = __builtin_fmaf(th, Cni, g2i);
+                g2r = __builtin_fmaf(dth, SCr, g2r);
+                g2i = __builtin_fmaf(dth, SCi, g2i);
             }
         }
+        const float h1r = Sr - h2r, h1i = Si - h2i;
+        const float g1r = SCr - g2r, g1i = SCi - g2i;
         const float sJ = rec.J * inv4pi;
         const float sh = -dn * sJ;
         float p0r = sh * h1r, p0i = sh * h1i;
         float p1r = sh * h2r, p1i = sh * h2i;
-        float p2r = sh * (htr - h1r - h2r), p2i = sh * (hti - h1i - h2i);
+        float p2r = sh * (htr - Sr), p2i = sh * (hti - Si);
         const float sg = a.rw * sJ;
         const float gsr = -sg * gi, gsi = sg * gr;
         if (IMP) {
