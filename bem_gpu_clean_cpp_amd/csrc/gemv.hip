@@ -688,6 +688,12 @@ int transpose_rm_ahb_f32(const float2 *A, long lda, long M, long N, float2 *Ar, 
     return EXIT_SUCCESS;
 }
 
+// xpad[i] = x[i] for i < n, 0 up to ldp (one launch instead of a copy and a fill)
+__global__ void k_pad_copy(const double2 *__restrict__ x, int n, double2 *__restrict__ xpad, int ldp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ldp) xpad[i] = i < n ? x[i] : make_double2(0.0, 0.0);
+}
+
 int ne_resid_f32a(const float2 *Ar, long ldr, long M, long N, const double2 *x, const double2 *b, double2 *r, void *ws,
                   hipStream_t s) {
     if (!ne_fused_fits(M, N)) BEM_FAIL("fused normal-equation residual: N = %ld is over its limit", N);
@@ -704,8 +710,8 @@ int ne_resid_f32a(const float2 *Ar, long ldr, long M, long N, const double2 *x, 
     double2 *xpad = nullptr;
     if (x) {
         xpad = (double2 *)((char *)ws + 2 * ws_ne_apply(M, N));
-        HIP_CALL(hipMemcpyAsync(xpad, x, sizeof(double2) * N, hipMemcpyDeviceToDevice, s));
-        if (ldr > N) HIP_CALL(hipMemsetAsync(xpad + N, 0, sizeof(double2) * (ldr - N), s));
+        k_pad_copy<<<(unsigned)((ldr + 255) / 256), 256, 0, s>>>(x, (int)N, xpad, (int)ldr);
+        HIP_CALL(hipGetLastError());
     }
 #define NE_LAUNCH(NT_, PPT_) \
     ne_resid_part<NT_, PPT_><<<nblk, NT_, 0, s>>>(A4, ldr4, (int)M, rpb, (const double4 *)xpad, b, (int)N, p4, ldp4)

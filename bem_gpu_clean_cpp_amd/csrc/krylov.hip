@@ -341,6 +341,13 @@ struct Operator {
 struct GmresCache {
     KrylovWS ws;
     bool ne = false;
+    // pinned host copies of the norms, the Hessenberg columns of a chunk and
+    // the update coefficients (device-to-host reads of pageable memory are
+    // staged, and each one stalls the host loop)
+    double2 *pin = nullptr;
+    ~GmresCache() {
+        if (pin) (void)hipHostFree(pin);
+    }
     bool matches(int N, int M, int R, int restart, bool ne_, hipStream_t s) const {
         return ws.n == N && ws.m == M && ws.R == R && ws.restart == restart && ne == ne_ && ws.st == s;
     }
@@ -378,6 +385,9 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             HOST_CALL(salloc(&w.nws, std::max({ws_ne_apply(M, N), ws_ne_resid(M, N), ws_ne_rhs(M, N)}), s));
         }
         HOST_CALL(salloc(&w.hch, (size_t)hcs * R, s));
+        if (cache)
+            HIP_CALL(hipHostMalloc((void **)&local->pin, sizeof(double2) * ((size_t)w.hstride * R * 2 + hcs * R),
+                                   hipHostMallocDefault));
         if (cache) {
             *cache = local.release();
             gc = *cache;
@@ -398,10 +408,21 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     // copy and A^H b come from one pass over A.
     if (ws.Ar) HOST_CALL(transpose_rm_ahb_f32(A32, lda, M, N, ws.Ar, ws.ldr, B, ws.r, ws.nws, s));
     else HOST_CALL(op.rhs(ws, B, ldb, ws.r, s));
-    std::vector<double2> hbuf((size_t)ws.hstride * R);
+    // host buffers: pinned when the workspace is kept, else plain vectors
+    const size_t nh = (size_t)ws.hstride * R;
+    std::vector<double2> hvec;
+    double2 *hbuf, *yh, *hch;
+    if (gc->pin) {
+        hbuf = gc->pin;
+    } else {
+        hvec.resize(2 * nh + (size_t)hcs * R);
+        hbuf = hvec.data();
+    }
+    yh = hbuf + nh;
+    hch = yh + nh;
     auto norms = [&](const double2 *w, std::vector<double> &out) -> int {
         HOST_CALL(bnorm2(ws, w, ldv, N, ws.h, ws.hstride, s));
-        HIP_CALL(hipMemcpyAsync(hbuf.data(), ws.h, sizeof(double2) * ws.hstride * R, hipMemcpyDeviceToHost, s));
+        HIP_CALL(hipMemcpyAsync(hbuf, ws.h, sizeof(double2) * ws.hstride * R, hipMemcpyDeviceToHost, s));
         HIP_CALL(hipStreamSynchronize(s));
         for (int q = 0; q < R; q++) out[q] = std::sqrt(std::max(0.0, hbuf[(size_t)q * ws.hstride].x));
         return EXIT_SUCCESS;
@@ -422,7 +443,6 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             done[q] = 1;
             HIP_CALL(hipMemsetAsync(X + (size_t)q * ldx, 0, sizeof(double2) * N, s));
         }
-    std::vector<double2> hch((size_t)hcs * R);
     double rate = 0.0;
     // Classical Gram-Schmidt, one pass while it is safe: the host sees every
     // step's products h and the new norm hn, and |w before|^2 = hn^2 + sum |h|^2,
@@ -520,7 +540,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                                                ws.vstride, op.A32 ? ws.v32 : nullptr, ldv, N);
                 HIP_CALL(hipGetLastError());
             }
-            HIP_CALL(hipMemcpyAsync(hch.data(), ws.hch, sizeof(double2) * hcs * R, hipMemcpyDeviceToHost, s));
+            HIP_CALL(hipMemcpyAsync(hch, ws.hch, sizeof(double2) * hcs * R, hipMemcpyDeviceToHost, s));
             HIP_CALL(hipStreamSynchronize(s));
             for (int q = 0; q < R; q++) {
                 g_start[q] = std::abs(g[(size_t)q * (K + 1) + j]);
@@ -594,7 +614,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
         int tmax = 0;
         for (int q = 0; q < R; q++) tmax = std::max(tmax, tlen[q]);
         if (tmax == 0) continue;
-        std::vector<double2> yh((size_t)ws.hstride * R, make_double2(0, 0));
+        std::fill(yh, yh + nh, make_double2(0, 0));
         for (int q = 0; q < R; q++) {
             const int t = tlen[q];
             if (done[q] || t == 0) continue;
@@ -608,11 +628,12 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             }
             for (int i = 0; i < t; i++) yh[(size_t)q * ws.hstride + i] = make_double2(y[i].real(), y[i].imag());
         }
-        HIP_CALL(hipMemcpyAsync(ws.h, yh.data(), sizeof(double2) * ws.hstride * R, hipMemcpyHostToDevice, s));
+        HIP_CALL(hipMemcpyAsync(ws.h, yh, sizeof(double2) * ws.hstride * R, hipMemcpyHostToDevice, s));
         dim3 gg((N + bs - 1) / bs, R);
         kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, tmax, ws.h, ws.hstride, 1.0, X, ldx, N);
         HIP_CALL(hipGetLastError());
-        HIP_CALL(hipStreamSynchronize(s));  // yh goes out of scope
+        // no wait here: yh is next written after the next chunk's synchronisation
+        // (and a copy from pageable memory is staged before the call returns)
         res_current = false;
     }
     // final FP64 residuals (unless the last check already saw the current X)
