@@ -451,6 +451,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     // from then on every step of the solve takes the second pass (CGS2,
     // "twice is enough").  BEM_CGS2 forces two passes throughout.
     bool cgs2 = getenv("BEM_CGS2") != nullptr;
+    const bool full_chunks = getenv("BEM_GMRES_FULL_CHUNKS") != nullptr;
     constexpr double kCancel = 1e-2;
     std::vector<double> g_start(R), g_end(R);
     std::vector<int> steps(R);
@@ -518,6 +519,10 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                     if (gj <= ctol[q] * beta[q]) continue;
                     need = std::max(need, (int)std::ceil(std::log(ctol[q] * beta[q] / gj) / std::log(rate)));
                 }
+                // one step short of the estimate: GMRES usually converges faster
+                // than its last rate says, and a further chunk costs one host
+                // round trip where a step past convergence costs a product
+                if (!full_chunks) need = need - 1;
                 c = std::max(1, std::min(c, need));
             } else if (j == 0 && cycles == 1) {
                 c = std::min(c, 4);
