@@ -313,7 +313,9 @@ int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 //                     reduction slots alternate by batch parity;
 //   ne_reduce         w = sum of the block partials in a fixed order, in FP64.
 constexpr int kNeBlocks = 256;
-constexpr int kNeMaxPairs = 8;  // column pairs per thread at 768 threads: N <= 12288
+// column pairs a row may have: 1024 threads x 6 pairs for the product,
+// 512 x 12 for the residual (N <= 12288)
+constexpr long kNeMaxRowPairs = 6144;
 
 // The row-major copy in 64 x 64 LDS tiles (Ar[i * ldr + j] = A[j * lda + i],
 // zero for N <= j < ldr, the padding the pair loads read), also forming the
@@ -500,7 +502,7 @@ __global__ __launch_bounds__(256) void ne_reduce(const float2 *__restrict__ part
     }
 }
 
-bool ne_fused_fits(long M, long N) { return M > 0 && N > 0 && (N + 1) / 2 <= 768L * kNeMaxPairs; }
+bool ne_fused_fits(long M, long N) { return M > 0 && N > 0 && (N + 1) / 2 <= kNeMaxRowPairs; }
 long ne_ldr(long N) { return (N + 63) / 64 * 64; }
 static void ne_shape(long M, int &rows_per_blk, int &nblk) {
     rows_per_blk = (int)((M + kNeBlocks - 1) / kNeBlocks);
@@ -525,23 +527,24 @@ int ne_apply_f32(const float2 *Ar, long ldr, long M, long N, const float2 *v, do
     const float4 *A4 = (const float4 *)Ar, *v4 = (const float4 *)v;
     float4 *p4 = (float4 *)ws;
     const long ldr4 = ldr / 2, ldp4 = ldp / 2;
-    // 1024 threads while 2 pairs per thread cover a row, then 512 threads
-    // (the register room for up to 12 pairs, with the next row in flight)
-    // two rows in flight per batch: more rows push the widest rows' registers
-    // into scratch (measured at N = 10242: 162 us with 2 rows, 184 with 3,
-    // 261 with 4)
-#define NE_LAUNCH(NT_, PPT_) \
-    ne_apply_part<NT_, PPT_, 2><<<nblk, NT_, 0, s>>>(A4, ldr4, (int)M, rpb, v4, (int)N, p4, ldp4)
+    // 1024 threads (4 waves per SIMD) with ceil(pairs / 1024) pairs each; two
+    // rows per batch while the pairs take few registers, then one (measured at
+    // N = 10242: 1024 threads x 6 pairs, one row, 155 us; 768 x 7, two rows,
+    // 165 us; two rows at 768 x 7 against three or four: 162, 184, 261 us)
+#define NE_LAUNCH(PPT_, RB_) \
+    ne_apply_part<1024, PPT_, RB_><<<nblk, 1024, 0, s>>>(A4, ldr4, (int)M, rpb, v4, (int)N, p4, ldp4)
     if (npairs <= 1024) {
-        NE_LAUNCH(1024, 1);
+        NE_LAUNCH(1, 2);
     } else if (npairs <= 2048) {
-        NE_LAUNCH(1024, 2);
-    } else if (npairs <= 768 * 6) {
-        NE_LAUNCH(768, 6);
-    } else if (npairs <= 768 * 7) {
-        NE_LAUNCH(768, 7);
+        NE_LAUNCH(2, 2);
+    } else if (npairs <= 3072) {
+        NE_LAUNCH(3, 2);
+    } else if (npairs <= 4096) {
+        NE_LAUNCH(4, 1);
+    } else if (npairs <= 5120) {
+        NE_LAUNCH(5, 1);
     } else {
-        NE_LAUNCH(768, 8);
+        NE_LAUNCH(6, 1);
     }
 #undef NE_LAUNCH
     HIP_CALL(hipGetLastError());
