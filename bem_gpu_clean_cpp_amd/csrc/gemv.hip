@@ -333,11 +333,24 @@ __global__ __launch_bounds__(256) void k_transpose_rm_ahb(const float2 *__restri
     __shared__ double2 s_p[4][64];
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    for (int jj = ty; jj < 64; jj += 4) {
-        const int i = i0 + tx, j = j0 + jj;
-        tile[jj][tx] = (i < M && j < N) ? A[(size_t)j * lda + i] : make_float2(0.f, 0.f);
+    // all 16 loads of a thread in flight at once: clamped addresses, zero
+    // selected outside the matrix (a conditional load per element serialises
+    // them, one wait each)
+    double2 bv = make_double2(0, 0);
+    if (ty == 0 && i0 + tx < M) bv = b[i0 + tx];
+    {
+        const int i = i0 + tx, ic = min(i, M - 1);
+        float2 v[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int j = j0 + ty + 4 * q;
+            const float2 x = A[(size_t)min(j, N - 1) * lda + ic];
+            v[q] = (i < M && j < N) ? x : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) tile[ty + 4 * q][tx] = v[q];
     }
-    if (ty == 0) s_b[tx] = i0 + tx < M ? b[i0 + tx] : make_double2(0, 0);
+    if (ty == 0) s_b[tx] = bv;
     __syncthreads();
     double pr = 0, pi = 0;
     for (int ii = ty; ii < 64; ii += 4) {
