@@ -290,7 +290,19 @@ __global__ __launch_bounds__(256) void gemm_reduce(const typename MF<T>::c2 *__r
     const int n = blockIdx.y;
     if (i >= M) return;
     S sr = 0, si = 0;
-    for (int c = 0; c < nchunks; c++) {
+    // 8 partials in flight per step, added in chunk order
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+        C2 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) p[u] = part[((size_t)(c + u) * R + n) * ldp + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            sr += (S)p[u].x;
+            si += (S)p[u].y;
+        }
+    }
+    for (; c < nchunks; c++) {
         const C2 p = part[((size_t)c * R + n) * ldp + i];
         sr += (S)p.x;
         si += (S)p.y;
@@ -561,7 +573,19 @@ __global__ __launch_bounds__(256) void mres_reduce(const double2 *__restrict__ p
     const int r = blockIdx.y;
     if (i >= M) return;
     double sr = 0, si = 0;
-    for (int c = 0; c < nchunks; c++) {
+    // 8 partials in flight per step, added in chunk order
+    int c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+        double2 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) p[u] = part[((size_t)(c + u) * R + r) * ldp + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            sr += p[u].x;
+            si += p[u].y;
+        }
+    }
+    for (; c < nchunks; c++) {
         const double2 p = part[((size_t)c * R + r) * ldp + i];
         sr += p.x;
         si += p.y;

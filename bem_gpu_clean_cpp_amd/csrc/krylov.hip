@@ -58,13 +58,21 @@ __global__ __launch_bounds__(256) void kb_mdot_part(const double2 *__restrict__ 
     const double2 *wr = w + (size_t)r * wstride;
     const int base = blockIdx.x * 1024;
     double sr = 0, si = 0;
-#pragma unroll 4
-    for (int u = 0; u < 16; u++) {
-        const int i = base + lane + 64 * u;
-        if (i < n) {
-            const double2 v = vl[i], x = wr[i];
-            sr = fma(v.x, x.x, fma(v.y, x.y, sr));
-            si = fma(v.x, x.y, fma(-v.y, x.x, si));
+    // loads in flight 8 pairs at a time: clamped rows, v zeroed past n
+#pragma unroll
+    for (int u0 = 0; u0 < 16; u0 += 8) {
+        double2 v[8], x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = base + lane + 64 * (u0 + u), ic = min(i, n - 1);
+            v[u] = vl[ic];
+            x[u] = wr[ic];
+            if (i >= n) v[u] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            sr = fma(v[u].x, x[u].x, fma(v[u].y, x[u].y, sr));
+            si = fma(v[u].x, x[u].y, fma(-v[u].y, x[u].x, si));
         }
     }
 #pragma unroll
@@ -82,7 +90,18 @@ __global__ void kb_mdot_final(const double2 *__restrict__ part, long pstride, in
     if (l >= nv) return;
     part += (size_t)r * pstride;
     double sr = 0, si = 0;
-    for (int b = 0; b < nblk; b++) {
+    int b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+        double2 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) p[u] = part[(size_t)(b + u) * nv + l];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            sr += p[u].x;
+            si += p[u].y;
+        }
+    }
+    for (; b < nblk; b++) {
         const double2 p = part[(size_t)b * nv + l];
         sr += p.x;
         si += p.y;
@@ -102,7 +121,15 @@ __global__ __launch_bounds__(1024) void kb_norm2(const double2 *__restrict__ w, 
     __shared__ double red[1024];
     const double2 *wr = w + (size_t)blockIdx.x * wstride;
     double sum = 0;
-    for (int i = threadIdx.x; i < n; i += 1024) sum = fma(wr[i].x, wr[i].x, fma(wr[i].y, wr[i].y, sum));
+    int i = threadIdx.x;
+    for (; i + 3 * 1024 < n; i += 4 * 1024) {
+        double2 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) x[u] = wr[i + 1024 * u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) sum = fma(x[u].x, x[u].x, fma(x[u].y, x[u].y, sum));
+    }
+    for (; i < n; i += 1024) sum = fma(wr[i].x, wr[i].x, fma(wr[i].y, wr[i].y, sum));
     red[threadIdx.x] = sum;
     __syncthreads();
     for (int m = 512; m >= 1; m >>= 1) {
@@ -121,7 +148,22 @@ __global__ void kb_maxpy(const double2 *__restrict__ V, long ldv, long vstride, 
     h += (size_t)r * hstride;
     w += (size_t)r * wstride;
     double sr = 0, si = 0;
-    for (int l = 0; l < nv; l++) {
+    // basis vectors 8 at a time with their loads in flight, added in order
+    int l = 0;
+    for (; l + 8 <= nv; l += 8) {
+        double2 v[8], c[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            v[u] = V[(size_t)(l + u) * ldv + i];
+            c[u] = h[l + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            sr = fma(v[u].x, c[u].x, fma(-v[u].y, c[u].y, sr));
+            si = fma(v[u].x, c[u].y, fma(v[u].y, c[u].x, si));
+        }
+    }
+    for (; l < nv; l++) {
         const double2 v = V[(size_t)l * ldv + i];
         const double2 c = h[l];
         sr = fma(v.x, c.x, fma(-v.y, c.y, sr));
