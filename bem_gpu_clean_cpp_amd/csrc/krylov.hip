@@ -189,6 +189,118 @@ __global__ void kb_normalize(const double2 *__restrict__ w, long wstride, const 
     if (out32) out32[(size_t)r * o32stride + i] = make_float2((float)v.x, (float)v.y);
 }
 
+// One Gram-Schmidt pass after kb_mdot_part, in one launch: every workgroup
+// forms h_r[l] = sum_b part_r[b][l] (fixed order) in LDS; workgroup 0 stores
+// it to hout (and adds it to acc); then w_r += sign V_r h_r over the
+// workgroup's 256 rows, and the workgroup's partial of |w_r|^2 (fixed-order
+// tree) goes to npart_r[blockIdx.x].  Dynamic LDS: nv double2.
+__global__ __launch_bounds__(256) void kb_final_maxpy(const double2 *__restrict__ part, long pstride, int nblk,
+                                                      const double2 *__restrict__ V, long ldv, long vstride, int nv,
+                                                      double2 *__restrict__ hout, long hstride,
+                                                      double2 *__restrict__ acc, long astride, double sign,
+                                                      double2 *__restrict__ w, long wstride, int n,
+                                                      double *__restrict__ npart, long nstride) {
+    extern __shared__ double2 s_h[];
+    __shared__ double s_n[4];
+    const int r = blockIdx.y;
+    const double2 *pr = part + (size_t)r * pstride;
+    for (int l = threadIdx.x; l < nv; l += 256) {
+        double sr = 0, si = 0;
+        int b = 0;
+        for (; b + 8 <= nblk; b += 8) {
+            double2 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) q[u] = pr[(size_t)(b + u) * nv + l];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                sr += q[u].x;
+                si += q[u].y;
+            }
+        }
+        for (; b < nblk; b++) {
+            const double2 q = pr[(size_t)b * nv + l];
+            sr += q.x;
+            si += q.y;
+        }
+        s_h[l] = make_double2(sr, si);
+        if (blockIdx.x == 0) {
+            hout[(size_t)r * hstride + l] = make_double2(sr, si);
+            if (acc) {
+                double2 a = acc[(size_t)r * astride + l];
+                a.x += sr;
+                a.y += si;
+                acc[(size_t)r * astride + l] = a;
+            }
+        }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double nrm = 0;
+    if (i < n) {
+        const double2 *Vr = V + (size_t)r * vstride;
+        double sr = 0, si = 0;
+        int l = 0;
+        for (; l + 8 <= nv; l += 8) {
+            double2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = Vr[(size_t)(l + u) * ldv + i];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double2 c = s_h[l + u];
+                sr = fma(v[u].x, c.x, fma(-v[u].y, c.y, sr));
+                si = fma(v[u].x, c.y, fma(v[u].y, c.x, si));
+            }
+        }
+        for (; l < nv; l++) {
+            const double2 v = Vr[(size_t)l * ldv + i];
+            const double2 c = s_h[l];
+            sr = fma(v.x, c.x, fma(-v.y, c.y, sr));
+            si = fma(v.x, c.y, fma(v.y, c.x, si));
+        }
+        double2 o = w[(size_t)r * wstride + i];
+        o.x += sign * sr;
+        o.y += sign * si;
+        w[(size_t)r * wstride + i] = o;
+        nrm = fma(o.x, o.x, o.y * o.y);
+    }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) nrm += __shfl_xor(nrm, m, 64);
+    if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) npart[(size_t)r * nstride + blockIdx.x] = (s_n[0] + s_n[1]) + (s_n[2] + s_n[3]);
+}
+
+// |w_r|^2 = sum of the nb partials of kb_final_maxpy (one wave, fixed order);
+// workgroup 0 stores it to hslot_r; out_r = w_r / |w_r| (and its FP32 copy)
+__global__ __launch_bounds__(256) void kb_normalize_np(const double2 *__restrict__ w, long wstride,
+                                                       const double *__restrict__ npart, long nstride, int nb,
+                                                       double2 *__restrict__ hslot, long hsstride,
+                                                       double2 *__restrict__ out, long ostride,
+                                                       float2 *__restrict__ out32, long o32stride, int n) {
+    __shared__ double s_q;
+    const int r = blockIdx.y;
+    if (threadIdx.x < 64) {
+        const double *np = npart + (size_t)r * nstride;
+        double q = 0;
+        for (int b = threadIdx.x; b < nb; b += 64) q += np[b];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) q += __shfl_xor(q, m, 64);
+        if (threadIdx.x == 0) {
+            s_q = q;
+            if (blockIdx.x == 0) hslot[(size_t)r * hsstride] = make_double2(q, 0.0);
+        }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double q = s_q;
+    const double sc = q > 0 ? 1.0 / sqrt(q) : 0.0;
+    const double2 x = w[(size_t)r * wstride + i];
+    const double2 v = make_double2(x.x * sc, x.y * sc);
+    out[(size_t)r * ostride + i] = v;
+    if (out32) out32[(size_t)r * o32stride + i] = make_float2((float)v.x, (float)v.y);
+}
+
 __global__ void kb_cvt(const float2 *__restrict__ in, long istride, double2 *__restrict__ out, long ostride, int n) {
     const int r = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -213,11 +325,13 @@ struct KrylovWS {
     long ldr = 0;
     char *nws = nullptr;
     double2 *hch = nullptr;  // Hessenberg columns of an Arnoldi chunk
+    double *npart = nullptr;  // per system: |w|^2 partials of the 256-row workgroups
+    long nstride = 0;
     hipStream_t st = nullptr;
     ~KrylovWS() {
         sfree(V, st); sfree(w64, st); sfree(t64, st); sfree(r, st); sfree(h, st); sfree(part, st);
         sfree(v32, st); sfree(w32, st); sfree(t32, st); sfree(gws, st); sfree(Ar, st); sfree(nws, st);
-        sfree(hch, st);
+        sfree(hch, st); sfree(npart, st);
     }
     int init(int n_, int m_, int R_, int restart_, hipStream_t s) {
         st = s;
@@ -241,6 +355,8 @@ struct KrylovWS {
         HOST_CALL(salloc(&t32, (size_t)ldv * R, s));
         const size_t g = std::max(std::max(ws_gemv_n(m, n, 1), ws_gemm(m, n, R)), ws_mres(m, n, R));
         HOST_CALL(salloc(&gws, g, s));
+        nstride = (n + 255) / 256;
+        HOST_CALL(salloc(&npart, (size_t)nstride * R, s));
         return EXIT_SUCCESS;
     }
 };
@@ -255,6 +371,16 @@ static int bdot(KrylovWS &ws, const double2 *V, long vstride, int nv, const doub
     dim3 g2(1, ws.R);
     kb_mdot_final<<<g2, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc,
                                                         astride ? astride : ostride);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
+// the partials of V_r^H w_r alone (kb_final_maxpy sums them)
+static int bdot_part(KrylovWS &ws, const double2 *V, long vstride, int nv, const double2 *w, long wstride, int n,
+                     hipStream_t s) {
+    const int nblk = (n + 1023) / 1024;
+    dim3 g1(nblk, ws.R, (nv + 3) / 4);
+    kb_mdot_part<<<g1, 256, 0, s>>>(V, ws.ldv, vstride, nv, w, wstride, n, ws.part, ws.pstride);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
@@ -494,6 +620,8 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     // "twice is enough").  BEM_CGS2 forces two passes throughout.
     bool cgs2 = getenv("BEM_CGS2") != nullptr;
     const bool full_chunks = getenv("BEM_GMRES_FULL_CHUNKS") != nullptr;
+    // fused Gram-Schmidt passes while h fits in LDS
+    const bool fused = !getenv("BEM_ARNOLDI_UNFUSED") && (size_t)(K + 1) * sizeof(double2) <= 48 * 1024;
     constexpr double kCancel = 1e-2;
     std::vector<double> g_start(R), g_end(R);
     std::vector<int> steps(R);
@@ -574,6 +702,27 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                 HOST_CALL(op.apply(ws, jj, s));
                 double2 *h1 = ws.hch + (size_t)t * hs, *h2 = ws.h + (K + 2);
                 dim3 gg((N + bs - 1) / bs, R);
+                if (fused) {
+                    // per pass: partial dots, then sums + update + |w|^2 partials
+                    // in one launch; the normalisation sums the partials
+                    const int nblk = (N + 1023) / 1024;
+                    const size_t lds = sizeof(double2) * (jj + 1);
+                    HOST_CALL(bdot_part(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, s));
+                    kb_final_maxpy<<<gg, bs, lds, s>>>(ws.part, ws.pstride, nblk, ws.V, ldv, ws.vstride, jj + 1, h1,
+                                                       hcs, nullptr, 0, -1.0, ws.w64, ldv, N, ws.npart, ws.nstride);
+                    HIP_CALL(hipGetLastError());
+                    if (cgs2) {
+                        HOST_CALL(bdot_part(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, s));
+                        kb_final_maxpy<<<gg, bs, lds, s>>>(ws.part, ws.pstride, nblk, ws.V, ldv, ws.vstride, jj + 1,
+                                                           h2, ws.hstride, h1, hcs, -1.0, ws.w64, ldv, N, ws.npart,
+                                                           ws.nstride);  // h1 += h2
+                        HIP_CALL(hipGetLastError());
+                    }
+                    kb_normalize_np<<<gg, bs, 0, s>>>(ws.w64, ldv, ws.npart, ws.nstride, (int)gg.x, h1 + (K + 1), hcs,
+                                                      ws.V + (size_t)(jj + 1) * ldv, ws.vstride,
+                                                      op.A32 ? ws.v32 : nullptr, ldv, N);
+                    HIP_CALL(hipGetLastError());
+                } else {
                 HOST_CALL(bdot(ws, ws.V, ws.vstride, jj + 1, ws.w64, ldv, N, h1, hcs, s));
                 kb_maxpy<<<gg, bs, 0, s>>>(ws.V, ldv, ws.vstride, jj + 1, h1, hcs, -1.0, ws.w64, ldv, N);
                 HIP_CALL(hipGetLastError());
@@ -586,6 +735,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                 kb_normalize<<<gg, bs, 0, s>>>(ws.w64, ldv, h1 + (K + 1), hcs, ws.V + (size_t)(jj + 1) * ldv,
                                                ws.vstride, op.A32 ? ws.v32 : nullptr, ldv, N);
                 HIP_CALL(hipGetLastError());
+                }
             }
             HIP_CALL(hipMemcpyAsync(hch, ws.hch, sizeof(double2) * hcs * R, hipMemcpyDeviceToHost, s));
             HIP_CALL(hipStreamSynchronize(s));
