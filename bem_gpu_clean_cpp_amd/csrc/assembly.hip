@@ -79,6 +79,8 @@ struct AsmArgs {
     const R *__restrict__ sing;  // (e*3+v)*16: p[0..5], c[6], bterm[7..8]
     const int *__restrict__ tile_ptr;
     const int *__restrict__ tile_entries;
+    const int *__restrict__ tile_slots;  // element-once plan: side column of each ghost vertex
+    float2 *side;
     typename std::conditional<sizeof(R) == 4, float2, double2>::type *A;
     R *cpart;
     typename std::conditional<sizeof(R) == 4, float2, double2>::type *bpart;
@@ -447,7 +449,7 @@ __device__ __forceinline__ constexpr float gl_Wrho(int n, int m) {
     return (float)(0.25 * GL6::w[n] * GL6::w[m] * (0.5 + 0.5 * GL6::eta[m]) * (0.5 + 0.5 * GL6::eta[m]));
 }
 
-template <bool IMP, bool RAW>
+template <bool IMP, bool RAW, bool GH>
 __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -567,8 +569,10 @@ __global__ __launch_bounds__(256) void asm_tile_fast(AsmArgs<float> a) {
             const float pr[3] = {p0r, p1r, p2r}, pi[3] = {p0i, p1i, p2i};
 #pragma unroll
             for (int j = 0; j < 3; j++) {
-                if ((ent >> j) & 1) {
-                    float2 *dst = a.A + (size_t)rec.nodes[j] * a.ldA + (row - a.r0);
+                float2 *dst = nullptr;
+                if ((ent >> j) & 1) dst = a.A + (size_t)rec.nodes[j] * a.ldA + (row - a.r0);
+                else if (GH) dst = a.side + (size_t)a.tile_slots[3 * it + j] * a.Mp + (row - a.r0);
+                if (dst) {
                     float2 v;
                     if ((ent >> (3 + j)) & 1) {
                         v.x = pr[j];
@@ -604,6 +608,25 @@ __global__ void zero_columns(C2 *A, long ldA, int M, const int *__restrict__ col
     z.x = 0;
     z.y = 0;
     A[(size_t)cols[c] * ldA + i] = z;
+}
+
+// Element-once plan: column of ghost node g += its side columns, in the order
+// they were made (the column is set when the node's own tile wrote nothing).
+__global__ void ghost_reduce(float2 *A, long ldA, int nrows, const float2 *__restrict__ side, long Mp,
+                             const int *__restrict__ gptr, const int *__restrict__ gnode,
+                             const int *__restrict__ gown, const int *__restrict__ gslot) {
+    const int g = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    float2 *col = A + (size_t)gnode[g] * ldA;
+    float2 acc = gown[g] ? col[i] : make_float2(0.f, 0.f);
+    const int s0 = gptr[g], s1 = gptr[g + 1];
+    for (int t = s0; t < s1; t++) {
+        const float2 v = side[(size_t)gslot[t] * Mp + i];
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    col[i] = acc;
 }
 
 // Per row: fixed-order sum of the tile partials in FP64.
@@ -881,6 +904,13 @@ bool tuned_shape(int M, int E, TileShape &out) {
 }
 }  // namespace
 
+// side columns the element-once plan may use: about the nodes on the tile
+// boundaries (4 sqrt(N P) + P; 4190 for N = 10242 in 102 tiles, which need
+// 3110), and never more than N / 2
+static long ghost_slot_bound(int N, int P) {
+    return std::min((long)N / 2, (long)(4.0 * std::sqrt((double)N * P)) + P);
+}
+
 static int choose_tiles(int M, int N, int E, int rows_per_wave) {
     const char *env = getenv("BEM_TILES");
     if (env && atoi(env) > 0) return std::min(atoi(env), N);
@@ -1001,6 +1031,12 @@ size_t device_problem_bytes(int M, int N, int E) {
     b += (sizeof(float) + sizeof(double)) * 48 * (size_t)E;                          // singular terms
     b += sizeof(int) * (P + 1 + 3 * (size_t)E + N);                                  // tile CSR, isolated
     b += (sizeof(float) + sizeof(float2) + sizeof(double) + sizeof(double2)) * P * Mp;  // tile partials
+    if (qg == 1) {
+        // element-once plan: its side columns and index arrays
+        const size_t gs = (size_t)ghost_slot_bound(N, (int)P);
+        b += sizeof(float2) * gs * Mp;
+        b += sizeof(int) * (P + 1 + 4 * (size_t)E + 2 * (size_t)N + 2 * gs);
+    }
     return b;
 }
 
@@ -1072,6 +1108,44 @@ int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
         HIP_CALL(hipMemcpyAsync(dp.isolated, plan.isolated.data(), plan.isolated.size() * sizeof(int),
                                 hipMemcpyHostToDevice, s));
     dp.qg = qg;
+    // element-once plan for the FP32 fast kernel: taken when the halo plan
+    // integrates at least 10 % of the elements twice and the side columns stay
+    // within ghost_slot_bound (device_problem_bytes counts that bound)
+    dp.use_ghost = 0;
+    if (qg == 1 && !getenv("BEM_NO_GHOST_PLAN")) {
+        GhostPlan gp;
+        build_ghost_plan(m, dp.P, gp);
+        // (ghost_reduce puts the ghost nodes on grid.y: at most 65535)
+        if (gp.P == dp.P && gp.dup >= 1.1 && gp.nslots > 0 && gp.nslots <= ghost_slot_bound(dp.N, dp.P) &&
+            gp.gnode.size() <= 65535) {
+            dp.use_ghost = 1;
+            dp.nslots = gp.nslots;
+            dp.ngnodes = (int)gp.gnode.size();
+            HOST_CALL(salloc(&dp.gtile_ptr, gp.ptr.size(), s));
+            HOST_CALL(salloc(&dp.gtile_entries, gp.entries.size(), s));
+            HOST_CALL(salloc(&dp.gtile_slots, gp.slots.size(), s));
+            HOST_CALL(salloc(&dp.gred_ptr, gp.gptr.size(), s));
+            HOST_CALL(salloc(&dp.gred_node, std::max<size_t>(1, gp.gnode.size()), s));
+            HOST_CALL(salloc(&dp.gred_own, std::max<size_t>(1, gp.gown.size()), s));
+            HOST_CALL(salloc(&dp.gred_slot, std::max<size_t>(1, gp.gslot.size()), s));
+            HOST_CALL(salloc(&dp.side, (size_t)dp.nslots * dp.Mp, s));
+            HIP_CALL(hipMemcpyAsync(dp.gtile_ptr, gp.ptr.data(), gp.ptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            HIP_CALL(hipMemcpyAsync(dp.gtile_entries, gp.entries.data(), gp.entries.size() * sizeof(int),
+                                    hipMemcpyHostToDevice, s));
+            HIP_CALL(hipMemcpyAsync(dp.gtile_slots, gp.slots.data(), gp.slots.size() * sizeof(int),
+                                    hipMemcpyHostToDevice, s));
+            HIP_CALL(hipMemcpyAsync(dp.gred_ptr, gp.gptr.data(), gp.gptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            if (!gp.gnode.empty()) {
+                HIP_CALL(hipMemcpyAsync(dp.gred_node, gp.gnode.data(), gp.gnode.size() * sizeof(int),
+                                        hipMemcpyHostToDevice, s));
+                HIP_CALL(hipMemcpyAsync(dp.gred_own, gp.gown.data(), gp.gown.size() * sizeof(int),
+                                        hipMemcpyHostToDevice, s));
+                HIP_CALL(hipMemcpyAsync(dp.gred_slot, gp.gslot.data(), gp.gslot.size() * sizeof(int),
+                                        hipMemcpyHostToDevice, s));
+            }
+            HIP_CALL(hipStreamSynchronize(s));  // gp goes out of scope
+        }
+    }
     dp.impedance = 0;
     for (const auto &e : m.elems)
         if (e.bc[0].x != 0.f || e.bc[0].y != 0.f) dp.impedance = 1;
@@ -1097,6 +1171,16 @@ void device_problem_free(DeviceProblem &dp) {
     sfree(dp.tile_ptr, dp.st);
     sfree(dp.tile_entries, dp.st);
     sfree(dp.isolated, dp.st);
+    sfree(dp.gtile_ptr, dp.st);
+    sfree(dp.gtile_entries, dp.st);
+    sfree(dp.gtile_slots, dp.st);
+    sfree(dp.gred_ptr, dp.st);
+    sfree(dp.gred_node, dp.st);
+    sfree(dp.gred_own, dp.st);
+    sfree(dp.gred_slot, dp.st);
+    sfree(dp.side, dp.st);
+    dp.use_ghost = 0;
+    dp.nslots = dp.ngnodes = 0;
     sfree(dp.cpart, dp.st);
     sfree(dp.bpart, dp.st);
     sfree(dp.cpart64, dp.st);
@@ -1176,6 +1260,9 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double
         a.q.xi2[q] = sizeof(R) == 4 ? (R)qt.xi2[q] : (R)qt.dxi2[q];
         a.q.w[q] = sizeof(R) == 4 ? (R)qt.w[q] : (R)qt.dw[q];
     }
+    a.tile_slots = nullptr;
+    a.side = nullptr;
+    bool ghost = false;
     {
         TimedRegion tr;
         timing_begin(tr, sizeof(R) == 4 ? "asm_tile_f32" : "asm_tile_f64", s);
@@ -1189,13 +1276,27 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double
             // distance (at most the bounding-box diagonal of the points) keeps
             // it below that
             const bool raw = kd / (2.0 * kPi) * dp.diam < 250.0;
-            if (dp.impedance) {
-                if (raw) asm_tile_fast<true, true><<<grid, block, 0, s>>>(*(AsmArgs<float> *)&a);
-                else asm_tile_fast<true, false><<<grid, block, 0, s>>>(*(AsmArgs<float> *)&a);
-            } else {
-                if (raw) asm_tile_fast<false, true><<<grid, block, 0, s>>>(*(AsmArgs<float> *)&a);
-                else asm_tile_fast<false, false><<<grid, block, 0, s>>>(*(AsmArgs<float> *)&a);
+            ghost = dp.use_ghost && !getenv("BEM_ASM_HALO");
+            AsmArgs<float> &af = *(AsmArgs<float> *)&a;
+            if (ghost) {
+                af.tile_ptr = dp.gtile_ptr;
+                af.tile_entries = dp.gtile_entries;
+                af.tile_slots = dp.gtile_slots;
+                af.side = dp.side;
             }
+#define BEM_FAST(IMP_, RAW_)                                                          \
+    do {                                                                              \
+        if (ghost) asm_tile_fast<IMP_, RAW_, true><<<grid, block, 0, s>>>(af);       \
+        else asm_tile_fast<IMP_, RAW_, false><<<grid, block, 0, s>>>(af);            \
+    } while (0)
+            if (dp.impedance) {
+                if (raw) BEM_FAST(true, true);
+                else BEM_FAST(true, false);
+            } else {
+                if (raw) BEM_FAST(false, true);
+                else BEM_FAST(false, false);
+            }
+#undef BEM_FAST
         } else {
             if (dp.impedance) asm_tile<R, 1, true><<<grid, block, 0, s>>>(a);
             else asm_tile<R, 1, false><<<grid, block, 0, s>>>(a);
@@ -1204,6 +1305,15 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double
         timing_end(tr, s);
     }
     const int nrows = dp.M - r0;
+    if (ghost && dp.ngnodes > 0) {
+        TimedRegion tr;
+        timing_begin(tr, "ghost_reduce", s);
+        dim3 grid((nrows + 255) / 256, dp.ngnodes);
+        ghost_reduce<<<grid, 256, 0, s>>>((float2 *)A, ldA, nrows, dp.side, dp.Mp, dp.gred_ptr, dp.gred_node,
+                                          dp.gred_own, dp.gred_slot);
+        HIP_CALL(hipGetLastError());
+        timing_end(tr, s);
+    }
     if (dp.n_isolated > 0) {
         dim3 grid((nrows + 255) / 256, dp.n_isolated);
         zero_columns<C2><<<grid, 256, 0, s>>>(A, ldA, nrows, dp.isolated, dp.n_isolated);

@@ -106,6 +106,24 @@ struct TilePlan {
     std::vector<int> isolated; // nodes referenced by no element
 };
 void build_tile_plan(const Mesh &m, int P, TilePlan &plan);
+// The same column tiles with every element integrated once, by the tile that
+// owns most of its vertices (else the tile of vertex 0).  A vertex owned by
+// another tile is a ghost: its contributions go to a side column of its own
+// per (tile, node), and ghost_reduce adds the side columns of each node to A
+// in a fixed order.
+struct GhostPlan {
+    int P = 0;
+    std::vector<int> ptr;      // P + 1
+    std::vector<int> entries;  // element index << 8 | flags (bits 0-2 owned vertex, 3-5 first touch)
+    std::vector<int> slots;    // 3 per entry: side column of a ghost vertex, -1 for an owned one
+    int nslots = 0;
+    std::vector<int> gptr;     // per ghost node: its side columns gslot[gptr[g] .. gptr[g + 1])
+    std::vector<int> gnode;    // the node
+    std::vector<int> gown;     // 1 when the node's own tile also writes its column
+    std::vector<int> gslot;
+    double dup = 0;            // entries of the halo plan per element (for the choice of plan)
+};
+void build_ghost_plan(const Mesh &m, int P, GhostPlan &g);
 
 // Element record read by the assembly kernels with scalar loads (128 B):
 //   p3, e1 = p1-p3, e2 = p2-p3, n (unit normal of e1 x e2), J = |e1 x e2|,
@@ -149,6 +167,12 @@ struct DeviceProblem {
     int *tile_entries = nullptr;
     int n_isolated = 0;
     int *isolated = nullptr;
+    // element-once plan of the FP32 fast kernel (GhostPlan), when it pays
+    int use_ghost = 0;
+    int *gtile_ptr = nullptr, *gtile_entries = nullptr, *gtile_slots = nullptr;
+    int nslots = 0, ngnodes = 0;
+    int *gred_ptr = nullptr, *gred_node = nullptr, *gred_own = nullptr, *gred_slot = nullptr;
+    float2 *side = nullptr;       // nslots * Mp side columns
     float *cpart = nullptr;       // P * Mp  c partial sums
     float2 *bpart = nullptr;      // P * Mp  boundary-condition partial sums
     double *cpart64 = nullptr;

@@ -655,4 +655,100 @@ void build_tile_plan(const Mesh &m, int P, TilePlan &plan) {
         if (!used[i]) plan.isolated.push_back(i);
 }
 
+void build_ghost_plan(const Mesh &m, int P, GhostPlan &g) {
+    const int N = m.N(), E = m.E();
+    P = std::max(1, std::min(P, N));
+    std::vector<int> ids(N), part(N, 0);
+    std::iota(ids.begin(), ids.end(), 0);
+    rcb(m, ids, 0, N, P, 0, part);
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = 0; i < N; i++)
+        for (int d = 0; d < 3; d++) {
+            mn[d] = std::min(mn[d], m.pnts[3 * i + d]);
+            mx[d] = std::max(mx[d], m.pnts[3 * i + d]);
+        }
+    std::vector<uint64_t> key(N);
+    for (int i = 0; i < N; i++) {
+        uint64_t k = 0;
+        for (int d = 0; d < 3; d++) {
+            double u = (m.pnts[3 * i + d] - mn[d]) / std::max(1e-30f, mx[d] - mn[d]);
+            k |= spread3((uint64_t)(u * 2097151.0)) << d;
+        }
+        key[i] = k;
+    }
+    std::vector<std::vector<std::pair<uint64_t, int>>> lists(P);
+    long halo = 0;
+    for (int e = 0; e < E; e++) {
+        const int *nd = m.elems[e].nodes;
+        const int pe[3] = {part[nd[0]], part[nd[1]], part[nd[2]]};
+        halo += 1 + (pe[1] != pe[0]) + (pe[2] != pe[0] && pe[2] != pe[1]);
+        const int owner = (pe[1] == pe[2]) ? pe[1] : pe[0];
+        uint64_t kmin = ~0ull;
+        for (int j = 0; j < 3; j++)
+            if (pe[j] == owner) kmin = std::min(kmin, key[nd[j]]);
+        lists[owner].push_back({kmin, e});
+    }
+    g.dup = E > 0 ? (double)halo / E : 1.0;
+    g.P = P;
+    g.ptr.assign(P + 1, 0);
+    g.entries.clear();
+    g.slots.clear();
+    g.entries.reserve(E);
+    g.slots.reserve(3 * (size_t)E);
+    g.nslots = 0;
+    std::vector<char> touched(N, 0);
+    std::vector<int> slot_node;           // node of each side column
+    std::vector<int> local(N, -1);        // side column of a node in the current tile
+    std::vector<int> local_nodes;
+    for (int p = 0; p < P; p++) {
+        auto &L = lists[p];
+        std::stable_sort(L.begin(), L.end(), [](const std::pair<uint64_t, int> &a, const std::pair<uint64_t, int> &b) {
+            return a.first < b.first;
+        });
+        for (auto &kv : L) {
+            const int e = kv.second;
+            int ent = (e << 8) | (1 << 6);
+            int sl[3];
+            for (int j = 0; j < 3; j++) {
+                const int nd = m.elems[e].nodes[j];
+                if (part[nd] == p) {
+                    ent |= 1 << j;
+                    sl[j] = -1;
+                    if (!touched[nd]) {
+                        touched[nd] = 1;
+                        ent |= 1 << (3 + j);
+                    }
+                } else {
+                    if (local[nd] < 0) {
+                        local[nd] = g.nslots++;
+                        slot_node.push_back(nd);
+                        local_nodes.push_back(nd);
+                        ent |= 1 << (3 + j);
+                    }
+                    sl[j] = local[nd];
+                }
+            }
+            g.entries.push_back(ent);
+            for (int j = 0; j < 3; j++) g.slots.push_back(sl[j]);
+        }
+        g.ptr[p + 1] = (int)g.entries.size();
+        for (int nd : local_nodes) local[nd] = -1;
+        local_nodes.clear();
+    }
+    // side columns per node in the order they were made (tile order)
+    std::vector<std::vector<int>> per(N);
+    for (int sidx = 0; sidx < g.nslots; sidx++) per[slot_node[sidx]].push_back(sidx);
+    g.gptr.assign(1, 0);
+    g.gnode.clear();
+    g.gown.clear();
+    g.gslot.clear();
+    for (int nd = 0; nd < N; nd++) {
+        if (per[nd].empty()) continue;
+        g.gnode.push_back(nd);
+        g.gown.push_back(touched[nd] ? 1 : 0);
+        for (int sidx : per[nd]) g.gslot.push_back(sidx);
+        g.gptr.push_back((int)g.gslot.size());
+    }
+}
+
 }  // namespace bem

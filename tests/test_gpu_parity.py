@@ -89,6 +89,46 @@ def test_assembly_fp32_phase_range(far, monkeypatch):
     assert err < 2e-5
 
 
+@pytest.mark.parametrize("imp", [False, True])
+def test_assembly_fp32_element_once_plan(imp, monkeypatch):
+    """The FP32 fast kernel's element-once plan: every element integrated by
+    one column tile, its vertices that other tiles own written to side columns
+    that ghost_reduce adds in a fixed order.  Forced on icosphere s3 with 12
+    tiles (where it is taken: 258 side columns, and 29 % of the elements
+    straddle tiles), it matches the oracle and the halo plan (every straddling
+    element integrated once per tile); with ca != 0 as well (the h - (ca/cb) g
+    path)."""
+    monkeypatch.setenv("BEM_QG", "1")
+    monkeypatch.setenv("BEM_TILES", "12")
+    m = _icosphere(3, nchief=8)
+    srcs = None
+    if imp:
+        m.set_bc(0.5 + 0.25j, 1.0 - 0.5j, 0.3 + 0.1j)
+        srcs = [[0.5, 0.2, 0.1]]
+    k = 36.6
+    print("case: icosphere s3 + 8 CHIEF, 12 tiles, k=%g, impedance %s" % (k, imp), flush=True)
+    bem.timing_reset()
+    bem.timing_enable(True)
+    A, B = bem.bem_system(m, k, srcs=srcs)
+    bem.sync()
+    bem.timing_enable(False)
+    _, n_ghost = bem.timing_get("ghost_reduce")
+    bem.timing_reset()
+    assert n_ghost >= 1, "the element-once plan was not taken"
+    monkeypatch.setenv("BEM_ASM_HALO", "1")
+    Ah, Bh = bem.bem_system(m, k, srcs=srcs)
+    p, e, bc = C.mesh_arrays(m)
+    pts = np.vstack([p, m.chief()]).astype(np.float64)
+    binit = np.zeros((pts.shape[0], 1)) if imp else None
+    Ao, Bo = O.assemble(k, pts, p.shape[0], e, bc, binit=binit, dens=RHO, speed=SPEED)
+    err, dh = _rel_max(A, Ao), _rel_max(A, Ah)
+    print("vs oracle %.3g, vs halo plan %.3g" % (err, dh))
+    assert err < 2e-5
+    assert dh < 1e-6
+    if imp:
+        assert _rel_max(B, Bo) < 2e-5
+
+
 def test_assembly_fp64_matches_oracle():
     m = _icosphere(2, nchief=4)
     s = bem.Solver(m)
