@@ -638,7 +638,24 @@ __global__ void asm_finalize(C2 *A, long ldA, int M, int N, int P, long Mp, int 
     const int row = r0 + i;
     if (row >= M) return;
     double cs = 0, br = 0, bi = 0;
-    for (int t = 0; t < P; t++) {
+    // 8 tiles' partials in flight per step, added in tile order
+    int t = 0;
+    for (; t + 8 <= P; t += 8) {
+        R c[8];
+        C2 b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            c[u] = cpart[(size_t)(t + u) * Mp + i];
+            b[u] = bpart[(size_t)(t + u) * Mp + i];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            cs += (double)c[u];
+            br += (double)b[u].x;
+            bi += (double)b[u].y;
+        }
+    }
+    for (; t < P; t++) {
         cs += (double)cpart[(size_t)t * Mp + i];
         const C2 b = bpart[(size_t)t * Mp + i];
         br += (double)b.x;
@@ -911,11 +928,13 @@ static long ghost_slot_bound(int N, int P) {
     return std::min((long)N / 2, (long)(4.0 * std::sqrt((double)N * P)) + P);
 }
 
-static int choose_tiles(int M, int N, int E, int rows_per_wave) {
+// wave_factor: the element-once plan has no halo to pay for more tiles, and
+// runs at twice the waves (icosphere s5: 5.56 ms with 102 tiles, 5.35 with 204)
+static int choose_tiles(int M, int N, int E, int rows_per_wave, int wave_factor = 1) {
     const char *env = getenv("BEM_TILES");
     if (env && atoi(env) > 0) return std::min(atoi(env), N);
     TileShape t;
-    const int target_waves = tuned_shape(M, E, t) ? t.waves : kDefaultWaves;
+    const int target_waves = (tuned_shape(M, E, t) ? t.waves : kDefaultWaves) * wave_factor;
     const int nrb = (M + rows_per_wave - 1) / rows_per_wave;
     int P = (target_waves + nrb - 1) / nrb;
     P = std::max(1, std::min(P, std::max(1, N / 8)));
@@ -1033,9 +1052,11 @@ size_t device_problem_bytes(int M, int N, int E) {
     b += (sizeof(float) + sizeof(float2) + sizeof(double) + sizeof(double2)) * P * Mp;  // tile partials
     if (qg == 1) {
         // element-once plan: its side columns and index arrays
-        const size_t gs = (size_t)ghost_slot_bound(N, (int)P);
+        const int gP = choose_tiles(M, N, E, 64, 2);
+        const size_t gs = (size_t)ghost_slot_bound(N, gP);
         b += sizeof(float2) * gs * Mp;
-        b += sizeof(int) * (P + 1 + 4 * (size_t)E + 2 * (size_t)N + 2 * gs);
+        if ((size_t)gP > P) b += (sizeof(float) + sizeof(float2)) * ((size_t)gP - P) * Mp;  // its tile partials
+        b += sizeof(int) * (gP + 1 + 4 * (size_t)E + 2 * (size_t)N + 2 * gs);
     }
     return b;
 }
@@ -1114,11 +1135,20 @@ int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
     dp.use_ghost = 0;
     if (qg == 1 && !getenv("BEM_NO_GHOST_PLAN")) {
         GhostPlan gp;
-        build_ghost_plan(m, dp.P, gp);
+        const int gP = choose_tiles(dp.M, dp.N, dp.E, 64, 2);
+        build_ghost_plan(m, gP, gp);
         // (ghost_reduce puts the ghost nodes on grid.y: at most 65535)
-        if (gp.P == dp.P && gp.dup >= 1.1 && gp.nslots > 0 && gp.nslots <= ghost_slot_bound(dp.N, dp.P) &&
+        if (gp.P == gP && gp.dup >= 1.1 && gp.nslots > 0 && gp.nslots <= ghost_slot_bound(dp.N, gP) &&
             gp.gnode.size() <= 65535) {
             dp.use_ghost = 1;
+            dp.gP = gP;
+            if (gP > dp.P) {
+                // the FP32 tile partials of c and the BC sums, one row per tile
+                sfree(dp.cpart, s);
+                sfree(dp.bpart, s);
+                HOST_CALL(salloc(&dp.cpart, (size_t)gP * dp.Mp, s));
+                HOST_CALL(salloc(&dp.bpart, (size_t)gP * dp.Mp, s));
+            }
             dp.nslots = gp.nslots;
             dp.ngnodes = (int)gp.gnode.size();
             HOST_CALL(salloc(&dp.gtile_ptr, gp.ptr.size(), s));
@@ -1279,6 +1309,8 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double
             ghost = dp.use_ghost && !getenv("BEM_ASM_HALO");
             AsmArgs<float> &af = *(AsmArgs<float> *)&a;
             if (ghost) {
+                af.nwg = af.nrbg * dp.gP;
+                grid = dim3(af.nwg);
                 af.tile_ptr = dp.gtile_ptr;
                 af.tile_entries = dp.gtile_entries;
                 af.tile_slots = dp.gtile_slots;
@@ -1322,7 +1354,8 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double
     {
         TimedRegion tr;
         timing_begin(tr, "asm_finalize", s);
-        asm_finalize<R, C2><<<(nrows + 255) / 256, 256, 0, s>>>(A, ldA, dp.M, dp.N, dp.P, dp.Mp, r0, a.cpart,
+        asm_finalize<R, C2><<<(nrows + 255) / 256, 256, 0, s>>>(A, ldA, dp.M, dp.N, ghost ? dp.gP : dp.P, dp.Mp, r0,
+                                                             a.cpart,
                                                              a.bpart, bterm);
         HIP_CALL(hipGetLastError());
         timing_end(tr, s);

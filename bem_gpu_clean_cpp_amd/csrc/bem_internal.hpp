@@ -169,6 +169,7 @@ struct DeviceProblem {
     int *isolated = nullptr;
     // element-once plan of the FP32 fast kernel (GhostPlan), when it pays
     int use_ghost = 0;
+    int gP = 0;                   // its column tiles
     int *gtile_ptr = nullptr, *gtile_entries = nullptr, *gtile_slots = nullptr;
     int nslots = 0, ngnodes = 0;
     int *gred_ptr = nullptr, *gred_node = nullptr, *gred_own = nullptr, *gred_slot = nullptr;
