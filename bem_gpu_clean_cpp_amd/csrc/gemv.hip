@@ -302,15 +302,15 @@ int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 //   transpose_rm_ahb_f32  Ar[i * ldr + j] = A[j * lda + i] for j < N, zero
 //                     for N <= j < ldr (the padding is read by the pair
 //                     loads), and A^H b from the same pass;
-//   ne_apply_part     kNeBlocks workgroups of NT = 1024 (768 for rows of more
-//                     than 4096 columns: 12 waves leave 168 registers a lane)
-//                     threads, each over a contiguous range of rows.  Thread
+//   ne_apply_part     kNeBlocks workgroups of NT = 1024 threads (4 waves per
+//                     SIMD), each over a contiguous range of rows.  Thread
 //                     t owns the column pairs t + NT k (16-byte loads, 1 KiB
 //                     per wave per row) and its partial of w in registers; v
-//                     sits in LDS.  Rows go in batches of RB: the batch's
-//                     loads are all in flight together, then one reduction
-//                     and one barrier (on LDS only) for all RB dots; the
-//                     reduction slots alternate by batch parity;
+//                     sits in LDS.  Rows go in batches of RB (2 while a
+//                     thread has at most 3 pairs, else 1): the batch's loads
+//                     are all in flight together, then one reduction and one
+//                     barrier (on LDS only) for all RB dots; the reduction
+//                     slots alternate by batch parity;
 //   ne_reduce         w = sum of the block partials in a fixed order, in FP64.
 constexpr int kNeBlocks = 256;
 // column pairs a row may have: 1024 threads x 6 pairs for the product,
