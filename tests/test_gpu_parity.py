@@ -129,6 +129,43 @@ def test_assembly_fp32_element_once_plan(imp, monkeypatch):
         assert _rel_max(B, Bo) < 2e-5
 
 
+def test_assembly_fp32_element_once_plan_at_s5(monkeypatch):
+    """At icosphere s5 + 16 CHIEF (10258 x 10242) the default launch takes the
+    element-once plan (204 column tiles, 4348 side columns).  Four blocks of
+    64 rows, one of them over the CHIEF rows, match the oracle's FP64 rows
+    (2e-5 of max |A|) and the halo plan's rows (1e-6)."""
+    for v in ("BEM_QG", "BEM_TILES", "BEM_ASM_HALO", "BEM_GENERIC_TILE", "BEM_NO_GHOST_PLAN"):
+        monkeypatch.delenv(v, raising=False)
+    m = _icosphere(5, nchief=16)
+    k = 2 * np.pi * 2000.0 / SPEED
+    N, M = m.num_pnts, m.num_pnts + m.num_chief
+    print("case: icosphere s5 + 16 CHIEF (M %d, N %d), k=%g" % (M, N, k), flush=True)
+    s = bem.Solver(m)
+    bem.timing_reset()
+    bem.timing_enable(True)
+    s.assemble(k, bem.PREC_FP32)
+    bem.sync()
+    bem.timing_enable(False)
+    _, n_ghost = bem.timing_get("ghost_reduce")
+    bem.timing_reset()
+    assert n_ghost >= 1, "the element-once plan was not taken"
+    starts = (0, 3000, 6100, M - 64)
+    rows = np.concatenate([np.arange(r, r + 64) for r in starts])
+    a = s.rows(rows)
+    monkeypatch.setenv("BEM_ASM_HALO", "1")
+    s.assemble(k, bem.PREC_FP32)
+    ah = s.rows(rows)
+    s.close()
+    p, e, bc = C.mesh_arrays(m)
+    pts = np.vstack([p, m.chief()]).astype(np.float64)
+    ao = np.vstack([O.assemble_rows(k, pts, N, e, bc, r, r + 64, dens=RHO, speed=SPEED, dtype=np.float64,
+                                    nthreads=min(16, os.cpu_count() or 1)) for r in starts])
+    err, dh = _rel_max(a, ao), _rel_max(a, ah)
+    print("rows vs oracle %.3g, vs halo plan %.3g" % (err, dh))
+    assert err < 2e-5
+    assert dh < 1e-6
+
+
 def test_assembly_fp64_matches_oracle():
     m = _icosphere(2, nchief=4)
     s = bem.Solver(m)
