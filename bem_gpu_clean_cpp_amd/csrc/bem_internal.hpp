@@ -247,10 +247,12 @@ int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 // multi-right-hand-side products (gemm.hip)
 size_t ws_gemm(long M, long N, int R);
 size_t ws_gemm_h_f32(long M, long N, int R);
+// long_chunks: fewer, longer FP32 column (row) chunks, faster for products a
+// refinement corrects; the default keeps the FP32 sums short (QR, field)
 int gemm_n_f32(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, float2 *Y, long ldy, int R,
-               void *ws, hipStream_t s);
+               void *ws, hipStream_t s, bool long_chunks = false);
 int gemm_h_f32(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R,
-               void *ws, hipStream_t s);
+               void *ws, hipStream_t s, bool long_chunks = false);
 int gemm_n_f32z(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, double2 *Y, long ldy, int R,
                 void *ws, hipStream_t s);
 int gemm_h_f32z(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R,

@@ -226,7 +226,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     auto prod_n = [&]() -> int {
         if (f32) {
             if (R == 1) HOST_CALL(gemv_n_f32(A32, lda, M, N, w.p32, 0, w.q32, 0, 1, w.gws, s));
-            else HOST_CALL(gemm_n_f32(A32, lda, M, N, w.p32, ld, w.q32, ld, R, w.gws, s));
+            else HOST_CALL(gemm_n_f32(A32, lda, M, N, w.p32, ld, w.q32, ld, R, w.gws, s, true));
         } else {
             for (int q = 0; q < R; q++)
                 HOST_CALL(gemv_n_f64(A64, lda, M, N, w.p + (size_t)q * ld, 0, w.q64 + (size_t)q * ld, 0, 1, w.gws,
@@ -238,7 +238,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     auto prod_h = [&]() -> int {
         if (f32) {
             if (R == 1) HOST_CALL(gemv_h_f32(A32, lda, M, N, w.q32, 0, w.w32, 0, 1, s));
-            else HOST_CALL(gemm_h_f32(A32, lda, M, N, w.q32, ld, w.w32, ld, R, w.gws, s));
+            else HOST_CALL(gemm_h_f32(A32, lda, M, N, w.q32, ld, w.w32, ld, R, w.gws, s, true));
         } else {
             for (int q = 0; q < R; q++)
                 HOST_CALL(gemv_h_f64(A64, lda, M, N, w.q64 + (size_t)q * ld, 0, w.w64 + (size_t)q * ld, 0, 1, s));

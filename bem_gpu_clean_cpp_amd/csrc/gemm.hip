@@ -315,16 +315,26 @@ __global__ __launch_bounds__(256) void gemm_reduce(const typename MF<T>::c2 *__r
     }
 }
 
-static void gemm_n_shape(long M, long N, int &nch, int &kc) {
+// column chunks: max(3, 1024 / row tiles).  Fewer, longer chunks than for
+// 4096 workgroups: measured Y = A X at R = 16, M = 2562: 25.3 us with 41 chunks,
+// 17.3 with 12; at R = 32, M = 10258: 364 us with 12, 252 with 3; at
+// M = 40994 three chunks (5.4 ms) beat one (8.2 ms)
+// (long_chunks: for products whose rounding a refinement absorbs; the short
+// rule, chunks for 4096 workgroups, keeps FP32 sums short where the product
+// is final, as in the QR updates and the field operator)
+static void gemm_n_shape(long M, long N, int &nch, int &kc, bool long_chunks) {
     const long tiles = (M + 31) / 32;
-    long target = std::max(1L, 4096 / std::max(1L, tiles));
+    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
     kc = (int)std::max(32L, ((N + target - 1) / target + 31) / 32 * 32);
     nch = (int)((N + kc - 1) / kc);
 }
 
-static void gemm_h_shape(long M, long N, int &nch, int &kr) {
+// row chunks: max(3, 1024 / column tiles), as for gemm_n (measured Z = A^H T at
+// R = 32: M = 10258, 334 -> 321 us; M = 40994, 5.81 -> 5.29 ms against chunks
+// for 4096 workgroups)
+static void gemm_h_shape(long M, long N, int &nch, int &kr, bool long_chunks) {
     const long tiles = (N + 15) / 16;
-    long target = std::max(1L, 4096 / std::max(1L, tiles));
+    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
     kr = (int)std::max(64L, ((M + target - 1) / target + 63) / 64 * 64);
     nch = (int)((M + kr - 1) / kr);
 }
@@ -332,26 +342,31 @@ static void gemm_h_shape(long M, long N, int &nch, int &kr) {
 // partials of the FP32 adjoint product alone (Z = A^H T, N x R)
 size_t ws_gemm_h_f32(long M, long N, int R) {
     int nch, kr;
-    gemm_h_shape(M, N, nch, kr);
+    gemm_h_shape(M, N, nch, kr, false);
     return (size_t)nch * (size_t)((N + 63) / 64 * 64) * (size_t)std::max(1, R) * sizeof(float2);
 }
 
 // partials of either product at FP64 for R right-hand sides
 size_t ws_gemm(long M, long N, int R) {
-    int a, b, c, d;
-    gemm_n_shape(M, N, a, b);
-    gemm_h_shape(M, N, c, d);
+    int nch = 0;
+    for (bool lc : {false, true}) {
+        int a, b, c, d;
+        gemm_n_shape(M, N, a, b, lc);
+        gemm_h_shape(M, N, c, d, lc);
+        nch = std::max({nch, a, c});
+    }
     const long ldp = (std::max(M, N) + 63) / 64 * 64;
-    return (size_t)std::max(a, c) * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
+    return (size_t)nch * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
 }
 
 template <typename T, typename S = T, typename OC2 = typename MF<T>::c2>
 static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *X, long ldx,
-                       const OC2 *b, long ldb, OC2 *Y, long ldy, int R, void *ws, const char *name, hipStream_t s) {
+                       const OC2 *b, long ldb, OC2 *Y, long ldy, int R, void *ws, const char *name, hipStream_t s,
+                       bool long_chunks) {
     if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
     if (lda % 2 || lda < (M + 31) / 32 * 32) BEM_FAIL("gemm_n needs an even lda >= roundup(M, 32)");
     int nch, kc;
-    gemm_n_shape(M, N, nch, kc);
+    gemm_n_shape(M, N, nch, kc, long_chunks);
     const long ldp = (M + 63) / 64 * 64;
     auto *part = (typename MF<T>::c2 *)ws;
     TimedRegion tr;
@@ -374,11 +389,11 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
 
 template <typename T, typename S = T, typename OC2 = typename MF<T>::c2>
 static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *Tv, long ldt, OC2 *Z,
-                       long ldz, int R, void *ws, const char *name, hipStream_t s) {
+                       long ldz, int R, void *ws, const char *name, hipStream_t s, bool long_chunks) {
     if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
     if (lda % 2) BEM_FAIL("gemm_h needs an even lda");
     int nch, kr;
-    gemm_h_shape(M, N, nch, kr);
+    gemm_h_shape(M, N, nch, kr, long_chunks);
     const long ldp = (N + 63) / 64 * 64;
     auto *part = (typename MF<T>::c2 *)ws;
     TimedRegion tr;
@@ -401,38 +416,40 @@ static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename
 
 // Y (M x R, ldy) = A X ; ws >= ws_gemm(M, N, R)
 int gemm_n_f32(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, float2 *Y, long ldy, int R,
-               void *ws, hipStream_t s) {
-    return gemm_n_impl<float>(A, lda, M, N, X, ldx, (const float2 *)nullptr, 0, Y, ldy, R, ws, "gemm_n_f32", s);
+               void *ws, hipStream_t s, bool long_chunks) {
+    return gemm_n_impl<float>(A, lda, M, N, X, ldx, (const float2 *)nullptr, 0, Y, ldy, R, ws, "gemm_n_f32", s,
+                              long_chunks);
 }
 
 // Z (N x R, ldz) = A^H T, T: M x R (ldt); ws >= ws_gemm(M, N, R)
 int gemm_h_f32(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R,
-               void *ws, hipStream_t s) {
-    return gemm_h_impl<float>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "gemm_h_f32", s);
+               void *ws, hipStream_t s, bool long_chunks) {
+    return gemm_h_impl<float>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "gemm_h_f32", s, long_chunks);
 }
 
-// FP32 products whose fixed-order reduce writes FP64 vectors (the Krylov basis input)
+// FP32 products whose fixed-order reduce writes FP64 vectors (the Krylov basis
+// input; refined, so long chunks)
 int gemm_n_f32z(const float2 *A, long lda, long M, long N, const float2 *X, long ldx, double2 *Y, long ldy, int R,
                 void *ws, hipStream_t s) {
     return gemm_n_impl<float, double, double2>(A, lda, M, N, X, ldx, (const double2 *)nullptr, 0, Y, ldy, R, ws,
-                                               "gemm_n_f32", s);
+                                               "gemm_n_f32", s, true);
 }
 int gemm_h_f32z(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R,
                 void *ws, hipStream_t s) {
-    return gemm_h_impl<float, double, double2>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "gemm_h_f32", s);
+    return gemm_h_impl<float, double, double2>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "gemm_h_f32", s, true);
 }
 
 // out (M x R) = b - A X (b == nullptr: out = A X): FP32 A widened to FP64,
 // FP64 vectors, FP64 MFMA accumulation; ws >= ws_gemm(M, N, R)
 int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long ldx, const double2 *b, long ldb,
               double2 *out, long ldo, int R, void *ws, hipStream_t s) {
-    return gemm_n_impl<double>(A, lda, M, N, X, ldx, b, ldb, out, ldo, R, ws, "mres_f64acc", s);
+    return gemm_n_impl<double>(A, lda, M, N, X, ldx, b, ldb, out, ldo, R, ws, "mres_f64acc", s, true);
 }
 
 // Z (N x R) = A^H T with FP64 accumulation; ws >= ws_gemm(M, N, R)
 int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
             void *ws, hipStream_t s) {
-    return gemm_h_impl<double>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "mh_f64acc", s);
+    return gemm_h_impl<double>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "mh_f64acc", s, true);
 }
 
 // ------------------------------------------------ QR rank-nb update ---

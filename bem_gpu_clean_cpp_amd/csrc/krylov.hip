@@ -423,7 +423,7 @@ struct Operator {
                 if (!normal()) {
                     HOST_CALL(gemm_n_f32z(A32, lda, M, N, ws.v32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
                 } else {
-                    HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.t32, ws.ldv, R, ws.gws, s));
+                    HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.t32, ws.ldv, R, ws.gws, s, true));
                     HOST_CALL(gemm_h_f32z(A32, lda, M, N, ws.t32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
                 }
             }
