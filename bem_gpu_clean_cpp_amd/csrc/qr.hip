@@ -1186,6 +1186,8 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
     // FP32: trailing updates on MFMA (Z = C^H V with the adjoint product, then
     // C -= V W with the rank-32 update); FP64 keeps the VALU kernels
     const bool mfma = sizeof(T) == 4 && !getenv("BEM_QR_VALU");
+    const bool global_panel = getenv("BEM_QR_GPANEL") != nullptr;
+    const bool one_wg_panel = getenv("BEM_QR_ONE_WG_PANEL") != nullptr;
     if (mfma) {
         f.ldv = (f.lda + 63) / 64 * 64;
         HOST_CALL(salloc(&f.Vd, (size_t)f.ldv * kNB, s));
@@ -1211,7 +1213,7 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
         // panels of up to 1024 rows: the register kernel also forms T and the
         // explicit V; taller ones (and a narrow last panel) run the global
         // panel kernel, then V, V^H V on MFMA and T from it
-        const bool fit = mfma && nb == kNB && !getenv("BEM_QR_GPANEL");
+        const bool fit = mfma && nb == kNB && !global_panel;
         const int reg = !fit ? 0 : Mp <= 256 ? 1 : Mp <= 512 ? 2 : Mp <= 1024 ? 3 : 0;
         float2 *Tf = (float2 *)Tp, *Vf = (float2 *)f.Vd;
         if (reg == 1) {
@@ -1220,7 +1222,7 @@ static int qr_factor(QRFactor<T> &f, int extra_cols, hipStream_t s) {
             qr_panel_reg<512, 1><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, Tf, Vf, f.ldv);
         } else if (reg == 3) {
             qr_panel_reg<512, 2><<<1, 512, 0, s>>>((float2 *)f.A, f.lda, f.M, j0, Tf, Vf, f.ldv);
-        } else if (fit && !getenv("BEM_QR_ONE_WG_PANEL")) {
+        } else if (fit && !one_wg_panel) {
             // tall panel: one launch per column over kColThreads-row workgroups
             const int nblk = (Mp + kColThreads - 1) / kColThreads;
             for (int jj = -1; jj < kNB; jj++) {
