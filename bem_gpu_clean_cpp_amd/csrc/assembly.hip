@@ -615,18 +615,32 @@ __global__ void zero_columns(C2 *A, long ldA, int M, const int *__restrict__ col
 __global__ void ghost_reduce(float2 *A, long ldA, int nrows, const float2 *__restrict__ side, long Mp,
                              const int *__restrict__ gptr, const int *__restrict__ gnode,
                              const int *__restrict__ gown, const int *__restrict__ gslot) {
+    // two adjacent rows per thread (16-byte accesses: ldA and Mp are even)
     const int g = blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= nrows) return;
     float2 *col = A + (size_t)gnode[g] * ldA;
-    float2 acc = gown[g] ? col[i] : make_float2(0.f, 0.f);
     const int s0 = gptr[g], s1 = gptr[g + 1];
-    for (int t = s0; t < s1; t++) {
-        const float2 v = side[(size_t)gslot[t] * Mp + i];
-        acc.x += v.x;
-        acc.y += v.y;
+    const bool own = gown[g] != 0;
+    if (i + 1 < nrows) {
+        float4 acc = own ? *(const float4 *)(col + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int t = s0; t < s1; t++) {
+            const float4 v = *(const float4 *)(side + (size_t)gslot[t] * Mp + i);
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+        }
+        *(float4 *)(col + i) = acc;
+    } else {
+        float2 acc = own ? col[i] : make_float2(0.f, 0.f);
+        for (int t = s0; t < s1; t++) {
+            const float2 v = side[(size_t)gslot[t] * Mp + i];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+        col[i] = acc;
     }
-    col[i] = acc;
 }
 
 // Per row: fixed-order sum of the tile partials in FP64.
@@ -1350,9 +1364,10 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double
     }
     const int nrows = dp.M - r0;
     if (ghost && dp.ngnodes > 0) {
+        if (ldA % 2 || dp.Mp % 2) BEM_FAIL("ghost_reduce needs even leading dimensions (ldA %ld, Mp %ld)", ldA, dp.Mp);
         TimedRegion tr;
         timing_begin(tr, "ghost_reduce", s);
-        dim3 grid((nrows + 255) / 256, dp.ngnodes);
+        dim3 grid((nrows + 511) / 512, dp.ngnodes);
         ghost_reduce<<<grid, 256, 0, s>>>((float2 *)A, ldA, nrows, dp.side, dp.Mp, dp.gred_ptr, dp.gred_node,
                                           dp.gred_own, dp.gred_slot);
         HIP_CALL(hipGetLastError());
