@@ -462,8 +462,9 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (--gpus %d requested); the path shards by "
-              "collocation rows or by request, see the design notes" % args.gpus, file=sys.stderr)
+        print("bench.py: measures one GPU only (--gpus %d requested); across GPUs the solver would "
+              "shard by collocation rows (one large system) or by frequency and request (many small "
+              "ones)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import bem_gpu_clean_cpp_amd as bem
     import bemcases as C

@@ -5,7 +5,8 @@ Tolerances:
     quadrature; the reference computes in FP32 with fast-math);
   * FP64 assembly vs FP64 oracle: max |dA| <= 1e-11 * max |A|;
   * FP32 solves with FP64 refinement vs FP64 oracle solution: relative 2-norm
-    error <= 1e-5 (the north-star accuracy bar);
+    error <= 1e-5 (the accuracy a surface-pressure solve must reach against
+    the FP64 solution);
   * repeat assemblies are bitwise identical (no atomics).
 """
 import os
