@@ -14,6 +14,8 @@ namespace bem {
 
 // ---------------------------------------------------------------- errors ---
 void set_error(const char *file, int line, const char *fmt, ...);
+// a callee failed and has set its message: prefix it with this call site
+void chain_error(const char *file, int line, const char *what);
 
 #define BEM_FAIL(...)                                       \
     do {                                                    \
@@ -33,7 +35,7 @@ void set_error(const char *file, int line, const char *fmt, ...);
 #define HOST_CALL(x)                                                     \
     do {                                                                 \
         if ((x) != EXIT_SUCCESS) {                                       \
-            ::bem::set_error(__FILE__, __LINE__, "call failed: %s", #x); \
+            ::bem::chain_error(__FILE__, __LINE__, #x);                  \
             return EXIT_FAILURE;                                         \
         }                                                                \
     } while (0)

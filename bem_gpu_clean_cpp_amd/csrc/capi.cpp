@@ -184,13 +184,14 @@ int bem_set_source_strength(float q) {
 static int system_impl(bem_mesh *mh, float k, const bem_cart_coord *srcs, int numSrcs, bem_complex *A, int lda,
                        bem_complex *B, int ldb) {
     std::lock_guard<std::mutex> lk(g_mu);
-    hipStream_t s;
-    HOST_CALL(stream(&s));
+    // arguments first: a bad call fails without touching the device
     HOST_CALL(validate_mesh(mh->m));
     const int M = mh->m.N() + mh->m.C(), N = mh->m.N();
     if (!A || lda < M) BEM_FAIL("lda (%d) must be >= M (%d)", lda, M);
     if (numSrcs < 0 || (numSrcs > 0 && (!B || !srcs || ldb < M))) BEM_FAIL("bad B / sources (ldb %d, M %d)", ldb, M);
     if (numSrcs > kMaxRhs) BEM_FAIL("at most %d sources", kMaxRhs);
+    hipStream_t s;
+    HOST_CALL(stream(&s));
     HOST_CALL(mesh_device(mh, s));
     DeviceProblem &dp = mh->dp;
     void *pa, *pb;

@@ -18,6 +18,11 @@ namespace bem {
 // ---------------------------------------------------------------- errors ---
 static thread_local std::string g_last_error;
 
+static bool quiet_errors() {
+    static const bool quiet = getenv("BEM_QUIET") && atoi(getenv("BEM_QUIET")) != 0;
+    return quiet;
+}
+
 void set_error(const char *file, int line, const char *fmt, ...) {
     char msg[1024];
     va_list ap;
@@ -30,8 +35,17 @@ void set_error(const char *file, int line, const char *fmt, ...) {
     g_last_error = full;
     // The reference prints "Error at file:line" before returning EXIT_FAILURE
     // (numerical.h:60-122); BEM_QUIET=1 silences it.
-    static const bool quiet = getenv("BEM_QUIET") && atoi(getenv("BEM_QUIET")) != 0;
-    if (!quiet) fprintf(stderr, "Error at %s:%d: %s\n", base ? base + 1 : file, line, msg);
+    if (!quiet_errors()) fprintf(stderr, "Error at %s:%d: %s\n", base ? base + 1 : file, line, msg);
+}
+// Every failing path sets a message before it returns, so the callee's
+// message is the current one; the caller's site goes in front of it
+// ("capi.cpp:188: validate_mesh(mh->m) failed: host.cpp:246: Not enough ...").
+void chain_error(const char *file, int line, const char *what) {
+    const char *base = strrchr(file, '/');
+    char head[512];
+    snprintf(head, sizeof head, "%s:%d: %s failed", base ? base + 1 : file, line, what);
+    g_last_error = g_last_error.empty() ? std::string(head) : std::string(head) + ": " + g_last_error;
+    if (!quiet_errors()) fprintf(stderr, "Error at %s\n", head);
 }
 const char *last_error() { return g_last_error.c_str(); }
 void clear_error() { g_last_error.clear(); }

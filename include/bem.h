@@ -15,7 +15,9 @@
  *   - the caller owns every host buffer; nothing keeps the caller's pointers;
  *   - functions return EXIT_SUCCESS (0) or EXIT_FAILURE (1) and print
  *     "Error at file:line" on failure (numerical.h:60-122); bem_last_error()
- *     returns the message of the last failure in this thread;
+ *     returns the message of the last failure in this thread, each call
+ *     site in front of the message of the call that failed inside it;
+ *     arguments are checked before any device work;
  *   - time convention e^{+i omega t}, G = exp(-ikr)/(4 pi r) (numerical.cu:70-74);
  *   - air density 1.2041 and speed of sound 343.21 by default (numerical.cu:12,14),
  *     the source strength of the incident point source is 0 by default

@@ -87,6 +87,38 @@ def test_mesh_errors_are_reported():
         m.set_bc(0, 0, 1)  # cb = 0 would divide by zero
 
 
+def test_reference_entry_points_reject_bad_arguments():
+    """The reference entry points return EXIT_FAILURE with a message on bad
+    arguments (numerical.h:60-122), and do so before any device work, so these
+    calls run without a GPU."""
+    L = bem.lib()
+    m = bem.Mesh().icosphere(1, 1.0)  # N = 42, no CHIEF points, so M = 42
+    A = np.zeros((42, 42), np.complex64)
+    B = np.zeros(42, np.complex64)
+    src = np.zeros(3, np.float32)
+    P = bem._ptr
+
+    def fails(rc, pattern):
+        assert rc == 1
+        assert re.search(pattern, bem.last_error()), bem.last_error()
+
+    # bemSystem (mesh.cu:985-1006)
+    empty = bem.Mesh()  # held: a temporary would be destroyed before the call
+    fails(L.bem_system(empty.handle, 1.0, None, 0, P(A), 42, None, 42), r"Not enough mesh information! \(0 nodes")
+    fails(L.bem_system(m.handle, 1.0, None, 0, P(A), 41, None, 42), r"lda \(41\) must be >= M \(42\)")
+    fails(L.bem_system(m.handle, 1.0, P(src), 1, P(A), 42, P(B), 41), "bad B / sources")
+    fails(L.bem_system(m.handle, 1.0, P(src), -1, P(A), 42, P(B), 42), "bad B / sources")
+    # atomicGenSystem (atomicFuncs.cu:218-298)
+    fails(L.bem_gen_system(1.0, None, 0, None, 0, 0, None, 0, P(A), 42, P(B), 42), "bad system arguments")
+    # qrSolver (numerical.cu:580-644) and lsqSolver (numerical.cu:506-554) need more rows than columns
+    fails(L.bem_qr_solve(P(A), 4, 5, 4, P(B), 1, 4), "Senario not included yet")
+    fails(L.bem_qr_solve(P(A), 6, 5, 5, P(B), 1, 6), "bad QR arguments")
+    fails(L.bem_lsq_solve(P(A), 5, 5, 5, P(B), 1, 5, None), "lsqSolver needs m > n")
+    # GMRES (GMRES.cu:145-279)
+    fails(L.bem_gmres(P(A), P(B), 0, 10, 1e-6, P(B)), "bad GMRES arguments")
+    fails(L.bem_gmres(None, P(B), 42, 10, 1e-6, P(B)), "bad GMRES arguments")
+
+
 @pytest.mark.parametrize("s", [0, 1, 2, 3, 4])
 def test_icosphere(s):
     a = 0.1
