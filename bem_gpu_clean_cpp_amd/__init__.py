@@ -505,12 +505,9 @@ def solve_params(method=SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_to
     return bem_solve_params(method, restart, tol, maxit, refine_tol, max_refine)
 
 
-def batch_solve(mesh, ks, kind, params, amp=1.0, precision=PREC_FP32, workers=3, **solve_kw):
+def batch_solve(mesh, ks, kind, params, amp=1.0, precision=PREC_FP32, workers=4, **solve_kw):
     """One mesh at many wavenumbers.  params: (nrhs, 3) for the same right-hand
     sides at every frequency, or (nk, nrhs, 3) for one set per frequency.
-    workers: concurrent solvers (at most 8).  On a 2562-node mesh with 16
-    right-hand sides 3 finished 64 frequencies first (2: 0.141 s, 3: 0.134-0.142 s,
-    4: 0.145-0.153 s, 6: 0.17 s, 8: 0.18-0.19 s on one MI355X).
     Returns (X with shape (N, nrhs, nk) complex128, list of per-frequency stats)."""
     k = np.ascontiguousarray(ks, np.float32).ravel()
     nk = k.size

@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--plain-reps", type=int, default=20)
     ap.add_argument("--only", choices=["plain", "batching", "large", "serving"],
                     help="run one secondary workload alone and print its section (not the benchmark line)")
-    ap.add_argument("--workers", type=int, default=3, help="concurrent solvers of the batch driver")
+    ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
