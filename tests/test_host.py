@@ -117,6 +117,15 @@ def test_reference_entry_points_reject_bad_arguments():
     # GMRES (GMRES.cu:145-279)
     fails(L.bem_gmres(P(A), P(B), 0, 10, 1e-6, P(B)), "bad GMRES arguments")
     fails(L.bem_gmres(None, P(B), 42, 10, 1e-6, P(B)), "bad GMRES arguments")
+    # the device-resident extension checks its handles and arguments the same way
+    fails(L.bem_solver_assemble(None, 1.0, bem.PREC_FP32), "null solver")
+    fails(L.bem_solver_set_rhs(None, bem.RHS_POINT, P(src), 1, 1.0), "null solver")
+    fails(L.bem_solver_solve(None, bem.SOLVE_GMRES, 30, 1e-6, 100, 1e-9, 5, None), "null solver")
+    fails(L.bem_solver_get_solution(None, P(B), 42), "null argument")
+    fails(L.bem_batch_solve(m.handle, None, 0, bem.PREC_FP32, bem.RHS_POINT, None, 1, 0, 1.0, None, 1, None, 42,
+                            None), "bad batch arguments")
+    fails(L.bem_mesh_get_bb(m.handle, P(np.zeros(6, np.float32))), "bounding box not computed")
+    fails(L.bem_mesh_set_chief(m.handle, None, 2), "bad CHIEF arguments")
 
 
 @pytest.mark.parametrize("s", [0, 1, 2, 3, 4])
