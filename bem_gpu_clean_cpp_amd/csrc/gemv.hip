@@ -311,7 +311,8 @@ int gemv_h_f64(const double2 *A, long lda, long M, long N, const double2 *x, lon
 //                     are all in flight together, then one reduction and one
 //                     barrier (on LDS only) for all RB dots; the reduction
 //                     slots alternate by batch parity;
-//   ne_reduce         w = sum of the block partials in a fixed order, in FP64.
+//   ne_reduce         w = sum of the block partials in a fixed order, in FP64
+//                     (16 groups of 64 lanes per 64 columns, then a tree).
 constexpr int kNeBlocks = 256;
 // column pairs a row may have: 1024 threads x 6 pairs for the product,
 // 512 x 12 for the residual (N <= 12288)
@@ -482,37 +483,56 @@ __global__ __launch_bounds__(NT) void ne_apply_part(const float4 *__restrict__ A
     }
 }
 
-// 64 columns per workgroup, 4 groups of the block partials per column, 8 loads
-// in flight per thread; the 4 group sums are combined in a fixed order
-__global__ __launch_bounds__(256) void ne_reduce(const float2 *__restrict__ part, long ldp, int nblk, int N,
-                                                 double2 *__restrict__ w) {
-    __shared__ double2 s_sum[4][64];
+// w[c] = sum over the nblk block partials of column c, for 64 columns per
+// workgroup: 16 groups of 64 lanes each add every 16th partial (8 loads in
+// flight), then the 16 group sums are added in a fixed pairwise tree, so the
+// result does not depend on timing.
+constexpr int kNeRedGroups = 16;
+template <typename P>
+__device__ __forceinline__ void ne_reduce_cols(const P *__restrict__ part, long ldp, int nblk, int N,
+                                               double2 *__restrict__ w) {
+    constexpr int NG = kNeRedGroups;
+    __shared__ double2 s_sum[NG][64];
     const int cl = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + cl;
     const int cc = min(c, N - 1);
     double sr = 0, si = 0;
     int b = g;
-    for (; b + 4 * 7 < nblk; b += 4 * 8) {
-        float2 q[8];
+    for (; b + NG * 7 < nblk; b += NG * 8) {
+        P q[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) q[u] = part[(size_t)(b + 4 * u) * ldp + cc];
+        for (int u = 0; u < 8; u++) q[u] = part[(size_t)(b + NG * u) * ldp + cc];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             sr += q[u].x;
             si += q[u].y;
         }
     }
-    for (; b < nblk; b += 4) {
-        const float2 q = part[(size_t)b * ldp + cc];
+    for (; b < nblk; b += NG) {
+        const P q = part[(size_t)b * ldp + cc];
         sr += q.x;
         si += q.y;
     }
     s_sum[g][cl] = make_double2(sr, si);
     __syncthreads();
     if (g == 0 && c < N) {
-        const double2 a0 = s_sum[0][cl], a1 = s_sum[1][cl], a2 = s_sum[2][cl], a3 = s_sum[3][cl];
-        w[c] = make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
+        double2 a[NG];
+#pragma unroll
+        for (int j = 0; j < NG; j++) a[j] = s_sum[j][cl];
+#pragma unroll
+        for (int h = NG / 2; h >= 1; h >>= 1)
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                a[j].x += a[j + h].x;
+                a[j].y += a[j + h].y;
+            }
+        w[c] = a[0];
     }
+}
+
+__global__ __launch_bounds__(64 * kNeRedGroups) void ne_reduce(const float2 *__restrict__ part, long ldp, int nblk,
+                                                                int N, double2 *__restrict__ w) {
+    ne_reduce_cols(part, ldp, nblk, N, w);
 }
 
 bool ne_fused_fits(long M, long N) { return M > 0 && N > 0 && (N + 1) / 2 <= kNeMaxRowPairs; }
@@ -562,7 +582,7 @@ int ne_apply_f32(const float2 *Ar, long ldr, long M, long N, const float2 *v, do
 #undef NE_LAUNCH
     HIP_CALL(hipGetLastError());
     timing_end(tq, s);
-    ne_reduce<<<(unsigned)((N + 63) / 64), 256, 0, s>>>((const float2 *)ws, ldp, nblk, (int)N, w);
+    ne_reduce<<<(unsigned)((N + 63) / 64), 64 * kNeRedGroups, 0, s>>>((const float2 *)ws, ldp, nblk, (int)N, w);
     HIP_CALL(hipGetLastError());
     timing_end(tr, s);
     return EXIT_SUCCESS;
@@ -655,36 +675,10 @@ __global__ __launch_bounds__(NT) void ne_resid_part(const float4 *__restrict__ A
     }
 }
 
-// 64 columns per workgroup, 4 groups of the block partials per column; FP64
-__global__ __launch_bounds__(256) void ne_reduce_d(const double2 *__restrict__ part, long ldp, int nblk, int N,
-                                                   double2 *__restrict__ w) {
-    __shared__ double2 s_sum[4][64];
-    const int cl = threadIdx.x & 63, g = threadIdx.x >> 6;
-    const int c = blockIdx.x * 64 + cl;
-    const int cc = min(c, N - 1);
-    double sr = 0, si = 0;
-    int b = g;
-    for (; b + 4 * 7 < nblk; b += 4 * 8) {
-        double2 q[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) q[u] = part[(size_t)(b + 4 * u) * ldp + cc];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            sr += q[u].x;
-            si += q[u].y;
-        }
-    }
-    for (; b < nblk; b += 4) {
-        const double2 q = part[(size_t)b * ldp + cc];
-        sr += q.x;
-        si += q.y;
-    }
-    s_sum[g][cl] = make_double2(sr, si);
-    __syncthreads();
-    if (g == 0 && c < N) {
-        const double2 a0 = s_sum[0][cl], a1 = s_sum[1][cl], a2 = s_sum[2][cl], a3 = s_sum[3][cl];
-        w[c] = make_double2((a0.x + a1.x) + (a2.x + a3.x), (a0.y + a1.y) + (a2.y + a3.y));
-    }
+// the same reduction of FP64 block partials
+__global__ __launch_bounds__(64 * kNeRedGroups) void ne_reduce_d(const double2 *__restrict__ part, long ldp, int nblk,
+                                                                  int N, double2 *__restrict__ w) {
+    ne_reduce_cols(part, ldp, nblk, N, w);
 }
 
 size_t ws_ne_resid(long M, long N) { return 2 * ws_ne_apply(M, N) + sizeof(double2) * (size_t)ne_ldr(N); }
@@ -699,7 +693,7 @@ int transpose_rm_ahb_f32(const float2 *A, long lda, long M, long N, float2 *Ar, 
     double2 *P = (double2 *)ws;
     k_transpose_rm_ahb<<<g, 256, 0, s>>>(A, lda, (int)M, (int)N, Ar, ldr, b, P);
     HIP_CALL(hipGetLastError());
-    ne_reduce_d<<<(unsigned)((N + 63) / 64), 256, 0, s>>>(P, ldr, nbx, (int)N, r);
+    ne_reduce_d<<<(unsigned)((N + 63) / 64), 64 * kNeRedGroups, 0, s>>>(P, ldr, nbx, (int)N, r);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
@@ -744,7 +738,7 @@ int ne_resid_f32a(const float2 *Ar, long ldr, long M, long N, const double2 *x, 
     }
 #undef NE_LAUNCH
     HIP_CALL(hipGetLastError());
-    ne_reduce_d<<<(unsigned)((N + 63) / 64), 256, 0, s>>>((const double2 *)ws, ldp, nblk, (int)N, r);
+    ne_reduce_d<<<(unsigned)((N + 63) / 64), 64 * kNeRedGroups, 0, s>>>((const double2 *)ws, ldp, nblk, (int)N, r);
     HIP_CALL(hipGetLastError());
     timing_end(tr, s);
     return EXIT_SUCCESS;
