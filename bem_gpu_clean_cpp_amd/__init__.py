@@ -117,11 +117,14 @@ _SIGS = {
     "bem_mesh_gen_icosphere": (_I, [_P, _I, _F]),
     "bem_mesh_gen_ellipsoid": (_I, [_P, _I, _F, _F, _F, _I, _I, _F, ctypes.c_ulonglong]),
     "bem_mesh_signed_volume": (_D, [_P]),
+    "bem_mesh_fix_orientation": (_I, [_P, ctypes.POINTER(_I)]),
+    "bem_mesh_points_inside": (_I, [_P, _P, _I, ctypes.c_ulonglong, _P]),
     "bem_mesh_surf_area": (_F, [_P]),
     "bem_quad_init": (_I, [_I]),
     "bem_quad_get": (_I, [_P, _P]),
     "bem_set_medium": (_I, [_F, _F]),
     "bem_set_source_strength": (_I, [_F]),
+    "bem_set_near_field": (_I, [_F, _I]),
     "bem_system": (_I, [_P, _F, _P, _I, _P, _I, _P, _I]),
     "bem_gen_system": (_I, [_F, _P, _I, _P, _I, _I, _P, _I, _P, _I, _P, _I]),
     "bem_qr_solve": (_I, [_P, _I, _I, _I, _P, _I, _I]),
@@ -228,6 +231,12 @@ def set_source_strength(q):
     _check(lib().bem_set_source_strength(q), "bem_set_source_strength")
 
 
+def set_near_field(factor, max_level=12):
+    """FP64 near-singular integration for (row, element) pairs closer than
+    factor x the element's longest edge (0: off, the reference's rule)."""
+    _check(lib().bem_set_near_field(factor, max_level), "bem_set_near_field")
+
+
 # ----------------------------------------------------------------- mesh ---
 class Mesh:
     """The reference's `mesh` class (mesh.h:216-259) over bem_mesh."""
@@ -321,6 +330,20 @@ class Mesh:
 
     def signed_volume(self):
         return lib().bem_mesh_signed_volume(self._h)
+
+    def fix_orientation(self):
+        """Consistent winding with the normals into the body; returns the
+        number of elements whose winding changed."""
+        n = _I()
+        _check(lib().bem_mesh_fix_orientation(self._h, ctypes.byref(n)), "fix_orientation")
+        return n.value
+
+    def points_inside(self, pts, seed=0):
+        """The CHIEF generator's inside test (3-ray majority) for (n, 3) points."""
+        p = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+        out = np.zeros(p.shape[0], np.int32)
+        _check(lib().bem_mesh_points_inside(self._h, _ptr(p), p.shape[0], seed, _ptr(out)), "points_inside")
+        return out.astype(bool)
 
     def surf_area(self):
         return lib().bem_mesh_surf_area(self._h)

@@ -902,6 +902,399 @@ int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2
     return EXIT_SUCCESS;
 }
 
+// ------------------------------------------- near-singular, FP64 ---
+// The reference integrates every element that does not hold the collocation
+// point with the 6x6 collapsed rule (g_h_c_l_nsgl, mesh.cu:597-672; the only
+// exclusion is the vertex test, atomicFuncs.cu:75-76).  For a point closer to
+// the element than about its size the integrand peaks inside the element and
+// the rule's error grows to 1e-3 at a quarter of the element size.  With
+// bem_set_near_field on, every (row, element) pair closer than a factor times
+// the element's longest edge gets an FP64 correction
+//     delta = (integral by adaptive subdivision) - (the 6x6 rule in FP64),
+// added to the assembled entries after the row-owned assembly, in a fixed
+// order per row.  The subdivision splits a triangle into four at its edge
+// midpoints, depth first; a sub-triangle is integrated with the same 6x6
+// collapsed rule once its distance from the point (centroid distance minus
+// circumradius) is at least kNearEta times its longest edge, or at the depth
+// limit.  The path through the quadtree is kept as two bits per level, so the
+// walk needs no stack: each sub-triangle is rebuilt from the root.
+constexpr double kNearEta = 1.5;
+constexpr int kNearOut = 13;  // g (3 complex), h (3 complex), c
+
+struct NearArgs {
+    const double *__restrict__ pts64;  // 3 M: nodes then extra rows
+    const int *__restrict__ elems;     // 3 E
+    const int *__restrict__ row;       // per pair
+    const int *__restrict__ elem;      // per pair
+    int p0, p1;                        // pairs [p0, p1)
+    int levels;
+    double k, rw;                      // k, rho omega
+    double qu[kNQ], qv[kNQ], qw[kNQ];  // 6x6 collapsed rule on the unit triangle: (u, v) and weight
+    double *__restrict__ out;
+};
+
+struct NearAcc {
+    double gr[3], gi[3], hr[3], hi[3], c;
+};
+
+// the collapsed rule over the sub-triangle (a, b, c) of the element's
+// (xi1, xi2) plane, scaled by its area ratio `det`; y = p3 + xi1 e1 + xi2 e2
+__device__ void near_leaf(const NearArgs &a, const double *x, const double *p3, const double *e1, const double *e2,
+                          const double *n, double J, const double *va, const double *vb, const double *vc,
+                          double det, NearAcc &acc) {
+    const double inv4pi = 1.0 / (4.0 * kPi);
+    for (int q = 0; q < kNQ; q++) {
+        const double u = a.qu[q], v = a.qv[q];
+        const double xi1 = vc[0] + u * (va[0] - vc[0]) + v * (vb[0] - vc[0]);
+        const double xi2 = vc[1] + u * (va[1] - vc[1]) + v * (vb[1] - vc[1]);
+        const double N[3] = {xi1, xi2, 1.0 - xi1 - xi2};
+        double rv[3];
+        for (int d = 0; d < 3; d++) rv[d] = p3[d] + xi1 * e1[d] + xi2 * e2[d] - x[d];
+        const double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+        const double r = sqrt(r2);
+        const double prpn2 = (rv[0] * n[0] + rv[1] * n[1] + rv[2] * n[2]) / r;  // dr/dn_y
+        double sn, cs;
+        sincos(a.k * r, &sn, &cs);
+        const double w = a.qw[q] * det * J;
+        // G = exp(-ikr) / (4 pi r); dG/dn = G (-1/r - ik) dr/dn; dG0/dn = -dr/dn / (4 pi r^2)
+        const double Gr = cs * inv4pi / r, Gi = -sn * inv4pi / r;
+        const double Hr = (Gr * (-1.0 / r) + Gi * a.k) * prpn2;
+        const double Hi = (Gi * (-1.0 / r) - Gr * a.k) * prpn2;
+        for (int j = 0; j < 3; j++) {
+            acc.gr[j] += w * N[j] * Gr;
+            acc.gi[j] += w * N[j] * Gi;
+            acc.hr[j] += w * N[j] * Hr;
+            acc.hi[j] += w * N[j] * Hi;
+        }
+        acc.c += -w * prpn2 * inv4pi / r2;
+    }
+}
+
+// One thread per pair: delta = adaptive - rule, with g scaled by i rho omega
+// as the reference scales it (mesh.cu:668-670).
+__global__ __launch_bounds__(64) void near_elems(NearArgs a) {
+    const int id = a.p0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= a.p1) return;
+    const int e = a.elem[id], row = a.row[id];
+    double x[3], p[3][3];
+    for (int d = 0; d < 3; d++) x[d] = a.pts64[3 * (size_t)row + d];
+    for (int j = 0; j < 3; j++)
+        for (int d = 0; d < 3; d++) p[j][d] = a.pts64[3 * (size_t)a.elems[3 * e + j] + d];
+    double e1[3], e2[3];
+    for (int d = 0; d < 3; d++) {
+        e1[d] = p[0][d] - p[2][d];
+        e2[d] = p[1][d] - p[2][d];
+    }
+    const double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double J = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    const double n[3] = {cr[0] / J, cr[1] / J, cr[2] / J};
+    const double A0[2] = {1.0, 0.0}, B0[2] = {0.0, 1.0}, C0[2] = {0.0, 0.0};
+    NearAcc rule = {}, ad = {};
+    near_leaf(a, x, p[2], e1, e2, n, J, A0, B0, C0, 1.0, rule);
+    // depth-first walk of the quadtree; digits of the path, 2 bits per level
+    unsigned long long path = 0;
+    int level = 0;
+    for (;;) {
+        double va[2] = {1.0, 0.0}, vb[2] = {0.0, 1.0}, vc[2] = {0.0, 0.0};
+        for (int l = 1; l <= level; l++) {
+            const int dgt = (int)((path >> (2 * (l - 1))) & 3);
+            const double ab[2] = {0.5 * (va[0] + vb[0]), 0.5 * (va[1] + vb[1])};
+            const double bc[2] = {0.5 * (vb[0] + vc[0]), 0.5 * (vb[1] + vc[1])};
+            const double ca[2] = {0.5 * (vc[0] + va[0]), 0.5 * (vc[1] + va[1])};
+            double na[2], nb[2], nc[2];
+            if (dgt == 0) { na[0] = va[0]; na[1] = va[1]; nb[0] = ab[0]; nb[1] = ab[1]; nc[0] = ca[0]; nc[1] = ca[1]; }
+            else if (dgt == 1) { na[0] = ab[0]; na[1] = ab[1]; nb[0] = vb[0]; nb[1] = vb[1]; nc[0] = bc[0]; nc[1] = bc[1]; }
+            else if (dgt == 2) { na[0] = ca[0]; na[1] = ca[1]; nb[0] = bc[0]; nb[1] = bc[1]; nc[0] = vc[0]; nc[1] = vc[1]; }
+            else { na[0] = bc[0]; na[1] = bc[1]; nb[0] = ca[0]; nb[1] = ca[1]; nc[0] = ab[0]; nc[1] = ab[1]; }
+            va[0] = na[0]; va[1] = na[1]; vb[0] = nb[0]; vb[1] = nb[1]; vc[0] = nc[0]; vc[1] = nc[1];
+        }
+        // physical vertices, centroid distance, circumradius bound, longest edge
+        double Y[3][3], cen[3] = {0, 0, 0};
+        const double *vv[3] = {va, vb, vc};
+        for (int t = 0; t < 3; t++)
+            for (int d = 0; d < 3; d++) {
+                Y[t][d] = p[2][d] + vv[t][0] * e1[d] + vv[t][1] * e2[d];
+                cen[d] += Y[t][d] / 3.0;
+            }
+        double rad = 0, diam = 0;
+        for (int t = 0; t < 3; t++) {
+            double q = 0, s = 0;
+            for (int d = 0; d < 3; d++) {
+                q += (Y[t][d] - cen[d]) * (Y[t][d] - cen[d]);
+                s += (Y[t][d] - Y[(t + 1) % 3][d]) * (Y[t][d] - Y[(t + 1) % 3][d]);
+            }
+            rad = fmax(rad, sqrt(q));
+            diam = fmax(diam, sqrt(s));
+        }
+        const double dc = sqrt((x[0] - cen[0]) * (x[0] - cen[0]) + (x[1] - cen[1]) * (x[1] - cen[1]) +
+                               (x[2] - cen[2]) * (x[2] - cen[2]));
+        if (level >= a.levels || dc - rad >= kNearEta * diam) {
+            near_leaf(a, x, p[2], e1, e2, n, J, va, vb, vc, ldexp(1.0, -2 * level), ad);
+            // next sibling, climbing past last children
+            while (level > 0 && ((path >> (2 * (level - 1))) & 3) == 3) {
+                path &= ~(3ull << (2 * (level - 1)));
+                level--;
+            }
+            if (level == 0) break;
+            path += 1ull << (2 * (level - 1));
+        } else {
+            level++;  // first child: digit 0
+        }
+    }
+    double *o = a.out + (size_t)id * kNearOut;
+    // g_j = i rho omega int N_j G
+    for (int j = 0; j < 3; j++) {
+        const double dgr = ad.gr[j] - rule.gr[j], dgi = ad.gi[j] - rule.gi[j];
+        o[2 * j] = -a.rw * dgi;
+        o[2 * j + 1] = a.rw * dgr;
+        o[6 + 2 * j] = ad.hr[j] - rule.hr[j];
+        o[6 + 2 * j + 1] = ad.hi[j] - rule.hi[j];
+    }
+    o[12] = ad.c - rule.c;
+}
+
+// One thread per row r0 + i: the row's pair corrections in pair order,
+// A[i, node_j] += dh_j - (ca/cb) dg_j, A[i, i] -= dc on node rows,
+// bterm[i] += (cc/cb) sum_j dg_j (atomicFuncs.cu:86-115 for the corrections)
+template <typename R, typename C2>
+__global__ void near_apply(C2 *A, long ldA, int M, int N, int r0, const int *__restrict__ ptr,
+                           const int *__restrict__ elem, const int *__restrict__ elems, const float2 *__restrict__ bc,
+                           const double *__restrict__ d, double2 *__restrict__ bterm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = r0 + i;
+    if (row >= M) return;
+    double btr = 0, bti = 0, dc = 0;
+    for (int q = ptr[row]; q < ptr[row + 1]; q++) {
+        const int e = elem[q];
+        const double *o = d + (size_t)q * kNearOut;
+        const float2 ca = bc[3 * e], cb = bc[3 * e + 1], cc = bc[3 * e + 2];
+        const double den = (double)cb.x * cb.x + (double)cb.y * cb.y;
+        const double br = ((double)ca.x * cb.x + (double)ca.y * cb.y) / den;
+        const double bi = ((double)ca.y * cb.x - (double)ca.x * cb.y) / den;
+        const double gr = ((double)cc.x * cb.x + (double)cc.y * cb.y) / den;
+        const double gi = ((double)cc.y * cb.x - (double)cc.x * cb.y) / den;
+        double sgr = 0, sgi = 0;
+        for (int j = 0; j < 3; j++) {
+            const double dgr = o[2 * j], dgi = o[2 * j + 1];
+            const double pr = o[6 + 2 * j] - (br * dgr - bi * dgi);
+            const double pi = o[6 + 2 * j + 1] - (br * dgi + bi * dgr);
+            C2 *a = A + (size_t)elems[3 * e + j] * ldA + i;
+            C2 v = *a;
+            v.x = (R)((double)v.x + pr);
+            v.y = (R)((double)v.y + pi);
+            *a = v;
+            sgr += dgr;
+            sgi += dgi;
+        }
+        btr += gr * sgr - gi * sgi;
+        bti += gr * sgi + gi * sgr;
+        dc += o[12];
+    }
+    if (row < N && dc != 0.0) {
+        C2 *a = A + (size_t)row * ldA + i;
+        C2 v = *a;
+        v.x = (R)((double)v.x - dc);
+        *a = v;
+    }
+    double2 b = bterm[i];
+    b.x += btr;
+    b.y += bti;
+    bterm[i] = b;
+}
+
+namespace {
+// distance from x to the triangle (a, b, c): closest point by the regions of
+// the plane around the triangle (the vertex, edge and face regions of the
+// barycentric test)
+double point_triangle_distance(const double *x, const double *a, const double *b, const double *c) {
+    auto sub = [](const double *u, const double *v, double *o) {
+        for (int d = 0; d < 3; d++) o[d] = u[d] - v[d];
+    };
+    auto dot = [](const double *u, const double *v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    double ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+    sub(b, a, ab);
+    sub(c, a, ac);
+    sub(x, a, ap);
+    auto dist_to = [&](const double *y) {
+        return std::sqrt((x[0] - y[0]) * (x[0] - y[0]) + (x[1] - y[1]) * (x[1] - y[1]) + (x[2] - y[2]) * (x[2] - y[2]));
+    };
+    auto on = [&](const double *o, const double *dir, double t) {
+        for (int d = 0; d < 3; d++) q[d] = o[d] + t * dir[d];
+        return dist_to(q);
+    };
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) return dist_to(a);
+    sub(x, b, bp);
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return dist_to(b);
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) return on(a, ab, d1 / (d1 - d3));
+    sub(x, c, cp);
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return dist_to(c);
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) return on(a, ac, d2 / (d2 - d6));
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && (d4 - d3) >= 0 && (d5 - d6) >= 0) {
+        double bc[3];
+        sub(c, b, bc);
+        return on(b, bc, (d4 - d3) / ((d4 - d3) + (d5 - d6)));
+    }
+    const double den = 1.0 / (va + vb + vc);
+    const double v = vb * den, w = vc * den;
+    for (int d = 0; d < 3; d++) q[d] = a[d] + ab[d] * v + ac[d] * w;
+    return dist_to(q);
+}
+}  // namespace
+
+// (row, element) pairs with dist(x_row, element) < factor * (longest edge),
+// the element not holding the row's node; sorted by row, then element.  A
+// uniform grid over the element centroids, cells as wide as the largest
+// reach, bounds the search to the 27 cells around each row point.
+void near_pairs(const std::vector<double> &pts, int M, int N, const std::vector<int> &elems, double factor,
+                std::vector<int> &ptr, std::vector<int> &prow, std::vector<int> &pelem) {
+    const int E = (int)(elems.size() / 3);
+    std::vector<double> cen(3 * (size_t)E), reach(E), hmax(E);
+    double cell = 0, lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int e = 0; e < E; e++) {
+        const double *v[3] = {&pts[3 * (size_t)elems[3 * e]], &pts[3 * (size_t)elems[3 * e + 1]],
+                              &pts[3 * (size_t)elems[3 * e + 2]]};
+        double h = 0, r = 0;
+        for (int d = 0; d < 3; d++) cen[3 * e + d] = (v[0][d] + v[1][d] + v[2][d]) / 3.0;
+        for (int t = 0; t < 3; t++) {
+            double s = 0, q = 0;
+            for (int d = 0; d < 3; d++) {
+                s += (v[t][d] - v[(t + 1) % 3][d]) * (v[t][d] - v[(t + 1) % 3][d]);
+                q += (v[t][d] - cen[3 * e + d]) * (v[t][d] - cen[3 * e + d]);
+            }
+            h = std::max(h, std::sqrt(s));
+            r = std::max(r, std::sqrt(q));
+        }
+        hmax[e] = h;
+        reach[e] = r + factor * h;
+        cell = std::max(cell, reach[e]);
+        for (int d = 0; d < 3; d++) {
+            lo[d] = std::min(lo[d], cen[3 * e + d]);
+            hi[d] = std::max(hi[d], cen[3 * e + d]);
+        }
+    }
+    cell = std::max(cell, 1e-30);
+    int nc[3];
+    for (int d = 0; d < 3; d++) nc[d] = std::max(1, std::min(1024, (int)((hi[d] - lo[d]) / cell) + 1));
+    auto cell_of = [&](const double *y, int *c) {
+        for (int d = 0; d < 3; d++) c[d] = std::max(0, std::min(nc[d] - 1, (int)((y[d] - lo[d]) / cell)));
+    };
+    std::vector<std::vector<int>> grid((size_t)nc[0] * nc[1] * nc[2]);
+    for (int e = 0; e < E; e++) {
+        int c[3];
+        cell_of(&cen[3 * e], c);
+        grid[((size_t)c[0] * nc[1] + c[1]) * nc[2] + c[2]].push_back(e);
+    }
+    ptr.assign((size_t)M + 1, 0);
+    prow.clear();
+    pelem.clear();
+    std::vector<int> found;
+    for (int i = 0; i < M; i++) {
+        const double *x = &pts[3 * (size_t)i];
+        int c[3];
+        cell_of(x, c);
+        found.clear();
+        for (int a = std::max(0, c[0] - 1); a <= std::min(nc[0] - 1, c[0] + 1); a++)
+            for (int b = std::max(0, c[1] - 1); b <= std::min(nc[1] - 1, c[1] + 1); b++)
+                for (int z = std::max(0, c[2] - 1); z <= std::min(nc[2] - 1, c[2] + 1); z++)
+                    for (int e : grid[((size_t)a * nc[1] + b) * nc[2] + z]) {
+                        const int *n = &elems[3 * e];
+                        if (i < N && (n[0] == i || n[1] == i || n[2] == i)) continue;  // singular rule
+                        const double dc = std::sqrt((x[0] - cen[3 * e]) * (x[0] - cen[3 * e]) +
+                                                    (x[1] - cen[3 * e + 1]) * (x[1] - cen[3 * e + 1]) +
+                                                    (x[2] - cen[3 * e + 2]) * (x[2] - cen[3 * e + 2]));
+                        if (dc >= reach[e]) continue;
+                        const double d = point_triangle_distance(x, &pts[3 * (size_t)n[0]], &pts[3 * (size_t)n[1]],
+                                                                 &pts[3 * (size_t)n[2]]);
+                        if (d < factor * hmax[e]) found.push_back(e);
+                    }
+        std::sort(found.begin(), found.end());
+        for (int e : found) {
+            prow.push_back(i);
+            pelem.push_back(e);
+        }
+        ptr[i + 1] = (int)prow.size();
+    }
+}
+
+// the pairs of the device problem for the current bem_set_near_field setting
+static int near_prepare(DeviceProblem &dp, hipStream_t s) {
+    const Config &cfg = config();
+    if (cfg.near_factor == dp.near_factor && cfg.near_levels == dp.near_levels) return EXIT_SUCCESS;
+    sfree(dp.near_ptr, s);
+    sfree(dp.near_row, s);
+    sfree(dp.near_elem, s);
+    sfree(dp.near_out, s);
+    dp.near_npairs = 0;
+    dp.hnear_ptr.clear();
+    dp.near_factor = cfg.near_factor;
+    dp.near_levels = cfg.near_levels;
+    if (cfg.near_factor <= 0) return EXIT_SUCCESS;
+    std::vector<int> prow, pelem;
+    near_pairs(dp.hpts, dp.M, dp.N, dp.helems, cfg.near_factor, dp.hnear_ptr, prow, pelem);
+    dp.near_npairs = (int)prow.size();
+    HOST_CALL(salloc(&dp.near_ptr, dp.hnear_ptr.size(), s));
+    HOST_CALL(salloc(&dp.near_row, std::max<size_t>(1, prow.size()), s));
+    HOST_CALL(salloc(&dp.near_elem, std::max<size_t>(1, pelem.size()), s));
+    HOST_CALL(salloc(&dp.near_out, std::max<size_t>(1, prow.size()) * kNearOut, s));
+    HIP_CALL(hipMemcpyAsync(dp.near_ptr, dp.hnear_ptr.data(), dp.hnear_ptr.size() * sizeof(int), hipMemcpyHostToDevice,
+                            s));
+    if (!prow.empty()) {
+        HIP_CALL(hipMemcpyAsync(dp.near_row, prow.data(), prow.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_CALL(hipMemcpyAsync(dp.near_elem, pelem.data(), pelem.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    HIP_CALL(hipStreamSynchronize(s));  // the host vectors go out of scope
+    return EXIT_SUCCESS;
+}
+
+// corrections of rows r0..M-1 of A (ld ldA, row r0 first) and their bterm
+template <typename R, typename C2>
+static int near_correct(DeviceProblem &dp, double k, double rw, C2 *A, long ldA, int r0, double2 *bterm,
+                        hipStream_t s) {
+    HOST_CALL(near_prepare(dp, s));
+    if (dp.near_npairs == 0) return EXIT_SUCCESS;
+    const int p0 = dp.hnear_ptr[r0], p1 = dp.hnear_ptr[dp.M];
+    if (p1 > p0) {
+        NearArgs a;
+        a.pts64 = dp.pts64;
+        a.elems = dp.elems;
+        a.row = dp.near_row;
+        a.elem = dp.near_elem;
+        a.p0 = p0;
+        a.p1 = p1;
+        a.levels = dp.near_levels;
+        a.k = k;
+        a.rw = rw;
+        const Config &cfg = config();
+        for (int n = 0; n < kQOrder; n++) {
+            const double theta = 0.5 + 0.5 * cfg.qp[n];
+            for (int m = 0; m < kQOrder; m++) {
+                const double rho = 0.5 + 0.5 * cfg.qp[m];
+                const int q = n * kQOrder + m;
+                a.qu[q] = rho * (1 - theta);  // xi1 = rho (1 - theta), xi2 = rho theta (mesh.cu:623-636)
+                a.qv[q] = rho * theta;
+                a.qw[q] = 0.25 * cfg.qw[n] * cfg.qw[m] * rho;
+            }
+        }
+        a.out = dp.near_out;
+        TimedRegion tr;
+        timing_begin(tr, "near_elems", s);
+        near_elems<<<(p1 - p0 + 63) / 64, 64, 0, s>>>(a);
+        HIP_CALL(hipGetLastError());
+        timing_end(tr, s);
+    }
+    const int nrows = dp.M - r0;
+    near_apply<R, C2><<<(nrows + 255) / 256, 256, 0, s>>>(A, ldA, dp.M, dp.N, r0, dp.near_ptr, dp.near_elem, dp.elems,
+                                                        dp.bc, dp.near_out, bterm);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
 // --------------------------------------------------------------- host ---
 // Launch shape of the assembly: qg lanes per row (1 or 4) and P column tiles,
 // P x ceil(M / (64 / qg)) waves in all.  More tiles cost more halo elements
@@ -1132,6 +1525,8 @@ int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
     HOST_CALL(salloc(&dp.cpart64, (size_t)dp.P * dp.Mp, s));
     HOST_CALL(salloc(&dp.bpart64, (size_t)dp.P * dp.Mp, s));
     HOST_CALL(salloc(&dp.bterm, (size_t)dp.M, s));
+    dp.hpts = p64;
+    dp.helems = el;
     HIP_CALL(hipMemcpyAsync(dp.pts, p4.data(), p4.size() * sizeof(float4), hipMemcpyHostToDevice, s));
     HIP_CALL(hipMemcpyAsync(dp.pts64, p64.data(), p64.size() * sizeof(double), hipMemcpyHostToDevice, s));
     HIP_CALL(hipMemcpyAsync(dp.elems, el.data(), el.size() * sizeof(int), hipMemcpyHostToDevice, s));
@@ -1242,6 +1637,16 @@ void device_problem_free(DeviceProblem &dp) {
     sfree(dp.cpart64, dp.st);
     sfree(dp.bpart64, dp.st);
     sfree(dp.bterm, dp.st);
+    sfree(dp.near_ptr, dp.st);
+    sfree(dp.near_row, dp.st);
+    sfree(dp.near_elem, dp.st);
+    sfree(dp.near_out, dp.st);
+    dp.near_factor = 0;
+    dp.near_levels = 0;
+    dp.near_npairs = 0;
+    dp.hnear_ptr.clear();
+    dp.hpts.clear();
+    dp.helems.clear();
     dp.N = dp.C = dp.M = dp.E = dp.P = 0;
 }
 
@@ -1387,6 +1792,8 @@ static int assemble_impl(DeviceProblem &dp, R k, C2 *A, long ldA, int r0, double
         HIP_CALL(hipGetLastError());
         timing_end(tr, s);
     }
+    // opt-in FP64 near-singular corrections, after every row is complete
+    if (cfg.near_factor > 0 || dp.near_factor > 0) HOST_CALL((near_correct<R, C2>(dp, kd, rw, A, ldA, r0, bterm, s)));
     return EXIT_SUCCESS;
 }
 

@@ -53,6 +53,11 @@ struct Config {
     float density = 1.2041f;  // numerical.cu:12
     float speed = 343.21f;    // numerical.cu:14
     float strength = 0.0f;    // STRENGTH, numerical.h:128-130
+    // near-singular FP64 integration: (row, element) pairs closer than
+    // near_factor times the element's longest edge are integrated by
+    // adaptive subdivision (0: off, the reference's rule everywhere)
+    double near_factor = 0.0;
+    int near_levels = 12;
 };
 Config &config();
 int ensure_quad();  // default order-6 tables if bem_quad_init was not called
@@ -182,6 +187,18 @@ struct DeviceProblem {
     double2 *bpart64 = nullptr;
     double2 *bterm = nullptr;     // M: sum_e (cc/cb) sum_j g_j(x_i)
     long Mp = 0;
+    // near-singular FP64 correction (bem_set_near_field): host copies of the
+    // points and elements for the pair search, the (row, element) pairs in
+    // row order, and each pair's correction (13 doubles: g, h complex x 3, c)
+    std::vector<double> hpts;     // 3 * M
+    std::vector<int> helems;      // 3 * E
+    double near_factor = 0;       // factor the pairs were found with (0: none)
+    int near_levels = 0;
+    int near_npairs = 0;
+    std::vector<int> hnear_ptr;   // M + 1: pairs of row i at [ptr[i], ptr[i + 1])
+    int *near_ptr = nullptr;
+    int *near_row = nullptr, *near_elem = nullptr;
+    double *near_out = nullptr;
     uint64_t mesh_version = 0;
     hipStream_t st = nullptr;     // buffers are stream-ordered on this stream
 };
@@ -303,6 +320,8 @@ int gen_chief(Mesh &m, int num, float threshold, uint64_t seed);
 int gen_icosphere(Mesh &m, int s, float radius);
 int gen_ellipsoid(Mesh &m, int s, float ax, float ay, float az, int nb, int lmax, float amp, uint64_t seed);
 double signed_volume(const Mesh &m);
+int fix_orientation(Mesh &m, int *flipped);
+int points_inside(const Mesh &m, const float *pts, int n, uint64_t seed, int *out);
 double surf_area(const Mesh &m);
 int gauss_legendre(int n, double *x, double *w);
 int quad_init(int order);

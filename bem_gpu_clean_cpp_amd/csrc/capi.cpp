@@ -155,6 +155,16 @@ int bem_mesh_gen_ellipsoid(bem_mesh *m, int subdiv, float ax, float ay, float az
 }
 
 double bem_mesh_signed_volume(const bem_mesh *m) { return m ? signed_volume(m->m) : 0.0; }
+
+int bem_mesh_fix_orientation(bem_mesh *m, int *flipped) {
+    if (!m) BEM_FAIL("null mesh");
+    return fix_orientation(m->m, flipped);
+}
+
+int bem_mesh_points_inside(const bem_mesh *m, const bem_cart_coord *pts, int n, unsigned long long seed, int *inside) {
+    if (!m) BEM_FAIL("null mesh");
+    return points_inside(m->m, (const float *)pts, n, seed, inside);
+}
 float bem_mesh_surf_area(const bem_mesh *m) { return m ? (float)surf_area(m->m) : 0.f; }
 
 // ----------------------------------------------------------- global set-up ---
@@ -177,6 +187,14 @@ int bem_set_medium(float density, float speed) {
 
 int bem_set_source_strength(float q) {
     config().strength = q;
+    return EXIT_SUCCESS;
+}
+
+int bem_set_near_field(float factor, int max_level) {
+    if (!(factor >= 0.f) || factor > 16.f) BEM_FAIL("near-field factor must be in [0, 16], got %g", factor);
+    if (max_level < 1 || max_level > 24) BEM_FAIL("near-field subdivision depth must be in [1, 24], got %d", max_level);
+    config().near_factor = factor;
+    config().near_levels = max_level;
     return EXIT_SUCCESS;
 }
 
@@ -305,7 +323,16 @@ int bem_gmres(const bem_complex *A, const bem_complex *b, int m, int maxIter, fl
     }
     HIP_CALL(hipMemcpyAsync(bd, hb.data(), sizeof(double2) * m, hipMemcpyHostToDevice, s));
     HIP_CALL(hipMemcpyAsync(xd, hx.data(), sizeof(double2) * m, hipMemcpyHostToDevice, s));
-    const int restart = std::min(std::min(maxIter, m), 256);
+    // unrestarted, like the reference: one Krylov space of up to maxIter
+    // steps (at most m).  Only when its FP64 basis would take more than half
+    // of the free HBM is the space restarted at the largest length that fits.
+    int restart = std::min(maxIter, m);
+    {
+        size_t free_b = 0, total_b = 0;
+        HIP_CALL(hipMemGetInfo(&free_b, &total_b));
+        const long fit = (long)(free_b / 2 / (sizeof(double2) * (size_t)ld)) - 1;
+        if ((long)restart > fit) restart = (int)std::max(1L, fit);
+    }
     HOST_CALL(gmres_device(Ad, nullptr, ld, m, m, bd, ld, xd, ld, false, 1, restart, (double)threshold, maxIter,
                            (double)threshold, 1 << 20, nullptr, s));
     HIP_CALL(hipMemcpyAsync(hx.data(), xd, sizeof(double2) * m, hipMemcpyDeviceToHost, s));

@@ -57,66 +57,71 @@ Config &config() {
 }
 
 // Gauss-Legendre nodes and weights by the Golub-Welsch method (the method of
-// gaussQuad::genGaussParams, numerical.cu:186-225): eigenvalues of the
-// symmetric tridiagonal Jacobi matrix with off-diagonal (i+1)/sqrt((2i+2)^2-1)
-// are the nodes, 2 * (first eigenvector component)^2 the weights.  Implicit
-// symmetric QL iteration with Wilkinson shifts (Golub & Van Loan, 8.3),
-// accumulating only the first row of the eigenvector matrix.
+// gaussQuad::genGaussParams, numerical.cu:186-225): the nodes are the
+// eigenvalues of the symmetric tridiagonal Jacobi matrix with off-diagonal
+// b_i = (i+1)/sqrt((2i+2)^2-1), the weights 2 (first component of the unit
+// eigenvector)^2.  The eigenproblem is solved by the cyclic Jacobi method
+// (Jacobi 1846; Golub & Van Loan, Matrix Computations, 4th ed., section 8.5):
+// sweep over the off-diagonal pairs (p, q) in row order, zero each with a
+// plane rotation J (J^T A J), and accumulate the rotations into the eigenvector
+// matrix, until the off-diagonal mass is at rounding level.  Jacobi
+// rotations converge quadratically once the off-diagonal part is small and
+// give eigenvectors orthogonal to working precision; n <= 32 keeps the
+// O(n^3) sweeps negligible.
 int gauss_legendre(int n, double *x, double *w) {
     if (n < 1 || n > 32) BEM_FAIL("quadrature order %d out of range [1, 32]", n);
-    std::vector<double> d(n, 0.0), e(n, 0.0), z(n, 0.0);
-    for (int i = 0; i < n - 1; i++) {
-        double t = std::sqrt(std::pow(2.0 * (i + 1), 2) - 1);
-        e[i] = (i + 1) / t;
+    std::vector<double> a((size_t)n * n, 0.0), v((size_t)n * n, 0.0);
+    auto A = [&](int i, int j) -> double & { return a[(size_t)i * n + j]; };
+    auto V = [&](int i, int j) -> double & { return v[(size_t)i * n + j]; };
+    for (int i = 0; i < n; i++) V(i, i) = 1.0;
+    double total = 0.0;
+    for (int i = 0; i + 1 < n; i++) {
+        const double b = (i + 1) / std::sqrt(4.0 * (i + 1) * (i + 1) - 1.0);
+        A(i, i + 1) = A(i + 1, i) = b;
+        total += 2.0 * b * b;
     }
-    z[0] = 1.0;  // first components of the eigenvectors, rotated with the matrix
-    for (int l = 0; l < n; l++) {
-        int iter = 0;
-        for (;;) {
-            int m = l;
-            for (; m < n - 1; m++) {
-                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(e[m]) <= 1e-16 * dd) break;
-            }
-            if (m == l) break;
-            if (++iter > 60) BEM_FAIL("Gauss-Legendre eigen-iteration did not converge");
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = std::hypot(g, 1.0);
-            g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
-            double s = 1.0, c = 1.0, p = 0.0;
-            int i = m - 1;
-            for (; i >= l; i--) {
-                double f = s * e[i], b = c * e[i];
-                r = std::hypot(f, g);
-                e[i + 1] = r;
-                if (r == 0.0) {
-                    d[i + 1] -= p;
-                    e[m] = 0.0;
-                    break;
+    auto offdiag = [&]() {
+        double o = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++)
+                if (i != j) o += A(i, j) * A(i, j);
+        return o;
+    };
+    int sweeps = 0;
+    while (offdiag() > 1e-32 * total) {
+        if (++sweeps > 64) BEM_FAIL("Gauss-Legendre eigen-iteration did not converge");
+        for (int p = 0; p + 1 < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A(p, q);
+                if (apq == 0.0) continue;
+                // rotation angle phi with tan(2 phi) = 2 a_pq / (a_qq - a_pp):
+                // t = tan(phi) is the root of t^2 + 2 tau t - 1 = 0 of smaller size
+                const double tau = (A(q, q) - A(p, p)) / (2.0 * apq);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c;
+                for (int k = 0; k < n; k++) {  // A J: columns p and q
+                    const double kp = A(k, p), kq = A(k, q);
+                    A(k, p) = c * kp - sn * kq;
+                    A(k, q) = sn * kp + c * kq;
                 }
-                s = f / r;
-                c = g / r;
-                g = d[i + 1] - p;
-                r = (d[i] - g) * s + 2.0 * c * b;
-                p = s * r;
-                d[i + 1] = g + p;
-                g = c * r - b;
-                double zf = z[i + 1];
-                z[i + 1] = s * z[i] + c * zf;
-                z[i] = c * z[i] - s * zf;
+                for (int k = 0; k < n; k++) {  // J^T (A J): rows p and q
+                    const double pk = A(p, k), qk = A(q, k);
+                    A(p, k) = c * pk - sn * qk;
+                    A(q, k) = sn * pk + c * qk;
+                }
+                for (int k = 0; k < n; k++) {  // V J
+                    const double kp = V(k, p), kq = V(k, q);
+                    V(k, p) = c * kp - sn * kq;
+                    V(k, q) = sn * kp + c * kq;
+                }
             }
-            if (r == 0.0 && i >= l) continue;
-            d[l] -= p;
-            e[l] = g;
-            e[m] = 0.0;
-        }
     }
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](int a, int b) { return d[a] < d[b]; });
+    std::sort(idx.begin(), idx.end(), [&](int i, int j) { return A(i, i) < A(j, j); });
     for (int i = 0; i < n; i++) {
-        x[i] = d[idx[i]];
-        w[i] = 2.0 * z[idx[i]] * z[idx[i]];
+        x[i] = A(idx[i], idx[i]);
+        w[i] = 2.0 * V(0, idx[i]) * V(0, idx[i]);
     }
     return EXIT_SUCCESS;
 }
@@ -377,15 +382,34 @@ static bool ray_hits(const double *o, const double *d, const float *a, const flo
     return t > 0.0;
 }
 
+// inObj (numerical.cu:119-131): a point is inside when a ray from it crosses
+// the surface an odd number of times.  A ray through a shared edge or vertex
+// can count one crossing twice or miss it, so three rays in independent
+// random directions (uniform on the sphere) vote and the majority decides:
+// one unlucky ray no longer places a CHIEF point outside the body.
 bool point_inside(const Mesh &m, const double *x, SplitMix64 &rng) {
-    // random ray direction, uniform on the sphere
-    double z = 2.0 * rng.uniform() - 1.0, ph = 2.0 * kPi * rng.uniform();
-    double rr = std::sqrt(std::max(0.0, 1.0 - z * z));
-    double d[3] = {rr * std::cos(ph), rr * std::sin(ph), z};
-    int cnt = 0;
-    for (const auto &e : m.elems)
-        cnt += ray_hits(x, d, &m.pnts[3 * e.nodes[0]], &m.pnts[3 * e.nodes[1]], &m.pnts[3 * e.nodes[2]]);
-    return (cnt & 1) != 0;  // inObj (numerical.cu:119-131)
+    int votes = 0;
+    for (int ray = 0; ray < 3; ray++) {
+        double z = 2.0 * rng.uniform() - 1.0, ph = 2.0 * kPi * rng.uniform();
+        double rr = std::sqrt(std::max(0.0, 1.0 - z * z));
+        double d[3] = {rr * std::cos(ph), rr * std::sin(ph), z};
+        int cnt = 0;
+        for (const auto &e : m.elems)
+            cnt += ray_hits(x, d, &m.pnts[3 * e.nodes[0]], &m.pnts[3 * e.nodes[1]], &m.pnts[3 * e.nodes[2]]);
+        votes += cnt & 1;
+    }
+    return votes >= 2;
+}
+
+int points_inside(const Mesh &m, const float *pts, int n, uint64_t seed, int *out) {
+    if (m.N() == 0 || m.E() == 0) BEM_FAIL("Not enough mesh information!");
+    if (n < 0 || (n > 0 && (!pts || !out))) BEM_FAIL("bad point arguments");
+    SplitMix64 rng(seed);
+    for (int i = 0; i < n; i++) {
+        const double x[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        out[i] = point_inside(m, x, rng) ? 1 : 0;
+    }
+    return EXIT_SUCCESS;
 }
 
 // mesh::genCHIEF (mesh.cu:1430-1483): rejection sampling in the bounding box.
@@ -546,6 +570,77 @@ double signed_volume(const Mesh &m) {
                a[2] * ((double)b[0] * c[1] - (double)b[1] * c[0]);
     }
     return vol / 6.0;
+}
+
+// Orientation repair.  Within each edge-connected component, neighbouring
+// triangles must run through their shared edge in opposite directions; a
+// breadth-first walk from the component's first triangle flips every
+// neighbour that runs through it in the same direction.  Each component is
+// then flipped as a whole if its signed volume is positive, so its normals
+// (p1 - p3) x (p2 - p3) point into the body like the reference meshes'.
+// Edges shared by more than two triangles (non-manifold) propagate from the
+// first two only.  Returns the number of elements whose winding changed.
+int fix_orientation(Mesh &m, int *flipped) {
+    HOST_CALL(validate_mesh(m));
+    const int E = m.E();
+    std::unordered_map<uint64_t, std::vector<int>> edge_elems;
+    auto key = [](int a, int b) {
+        return a < b ? ((uint64_t)(uint32_t)a << 32) | (uint32_t)b : ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
+    };
+    for (int e = 0; e < E; e++)
+        for (int j = 0; j < 3; j++) edge_elems[key(m.elems[e].nodes[j], m.elems[e].nodes[(j + 1) % 3])].push_back(e);
+    // does element e run from a to b along one of its edges?
+    auto runs = [&](int e, int a, int b) {
+        const int *n = m.elems[e].nodes;
+        for (int j = 0; j < 3; j++)
+            if (n[j] == a && n[(j + 1) % 3] == b) return true;
+        return false;
+    };
+    auto flip = [&](int e) { std::swap(m.elems[e].nodes[1], m.elems[e].nodes[2]); };
+    std::vector<int> comp(E, -1);
+    std::vector<char> changed(E, 0);
+    int ncomp = 0;
+    for (int s = 0; s < E; s++) {
+        if (comp[s] >= 0) continue;
+        std::vector<int> members{s}, queue{s};
+        comp[s] = ncomp;
+        for (size_t h = 0; h < queue.size(); h++) {
+            const int e = queue[h];
+            for (int j = 0; j < 3; j++) {
+                const int a = m.elems[e].nodes[j], b = m.elems[e].nodes[(j + 1) % 3];
+                const std::vector<int> &nb = edge_elems[key(a, b)];
+                for (size_t t = 0; t < nb.size() && t < 2; t++) {
+                    const int f = nb[t];
+                    if (f == e || comp[f] >= 0) continue;
+                    comp[f] = ncomp;
+                    if (runs(f, a, b)) {  // same direction as e: inconsistent
+                        flip(f);
+                        changed[f] ^= 1;
+                    }
+                    queue.push_back(f);
+                    members.push_back(f);
+                }
+            }
+        }
+        double vol = 0;
+        for (int e : members) {
+            const float *a = &m.pnts[3 * m.elems[e].nodes[0]], *b = &m.pnts[3 * m.elems[e].nodes[1]],
+                        *c = &m.pnts[3 * m.elems[e].nodes[2]];
+            vol += a[0] * ((double)b[1] * c[2] - (double)b[2] * c[1]) -
+                   a[1] * ((double)b[0] * c[2] - (double)b[2] * c[0]) + a[2] * ((double)b[0] * c[1] - (double)b[1] * c[0]);
+        }
+        if (vol > 0)
+            for (int e : members) {
+                flip(e);
+                changed[e] ^= 1;
+            }
+        ncomp++;
+    }
+    int n = 0;
+    for (int e = 0; e < E; e++) n += changed[e];
+    if (n > 0) m.version++;
+    if (flipped) *flipped = n;
+    return EXIT_SUCCESS;
 }
 
 // trnglArea (mesh.cu:1075-1080) summed like cmptSurfArea (mesh.cu:1091-1117).

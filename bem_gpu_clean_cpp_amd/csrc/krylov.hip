@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "bem_internal.hpp"
+#include "hessenberg_ls.hpp"
 
 namespace bem {
 
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(256) void kb_mdot_part(const double2 *__restrict__ 
 // out[r * ostride + l] = sum_blocks part; with acc: acc[r * ostride + l] += the sum
 __global__ void kb_mdot_final(const double2 *__restrict__ part, long pstride, int nblk, int nv,
                               double2 *__restrict__ out, long ostride, double2 *__restrict__ acc, long astride) {
-    const int r = blockIdx.y, l = threadIdx.x;
+    const int r = blockIdx.y, l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nv) return;
     part += (size_t)r * pstride;
     double sr = 0, si = 0;
@@ -368,8 +369,8 @@ static int bdot(KrylovWS &ws, const double2 *V, long vstride, int nv, const doub
     dim3 g1(nblk, ws.R, (nv + 3) / 4);
     kb_mdot_part<<<g1, 256, 0, s>>>(V, ws.ldv, vstride, nv, w, wstride, n, ws.part, ws.pstride);
     HIP_CALL(hipGetLastError());
-    dim3 g2(1, ws.R);
-    kb_mdot_final<<<g2, 64 * ((nv + 63) / 64), 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc,
+    dim3 g2((nv + 255) / 256, ws.R);
+    kb_mdot_final<<<g2, 256, 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc,
                                                         astride ? astride : ostride);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
@@ -625,8 +626,8 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     constexpr double kCancel = 1e-2;
     std::vector<double> g_start(R), g_end(R);
     std::vector<int> steps(R);
-    std::vector<zc> H((size_t)R * (K + 1) * K), cs((size_t)R * K), sn((size_t)R * K), g((size_t)R * (K + 1));
-    std::vector<int> tlen(R, 0);
+    std::vector<HessenbergLS> ls(R);
+    std::vector<zc> hcol((size_t)K + 1), ysol((size_t)K);
     std::vector<char> live(R, 0);
     int total = 0, cycles = 0;
     // ws.r and beta hold the residual of the current X (and ws.h[.., 0] its
@@ -665,9 +666,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
         }
         for (int q = 0; q < R; q++) {
             live[q] = !done[q];
-            tlen[q] = 0;
-            for (int i = 0; i <= K; i++) g[(size_t)q * (K + 1) + i] = 0;
-            g[(size_t)q * (K + 1)] = beta[q];
+            ls[q].reset(K, beta[q]);
             if (done[q] && op.A32) HIP_CALL(hipMemsetAsync(ws.v32 + (size_t)q * ldv, 0, sizeof(float2) * ldv, s));
         }
         // Arnoldi steps in chunks: the steps of a chunk are launched back to
@@ -685,7 +684,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                 int need = 1;
                 for (int q = 0; q < R; q++) {
                     if (!live[q]) continue;
-                    const double gj = std::abs(g[(size_t)q * (K + 1) + j]);
+                    const double gj = ls[q].residual();
                     if (gj <= ctol[q] * beta[q]) continue;
                     need = std::max(need, (int)std::ceil(std::log(ctol[q] * beta[q] / gj) / std::log(rate)));
                 }
@@ -740,7 +739,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             HIP_CALL(hipMemcpyAsync(hch, ws.hch, sizeof(double2) * hcs * R, hipMemcpyDeviceToHost, s));
             HIP_CALL(hipStreamSynchronize(s));
             for (int q = 0; q < R; q++) {
-                g_start[q] = std::abs(g[(size_t)q * (K + 1) + j]);
+                g_start[q] = ls[q].residual();
                 steps[q] = 0;
             }
             bool anylive = false;
@@ -751,42 +750,26 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                 for (int q = 0; q < R; q++) {
                     if (!live[q]) continue;
                     const double2 *hq = &hch[(size_t)q * hcs + (size_t)t * hs];
-                    zc *hc = &H[((size_t)q * K + jj) * (K + 1)];
-                    zc *csq = &cs[(size_t)q * K], *snq = &sn[(size_t)q * K], *gq = &g[(size_t)q * (K + 1)];
                     double proj2 = 0;
                     for (int l = 0; l <= jj; l++) {
-                        hc[l] = zc(hq[l].x, hq[l].y);
-                        proj2 += std::norm(hc[l]);
+                        hcol[l] = zc(hq[l].x, hq[l].y);
+                        proj2 += std::norm(hcol[l]);
                     }
                     const double hn2 = std::max(0.0, hq[K + 1].x);
                     const double hn = std::sqrt(hn2);
-                    hc[jj + 1] = hn;
                     if (!cgs2 && hn2 < kCancel * (hn2 + proj2)) {
                         cgs2 = true;
                         if (trace)
                             fprintf(stderr, "gmres: step %d, system %d kept %.2e of |w|^2: second Gram-Schmidt pass from here\n",
                                     jj, q, hn2 / (hn2 + proj2));
                     }
-                    // previous rotations, then a new one (GMRES.cu:58-88 / 44-56)
-                    for (int l = 0; l < jj; l++) {
-                        const zc t0 = csq[l] * hc[l] + snq[l] * hc[l + 1];
-                        const zc t1 = -std::conj(snq[l]) * hc[l] + std::conj(csq[l]) * hc[l + 1];
-                        hc[l] = t0;
-                        hc[l + 1] = t1;
-                    }
-                    const double mag = std::sqrt(std::norm(hc[jj]) + std::norm(hc[jj + 1]));
-                    tlen[q] = jj + 1;
-                    if (mag == 0) {
+                    // previous rotations, then a new one (GMRES.cu:58-88 / 44-56);
+                    // a zero rotated column (singular H) is left out
+                    if (!ls[q].add_column(jj, hcol.data(), hn)) {
                         live[q] = 0;
                         continue;
                     }
-                    csq[jj] = std::conj(hc[jj]) / mag;
-                    snq[jj] = std::conj(hc[jj + 1]) / mag;
-                    hc[jj] = csq[jj] * hc[jj] + snq[jj] * hc[jj + 1];
-                    hc[jj + 1] = 0;
-                    gq[jj + 1] = -std::conj(snq[jj]) * gq[jj];
-                    gq[jj] = csq[jj] * gq[jj];
-                    const double gnew = std::abs(gq[jj + 1]);
+                    const double gnew = ls[q].residual();
                     steps[q]++;
                     g_end[q] = gnew;
                     if (gnew <= ctol[q] * beta[q] || hn == 0) live[q] = 0;
@@ -809,21 +792,16 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
         }
         // back substitution per system (Rsolver, GMRES.cu:26-42), x_r += V_r y_r
         int tmax = 0;
-        for (int q = 0; q < R; q++) tmax = std::max(tmax, tlen[q]);
+        for (int q = 0; q < R; q++)
+            if (!done[q]) tmax = std::max(tmax, ls[q].len());
         if (tmax == 0) continue;
         std::fill(yh, yh + nh, make_double2(0, 0));
         for (int q = 0; q < R; q++) {
-            const int t = tlen[q];
+            const int t = ls[q].len();
             if (done[q] || t == 0) continue;
-            std::vector<zc> y(t);
-            const zc *Hq = &H[(size_t)q * K * (K + 1)];
-            const zc *gq = &g[(size_t)q * (K + 1)];
-            for (int i = t - 1; i >= 0; i--) {
-                zc acc = gq[i];
-                for (int l = i + 1; l < t; l++) acc -= Hq[(size_t)l * (K + 1) + i] * y[l];
-                y[i] = acc / Hq[(size_t)i * (K + 1) + i];
-            }
-            for (int i = 0; i < t; i++) yh[(size_t)q * ws.hstride + i] = make_double2(y[i].real(), y[i].imag());
+            ls[q].solve(ysol.data());
+            for (int i = 0; i < t; i++)
+                yh[(size_t)q * ws.hstride + i] = make_double2(ysol[i].real(), ysol[i].imag());
         }
         HIP_CALL(hipMemcpyAsync(ws.h, yh, sizeof(double2) * ws.hstride * R, hipMemcpyHostToDevice, s));
         dim3 gg((N + bs - 1) / bs, R);

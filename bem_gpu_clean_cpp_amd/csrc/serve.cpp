@@ -180,6 +180,7 @@ struct bem_server {
     long next_id = 1;
     int running = 0;
     int max_running = 0;
+    int waiters = 0;  // threads inside bem_server_wait
     bool stop = false;
     std::vector<std::thread> threads;
     std::vector<bem_solver *> slots;
@@ -299,9 +300,14 @@ int bem_server_wait(bem_server *s, long id, bem_zcomplex *x, int ldx, bem_reques
         auto it = s->reqs.find(id);
         if (it == s->reqs.end()) BEM_FAIL("unknown request %ld", id);
         r = it->second;
+        // counted while inside: bem_server_destroy frees the server only
+        // after every waiter has left it
+        s->waiters++;
         s->cv_done.wait(lk, [&] { return r->state >= 2; });
-        s->reqs.erase(it);
+        s->reqs.erase(id);  // by key: submissions during the wait may rehash
+        if (--s->waiters == 0) s->cv_done.notify_all();
     }
+    // nothing below touches the server
     const int N = r->mesh.m.N();
     if (info) {
         info->id = r->id;
@@ -346,9 +352,12 @@ void bem_server_destroy(bem_server *s) {
         for (auto &r : s->queue) r->state = 4;
         s->queue.clear();
     }
+    s->cv_done.notify_all();  // waiters of cancelled requests return
     {
+        // the server is freed only after the running requests have ended and
+        // every thread inside bem_server_wait has left it
         std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_done.wait(lk, [&] { return s->running == 0; });
+        s->cv_done.wait(lk, [&] { return s->running == 0 && s->waiters == 0; });
         s->stop = true;
     }
     s->cv_work.notify_all();

@@ -1,21 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark: BEM surface-pressure solves/sec on one MI355X.
 
-Workload (default, "precision"): rigid sphere r = 0.1 m as an icosphere at
-subdivision 5 (10242 nodes, 20480 triangles), 2 kHz, c = 343 m/s,
-rho = 1.21 kg/m^3, 16 CHIEF points seeded uniformly in r < 0.05 m (M = 10258
-rows), unit plane wave along +z.  One step = complex FP32 assembly of A
-(M x N, in HBM) + GMRES(30, tol 1e-6) on the CHIEF least-squares system (the
-normal equations) with FP64 residual refinement -> nodal surface pressure.
+Headline workload ("large"): seeded perturbed ellipsoid (semi-axes
+0.075/0.09/0.11 m, 8 spherical-harmonic bumps up to degree 6, amplitude <= 5 %)
+at icosphere subdivision 6 (40962 nodes, 81920 triangles), rigid, 12 kHz
+(ka ~ 24), c = 343 m/s, rho = 1.21 kg/m^3, 32 CHIEF points (M = 40994 rows,
+a 13.4 GB complex64 matrix), 32 unit point sources 1 cm outside the surface
+as right-hand sides.  One step = complex FP32 assembly of A in HBM + the
+least-squares solve of all 32 right-hand sides (conjugate gradients on the
+normal equations with FP32 MFMA products and FP64 refinement of the
+normal-equation residual to 1e-11); value = 32 solves / step time.
 
-Also reported: assembly GB/s (8 M N + 8 M R bytes written / assembly time),
-p50/p99 step latency, the error of the FP32 result against a plain all-FP64 GPU
-solve of the same discretisation and against the rigid-sphere series, the
-roofline of the dominant kernel (the FP32 quadrature/assembly kernel, timed with
-HIP events on the library stream), and the CPU port (the oracle's complex-FP32
-restatement) timed on a row sample on the host cores.
+Also reported on that line: p50/p99 step latency, assembly time and GB/s,
+the error against the all-FP64 device solve of the same system, the plain
+single-RHS FP64 GPU solve time (the 1x yardstick), the MFMA roofline of the
+dominant kernels (the multi-right-hand-side products, timed with HIP events on
+the library stream in an instrumented step after the timed ones), and the CPU
+baseline: the oracle's complex-FP32 port of the assembly on a 2048-row sample
+plus a BLAS product on the same rows, on all the host cores this process may
+use, extrapolated to the whole step.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
+Sections of the same JSON line: precision (icosphere s5, 2 kHz, 16 CHIEF, one
+plane wave; GMRES on the normal equations), plain (s3, QR + field operator),
+batching (64 frequencies x 16 plane waves), serving (Poisson request stream).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--only SECTION]
 """
 import argparse
 import json
@@ -30,33 +39,34 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector and dense MFMA)
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and dense MFMA)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 FLOP_PER_EVAL = 80.0       # nominal FP32 flops per (row, quadrature point) evaluation
+# measured VALU issue rate of the FP32 assembly kernel (wave-instructions per
+# SIMD-cycle, SQ counters of profiles/precision_s5_asm_valu_counters.txt)
+ASM_VALU_ISSUE = 0.237
 
-CONFIGS = {
-    # name: (subdiv, radius, freq, n_chief, chief_seed)
-    "plain": dict(subdiv=3, radius=0.1, freq=171.0, nchief=8, seed=101, method="qr"),
-    "precision": dict(subdiv=5, radius=0.1, freq=2000.0, nchief=16, seed=102, method="gmres"),
-}
+PRECISION = dict(subdiv=5, radius=0.1, freq=2000.0, nchief=16, seed=102)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="precision", choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=5, help="timed steps of the headline (large) workload")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--large-method", choices=["cgls", "gmres"], default="cgls",
+                    help="solver of the headline workload (gmres: BA-GMRES(100) on the normal equations)")
+    ap.add_argument("--precision-steps", type=int, default=10)
     ap.add_argument("--plain-subdiv", type=int, default=3, help="icosphere subdivision of the plain workload")
     ap.add_argument("--plain-reps", type=int, default=20)
-    ap.add_argument("--only", choices=["plain", "batching", "large", "serving"],
-                    help="run one secondary workload alone and print its section (not the benchmark line)")
+    ap.add_argument("--only", choices=["plain", "precision", "batching", "large", "serving"],
+                    help="run one workload alone and print its section (not the benchmark line)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (requests/s)")
@@ -68,7 +78,7 @@ def parse():
 
 def calibrate_copy(mib):
     """A plain copy kernel of known size, to scale the FETCH_SIZE / WRITE_SIZE
-    hardware counters of the assembly kernel."""
+    hardware counters of the library's kernels."""
     import torch
     n = mib * (1 << 20) // 4
     a = torch.ones(n, device="cuda", dtype=torch.float32)
@@ -100,36 +110,78 @@ def measured_traffic(kernel, workload):
     return None
 
 
-def cpu_baseline(mesh, k, rho, c, target_s, threads):
-    """The oracle's complex-FP32 port of the reference assembly, timed on a row
-    sample on the host cores and scaled to a full assembly (solve not counted)."""
+def host_cores():
+    """Cores this process may run on: its CPU affinity, bounded by a cgroup
+    CPU quota when one is set (cpu.max), so the CPU baseline uses every core
+    it has and no more."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_of(fn, n=5):
+    """1 warm-up call, then the median of n timed calls (seconds)."""
+    fn()
+    ts = []
+    for _ in range(n):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline_large(mesh, k, rho, c, R, products, threads):
+    """The CPU path of the headline step, extrapolated from samples: the
+    oracle's complex-FP32 port of the reference assembly (OpenMP over rows) on
+    512-row samples (1 warm-up, median of 5 at 5 places, 2048 rows in all),
+    scaled to all M rows, plus the solve's `products` passes of the matrix
+    against the 32 right-hand sides as complex64 BLAS products (numpy, the
+    same threads) on a 2048-row block, scaled to M rows."""
     import oracle_py as O
     import bemcases as C
+    from threadpoolctl import threadpool_limits
     O.quad_init(6)
     p, e, bc = C.mesh_arrays(mesh)
-    bc = np.tile(np.array([0, 1, 0], complex), (len(e), 1))
     pts_all = np.vstack([p, mesh.chief()]).astype(np.float32)
-    M, N, E = pts_all.shape[0], p.shape[0], e.shape[0]
+    M, N = pts_all.shape[0], p.shape[0]
+    starts = iter([int(M * f) for f in (0.1, 0.3, 0.5, 0.7, 0.9, 0.05)])
 
-    def run(rows):
-        r0 = (M - rows) // 2
-        t0 = time.perf_counter()
-        O.assemble_rows(k, pts_all, N, e, bc, r0, r0 + rows, dens=rho, speed=c, dtype=np.float32,
-                        nthreads=threads)
-        return time.perf_counter() - t0
+    def asm():
+        r0 = next(starts)
+        O.assemble_rows(k, pts_all, N, e, bc, r0, r0 + 512, dens=rho, speed=c, dtype=np.float32, nthreads=threads)
 
-    # calibrate on a small sample, then size the timed sample for ~target_s
-    t_cal = run(min(M, 2 * threads))
-    per_row = t_cal / min(M, 2 * threads)
-    rows = int(max(threads, min(M, target_s / max(per_row, 1e-9))))
-    t = run(rows)
-    per_row = t / rows
-    full = per_row * M
+    t_asm, ts_asm = median_of(asm)
+    rng = np.random.default_rng(7)
+    a = (rng.standard_normal((2048, N)) + 1j * rng.standard_normal((2048, N))).astype(np.complex64)
+    x = (rng.standard_normal((N, R)) + 1j * rng.standard_normal((N, R))).astype(np.complex64)
+    with threadpool_limits(limits=threads):
+        t_mv, ts_mv = median_of(lambda: a @ x)
+    asm_full = t_asm / 512 * M
+    solve_full = products * t_mv / 2048 * M
+    total = asm_full + solve_full
     return {
-        "value": 1.0 / full, "unit": "solves/s", "cores": threads, "kind": "port",
-        "sample": "complex-FP32 oracle assembly of %d of %d rows on %d threads in %.1f s, scaled to the full "
-                  "%d x %d system: %.2f s per solve (solve not counted)" % (rows, M, threads, t, M, N, full),
-        "full_assembly_s": full,
+        "value": R / total, "unit": "solves/s", "cores": threads, "kind": "port", "extrapolated": True,
+        "cpu_model": cpu_model(),
+        "sample": "extrapolated: complex-FP32 oracle assembly, median of 5 samples of 512 rows (%.2f s each) "
+                  "-> %.1f s for all %d rows; %d products of the matrix with the %d right-hand sides as complex64 "
+                  "BLAS on a 2048-row block (median of 5, %.3f s) -> %.1f s; %.1f s per %d-solve step on %d "
+                  "threads" % (t_asm, asm_full, M, products, R, t_mv, solve_full, total, R, threads),
+        "assembly_s": asm_full, "solve_s": solve_full,
     }
 
 
@@ -274,59 +326,99 @@ def bench_plain(bem, C, reps=20, subdiv=3):
     }
 
 
-def bench_large(bem, C, fp64=True):
-    """Large: seeded perturbed ellipsoid at icosphere subdivision 6 (40962
+def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="cgls"):
+    """Headline: seeded perturbed ellipsoid at icosphere subdivision 6 (40962
     nodes, 81920 triangles, 13.4 GB complex64 matrix), rigid, 12 kHz, 32 CHIEF
-    points, 32 point sources 1 cm outside the surface; FP32 assembly + CG on
-    the normal equations with FP32 MFMA products (R = 32) and FP64 refinement
-    to a normal-equation residual of 1e-11."""
-    c = 343.0
+    points, 32 point sources 1 cm outside the surface.  One step = FP32
+    assembly + the least-squares solve of the 32 right-hand sides (CG on the
+    normal equations with FP32 MFMA products, FP64 refinement to a
+    normal-equation residual of 1e-11).  Timed steps run without per-launch
+    events (the CG iterations replay a captured HIP graph); kernel durations
+    for the rooflines come from one instrumented step after them."""
+    c, rho = 343.0, 1.21
     m, src = C.large_case(bem)
     N, E, Cn = m.num_pnts, m.num_elems, m.num_chief
     M = N + Cn
     R = src.shape[0]
     k = 2 * np.pi * 12000.0 / c
-    kw = dict(tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=10)
+    if method == "gmres":
+        kw = dict(method=bem.SOLVE_GMRES, restart=100, tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=20)
+    else:
+        kw = dict(method=bem.SOLVE_CGLS, tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=10)
     sv = bem.Solver(m)
     sv.set_rhs(bem.RHS_POINT, src, 1.0)
-    sv.assemble(k, bem.PREC_FP32)  # warm-up of the assembly kernels at this size
+
+    def step():
+        sv.assemble(k, bem.PREC_FP32)
+        return sv.solve(**kw)
+
+    st = None
+    for i in range(warmup):
+        st = step()
+        log("large warm-up %d: %s" % (i, st))
     bem.sync()
-    bem.timing_reset()
-    bem.timing_enable(True)
+    lat = []
     t0 = time.perf_counter()
-    sv.assemble(k, bem.PREC_FP32)
-    st = sv.solve(bem.SOLVE_CGLS, **kw)
+    for i in range(steps):
+        ts = time.perf_counter()
+        st = step()
+        lat.append((time.perf_counter() - ts) * 1e3)
+        log("large step %d: %.3f s, %s" % (i, lat[-1] / 1e3, st))
     bem.sync()
     wall = time.perf_counter() - t0
-    bem.timing_enable(False)
-    g_ms, g_n = bem.timing_get("gemm_n_f32")
-    h_ms, h_n = bem.timing_get("gemm_h_f32")
-    a_ms, a_n = bem.timing_get("asm_tile_f32")
-    bem.timing_reset()
+    ms_per_step = wall * 1e3 / steps
     x32 = sv.solution()
+    # one instrumented step: per-launch HIP events on the library stream
+    bem.timing_reset()
+    bem.timing_enable(True)
+    sti = step()
+    bem.sync()
+    bem.timing_enable(False)
+    t = {name: bem.timing_get(name) for name in ("gemm_n_f32", "gemm_h_f32", "asm_tile_f32", "sing_elems",
+                                                 "asm_finalize", "ghost_reduce", "mres_f64acc", "mh_f64acc")}
+    bem.timing_reset()
     sv.close()
-    log("large: %.2f s, stats %s" % (wall, st))
-    g_avg, h_avg = g_ms / max(g_n, 1), h_ms / max(h_n, 1)
+    avg = {kk: (v[0] / v[1] if v[1] else None) for kk, v in t.items()}
     flops = 8.0 * M * N * R
+    gbytes = 8.0 * M * N + 8.0 * (M + N) * R
+    g_avg, h_avg = avg["gemm_n_f32"], avg["gemm_h_f32"]
+    asm_ms = sum(v[0] for kk, v in t.items() if kk in ("asm_tile_f32", "sing_elems", "asm_finalize",
+                                                        "ghost_reduce")) / max(t["asm_tile_f32"][1], 1)
     out = {
         "workload": "large: perturbed ellipsoid s6 (N=%d, E=%d, M=%d, A %.1f GB), rigid, 12 kHz, 32 CHIEF, "
-                    "32 point sources 1 cm outside; FP32 assembly + CG on the normal equations (FP32 MFMA "
-                    "products, R=32) with FP64 refinement to 1e-11" % (N, E, M, 8.0 * M * N / 1e9),
-        "value": round(R / wall, 3), "unit": "solves/s", "wall_s": round(wall, 3),
-        "assembly_ms": round(a_ms / max(a_n, 1), 2),
+                    "32 point sources 1 cm outside; FP32 assembly + %s (FP32 MFMA products, R=32) with FP64 "
+                    "refinement to 1e-11" % (N, E, M, 8.0 * M * N / 1e9,
+                                             "BA-GMRES(100) on the normal equations" if method == "gmres"
+                                             else "CG on the normal equations"),
+        "N": N, "E": E, "M": M, "rhs": R,
+        "value": R / (ms_per_step / 1e3), "ms_per_step": ms_per_step,
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2)},
+        "assembly_ms": round(asm_ms, 2),
+        "assembly_gbs": round((8.0 * M * N + 8.0 * M * R) / (asm_ms * 1e-3) / 1e9, 1),
         "solve": {kk: st[kk] for kk in ("iterations", "refinements", "converged", "residual")},
-        "roofline_gemm": {"bound": "mfma", "kernel": "gemm_n_f32", "avg_ms": round(g_avg, 3), "launches": g_n,
-                          "achieved": round(flops / (g_avg * 1e-3) / 1e12, 2), "peak": PEAK_FP32_TFLOPS,
-                          "unit": "TFLOP/s", "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-                          "traffic": measured_traffic("gemm_n_f32", "large"),
-                          "adjoint_traffic": measured_traffic("gemm_h_f32", "large"),
-                          "adjoint_avg_ms": round(h_avg, 3),
-                          "adjoint_achieved": round(flops / (h_avg * 1e-3) / 1e12, 2),
-                          "hbm_gbs": round(8.0 * M * N / (g_avg * 1e-3) / 1e9, 1), "flops_per_launch": flops},
+        "instrumented_step": {"products_n": t["gemm_n_f32"][1], "products_h": t["gemm_h_f32"][1],
+                              "solve_ms": round(sti["solve_ms"], 1)},
+        "roofline": {"bound": "mfma", "kernel": "gemm_n_f32", "achieved": round(flops / (g_avg * 1e-3) / 1e12, 2),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                     "traffic": measured_traffic("gemm_n_f32", "large"), "avg_ms": round(g_avg, 3),
+                     "launches": t["gemm_n_f32"][1], "flops_per_launch": flops, "bytes_per_launch": gbytes,
+                     "hbm_gbs": round(8.0 * M * N / (g_avg * 1e-3) / 1e9, 1),
+                     "adjoint": {"kernel": "gemm_h_f32", "avg_ms": round(h_avg, 3),
+                                 "achieved": round(flops / (h_avg * 1e-3) / 1e12, 2),
+                                 "frac": round(flops / (h_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+                                 "traffic": measured_traffic("gemm_h_f32", "large")}},
     }
+    if avg["mres_f64acc"]:
+        out["roofline_fp64_residual"] = {"bound": "mfma", "kernel": "mres_f64acc (FP64 MFMA, FP32 A widened)",
+                                         "avg_ms": round(avg["mres_f64acc"], 3),
+                                         "achieved": round(flops / (avg["mres_f64acc"] * 1e-3) / 1e12, 2),
+                                         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                         "frac": round(flops / (avg["mres_f64acc"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                                                       4)}
     if fp64:
         # 1x yardstick: plain single-RHS FP64 solve (FP64 assembly + FP64 CG),
-        # also the reference solution for the first source
+        # also the reference solution for the first two sources
         s64 = bem.Solver(m)
         s64.set_rhs(bem.RHS_POINT, src[:1], 1.0)
         bem.sync()
@@ -340,7 +432,125 @@ def bench_large(bem, C, fp64=True):
         err = float(np.linalg.norm(x32[:, 0] - x64[:, 0]) / np.linalg.norm(x64[:, 0]))
         log("large fp64: %.2f s, stats %s, err %.3g" % (t64, st64, err))
         out.update({"rel_err_vs_fp64": err, "fp64_single_rhs_s": round(t64, 3),
-                    "speedup_vs_fp64_single_rhs": round((R / wall) * t64, 2)})
+                    "speedup_vs_fp64_single_rhs": round(out["value"] * t64, 2)})
+    if cpu:
+        products = 2 * st["iterations"] + 2 * (st["refinements"] + 1)
+        out["cpu_baseline"] = cpu_baseline_large(m, k, rho, c, R, products, host_cores())
+        log("cpu baseline", out["cpu_baseline"])
+    return out
+
+
+def bench_precision(bem, C, steps, warmup=2, fp64=True):
+    """Precision: rigid sphere r = 0.1 m, icosphere s5 (10242 nodes, 20480
+    triangles), 2 kHz, 16 CHIEF points in r < 0.05 m, one plane wave along +z.
+    One step = FP32 assembly + GMRES(30, 1e-6) on the normal equations with
+    FP64 refinement to 1e-8.  Reports the step rate, the error against the
+    all-FP64 GPU solve (also timed: the 1x yardstick) and the sphere series,
+    the assembly kernel's nominal and counter-based VALU figures, the FP64
+    yardstick assembly against the FP64 peak, and the one-pass
+    normal-equation product against HBM."""
+    cfg = PRECISION
+    c = 343.0
+    a = cfg["radius"]
+    k = 2 * np.pi * cfg["freq"] / c
+    mesh = bem.Mesh().icosphere(cfg["subdiv"], a)
+    mesh.set_bc(0, 1, 0)
+    mesh.set_chief(C.ball_points(cfg["seed"], cfg["nchief"], a / 2))
+    N, E, Cn = mesh.num_pnts, mesh.num_elems, mesh.num_chief
+    M = N + Cn
+    solver = bem.Solver(mesh)
+    solver.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+
+    def step():
+        solver.assemble(k, bem.PREC_FP32)
+        return solver.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-8, max_refine=50)
+
+    for _ in range(warmup):
+        st = step()
+    bem.sync()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        st = step()
+        lat.append((time.perf_counter() - ts) * 1e3)
+    bem.sync()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / steps
+    bem.timing_reset()
+    bem.timing_enable(True)
+    for _ in range(3):
+        step()
+    bem.sync()
+    bem.timing_enable(False)
+    tm = {name: bem.timing_get(name) for name in ("asm_tile_f32", "sing_elems", "asm_finalize", "ghost_reduce",
+                                                  "ne_apply_part", "ne_apply_f32")}
+    bem.timing_reset()
+    x32 = solver.solution()
+    evals = 36.0 * (M * E - 3 * E)
+    asm_avg_ms = tm["asm_tile_f32"][0] / max(tm["asm_tile_f32"][1], 1)
+    asm_total_ms = sum(tm[kk][0] for kk in ("asm_tile_f32", "sing_elems", "asm_finalize", "ghost_reduce")) / max(
+        tm["asm_tile_f32"][1], 1)
+    nominal = FLOP_PER_EVAL * evals / (asm_avg_ms * 1e-3) / 1e12
+    ne_ms, ne_n = tm["ne_apply_part"]
+    ne_bytes = 8.0 * M * N + 16.0 * N
+    ne_avg_us = ne_ms / ne_n * 1e3 if ne_n else None
+    ne_gbs = ne_bytes / (ne_avg_us * 1e-6) / 1e9 if ne_n else None
+    out = {
+        "workload": "precision: rigid sphere icosphere s5 r=0.10 m (N=%d, E=%d, M=%d), 2000 Hz, 16 CHIEF, plane wave; "
+                    "FP32 assembly + GMRES(30,1e-6) on the normal equations with FP64 refinement to 1e-8" % (N, E, M),
+        "value": round(1000.0 / ms_per_step, 3), "unit": "solves/s", "ms_per_step": round(ms_per_step, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
+        "assembly_ms": round(asm_total_ms, 3),
+        "assembly_gbs": round((8.0 * M * N + 8.0 * M) / (asm_total_ms * 1e-3) / 1e9, 2),
+        "solve": {kk: st[kk] for kk in ("iterations", "refinements", "converged", "residual")},
+        "roofline_asm": {"bound": "valu", "kernel": "asm_tile_f32", "avg_ms": round(asm_avg_ms, 4),
+                         "nominal_flops_per_launch": FLOP_PER_EVAL * evals,
+                         "nominal_achieved": round(nominal, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                         "nominal_frac": round(nominal / PEAK_FP32_TFLOPS, 4),
+                         "valu_issue_frac_counters": ASM_VALU_ISSUE,
+                         "traffic": measured_traffic("asm_tile_f32", "precision")},
+        "roofline_ne": {"bound": "hbm", "kernel": "ne_apply_part", "achieved": ne_gbs and round(ne_gbs, 1),
+                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ne_gbs and round(ne_gbs / PEAK_HBM_GBS, 4),
+                        "bytes_per_launch": ne_bytes, "avg_us": ne_avg_us and round(ne_avg_us, 2),
+                        "avg_us_with_reduce": tm["ne_apply_f32"][1] and round(
+                            tm["ne_apply_f32"][0] / tm["ne_apply_f32"][1] * 1e3, 2),
+                        "traffic": measured_traffic("ne_apply_part", "precision"), "launches": ne_n},
+    }
+    if fp64:
+        solver64 = bem.Solver(mesh)
+        solver64.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
+        t64 = []
+        for i in range(2):
+            ts = time.perf_counter()
+            solver64.assemble(k, bem.PREC_FP64)
+            st64 = solver64.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=5000, refine_tol=0, max_refine=0)
+            bem.sync()
+            t64.append((time.perf_counter() - ts) * 1e3)
+        # the yardstick's assembly kernel against the FP64 peak (same nominal count)
+        bem.timing_reset()
+        bem.timing_enable(True)
+        solver64.assemble(k, bem.PREC_FP64)
+        bem.sync()
+        bem.timing_enable(False)
+        a64_ms, a64_n = bem.timing_get("asm_tile_f64")
+        bem.timing_reset()
+        x64 = solver64.solution()
+        solver64.close()
+        a64 = a64_ms / max(a64_n, 1)
+        nominal64 = FLOP_PER_EVAL * evals / (a64 * 1e-3) / 1e12
+        out.update({
+            "rel_err_vs_fp64": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
+            "fp64_gpu_ms_per_solve": round(min(t64), 3),
+            "speedup_vs_fp64_gpu": round(min(t64) / ms_per_step, 3),
+            "fp64_yardstick_assembly": {"kernel": "asm_tile_f64 (generic FP64 kernel, halo plan)",
+                                        "avg_ms": round(a64, 3), "nominal_achieved": round(nominal64, 3),
+                                        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                                        "nominal_frac": round(nominal64 / PEAK_FP64_TFLOPS, 4)},
+        })
+    pn = mesh.pnts().astype(np.float64)
+    ser = C.rigid_sphere_surface(a, k, pn[:, 2] / np.linalg.norm(pn, axis=1))
+    out["rel_err_vs_series"] = float(np.linalg.norm(x32[:, 0] - ser) / np.linalg.norm(ser))
+    solver.close()
     return out
 
 
@@ -469,192 +679,65 @@ def main():
     import bem_gpu_clean_cpp_amd as bem
     import bemcases as C
 
+    bem.quad_init(6)
+    bem.set_medium(1.21, 343.0)
+    fp64 = not args.no_fp64
     if args.only:
-        # one secondary workload alone (for profiling); prints that section's JSON
-        bem.quad_init(6)
-        bem.set_medium(1.21, 343.0)
+        # one workload alone (for profiling); prints that section's JSON
         if args.only == "plain":
             res = bench_plain(bem, C, args.plain_reps, args.plain_subdiv)
+        elif args.only == "precision":
+            res = bench_precision(bem, C, args.precision_steps, fp64=fp64)
         elif args.only == "batching":
-            res = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
+            res = bench_batching(bem, C, args.workers, fp64=fp64)
         elif args.only == "large":
-            res = bench_large(bem, C, fp64=not args.no_fp64)
+            res = bench_large(bem, C, args.steps, args.warmup, fp64=fp64, cpu=not args.no_cpu_baseline,
+                              method=args.large_method)
         else:
             res = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
         print(json.dumps({args.only: res}), flush=True)
         return
 
-    cfg = CONFIGS[args.config]
-    rho, c = 1.21, 343.0
-    a = cfg["radius"]
-    k = 2 * np.pi * cfg["freq"] / c
-    bem.quad_init(6)
-    bem.set_medium(rho, c)
-    mesh = bem.Mesh().icosphere(cfg["subdiv"], a)
-    mesh.set_bc(0, 1, 0)
-    mesh.set_chief(C.ball_points(cfg["seed"], cfg["nchief"], a / 2))
-    N, E, Cn = mesh.num_pnts, mesh.num_elems, mesh.num_chief
-    M = N + Cn
-    R = 1
-    log("config %s: N=%d E=%d C=%d M=%d k=%.4f ka=%.3f" % (args.config, N, E, Cn, M, k, k * a))
-    method = bem.SOLVE_GMRES if cfg["method"] == "gmres" else bem.SOLVE_QR
-    solver = bem.Solver(mesh)
-    solver.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
-
-    def step():
-        solver.assemble(k, bem.PREC_FP32)
-        return solver.solve(method, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-8, max_refine=50)
-
-    st = None
-    for _ in range(args.warmup):
-        st = step()
-    log("warmup stats", st)
-    bem.sync()
-    lat = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        st = step()
-        lat.append((time.perf_counter() - ts) * 1e3)
-    bem.sync()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    # kernel durations for the rooflines from further steps with per-launch
-    # event timing on (kept out of the timed steps above: the events add host
-    # work around every launch)
-    bem.timing_reset()
-    bem.timing_enable(True)
-    for _ in range(3):
-        step()
-    bem.sync()
-    bem.timing_enable(False)
-    asm_ms, asm_n = bem.timing_get("asm_tile_f32")
-    sing_ms, sing_n = bem.timing_get("sing_elems")
-    fin_ms, fin_n = bem.timing_get("asm_finalize")
-    ne_ms, ne_n = bem.timing_get("ne_apply_part")
-    nt_ms, nt_n = bem.timing_get("ne_apply_f32")
-    rn_ms, rn_n = bem.timing_get("resid_n_f32a")
-    bem.timing_reset()
-    x32 = solver.solution()
-    log("timed stats", st)
-
-    # roofline of the dominant kernel: regular quadrature + assembly
-    evals = 36.0 * (M * E - 3 * E)
-    asm_avg_ms = asm_ms / max(asm_n, 1)
-    achieved_tflops = FLOP_PER_EVAL * evals / (asm_avg_ms * 1e-3) / 1e12
-    asm_total_ms = (asm_ms + sing_ms + fin_ms) / max(asm_n, 1)
-    asm_gbs = (8.0 * M * N + 8.0 * M * R) / (asm_total_ms * 1e-3) / 1e9
-    # the solve's operator: A^H (A v) in one pass over the row-major copy of A
-    # (algorithmic bytes: A once, v in, w out)
-    ne_bytes = 8.0 * M * N + 16.0 * N
-    ne_avg_us = ne_ms / ne_n * 1e3 if ne_n else None
-    ne_gbs = ne_bytes / (ne_avg_us * 1e-6) / 1e9 if ne_n else None
-
-    # the same solves run two at a time (two resident solvers on their own
-    # streams): one solve's assembly overlaps the other's solve
-    conc = None
-    if args.steps >= 4:
-        ks = np.full(args.steps, k, np.float32)
-        bem.batch_solve(mesh, ks[:2], bem.RHS_PLANE, [[0, 0, 1]], workers=2, restart=30, tol=1e-6,
-                        refine_tol=1e-8, max_refine=50)
-        bem.sync()
-        tc = time.perf_counter()
-        bem.batch_solve(mesh, ks, bem.RHS_PLANE, [[0, 0, 1]], workers=2, restart=30, tol=1e-6, refine_tol=1e-8,
-                        max_refine=50)
-        bem.sync()
-        conc = round(args.steps / (time.perf_counter() - tc), 3)
-        log("two concurrent solvers: %.3f solves/s" % conc)
-
-    # accuracy: plain all-FP64 GPU solve of the same discretisation (also timed:
-    # the 1x yardstick), and the rigid-sphere series
-    err_fp64 = None
-    fp64_ms = None
-    if not args.no_fp64:
-        solver64 = bem.Solver(mesh)
-        solver64.set_rhs(bem.RHS_PLANE, [[0, 0, 1]], 1.0)
-        t64 = []
-        for i in range(2):
-            ts = time.perf_counter()
-            solver64.assemble(k, bem.PREC_FP64)
-            st64 = solver64.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=5000, refine_tol=0, max_refine=0)
-            bem.sync()
-            t64.append((time.perf_counter() - ts) * 1e3)
-        fp64_ms = min(t64)
-        x64 = solver64.solution()
-        err_fp64 = float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64))
-        log("fp64 stats", st64, "ms", t64)
-        solver64.close()
-    pn = mesh.pnts().astype(np.float64)
-    ser = C.rigid_sphere_surface(a, k, pn[:, 2] / np.linalg.norm(pn, axis=1))
-    err_series = float(np.linalg.norm(x32[:, 0] - ser) / np.linalg.norm(ser))
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(mesh, k, rho, c, args.cpu_seconds, threads)
-        log("cpu baseline", cpu)
-
-    plain = None
+    large = bench_large(bem, C, args.steps, args.warmup, fp64=fp64, cpu=not args.no_cpu_baseline,
+                        method=args.large_method)
+    log("large", {kk: v for kk, v in large.items() if kk != "cpu_baseline"})
+    sections = {}
+    if not args.no_precision:
+        sections["precision"] = bench_precision(bem, C, args.precision_steps, fp64=fp64)
+        log("precision", sections["precision"])
     if not args.no_plain:
-        plain = bench_plain(bem, C, args.plain_reps, args.plain_subdiv)
-        log("plain", plain)
-    batching = None
+        sections["plain"] = bench_plain(bem, C, args.plain_reps, args.plain_subdiv)
+        log("plain", sections["plain"])
     if not args.no_batching:
-        batching = bench_batching(bem, C, args.workers, fp64=not args.no_fp64)
-        log("batching", batching)
-    large = None
-    if not args.no_large:
-        large = bench_large(bem, C, fp64=not args.no_fp64)
-        log("large", large)
-    serving = None
+        sections["batching"] = bench_batching(bem, C, args.workers, fp64=fp64)
+        log("batching", sections["batching"])
     if not args.no_serving:
-        serving = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
-        log("serving", serving)
+        sections["serving"] = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
+        log("serving", sections["serving"])
 
-    value = 1000.0 / ms_per_step * R
+    head_keys = ("value", "ms_per_step", "latency_ms", "roofline", "cpu_baseline", "workload", "N", "E", "M", "rhs")
     out = {
         "metric": "BEM solves/sec (surface pressure, <=1e-5 rel err vs FP64)",
-        "value": round(value, 3),
+        "value": round(large["value"], 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(large["ms_per_step"], 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64",
         "data": "synthetic",
-        "config": {"workload": "%s: rigid sphere icosphere s%d r=%.2f m, %g Hz, %d CHIEF, plane wave; "
-                               "FP32 assembly + %s with FP64 refinement" %
-                               (args.config, cfg["subdiv"], a, cfg["freq"], Cn,
-                                "GMRES(30,1e-6) on normal equations" if method == bem.SOLVE_GMRES else "QR"),
-                   "N": N, "E": E, "M": M, "rhs": R},
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3)},
-        "solves_per_s_two_concurrent_solvers": conc,
-        "assembly_gbs": round(asm_gbs, 2),
-        "assembly_ms": round(asm_total_ms, 3),
-        "solve": {"iterations": st["iterations"], "refinements": st["refinements"], "converged": st["converged"],
-                  "residual": st["residual"]},
-        "rel_err_vs_fp64": err_fp64,
-        "rel_err_vs_series": err_series,
-        "fp64_gpu_ms_per_solve": fp64_ms,
-        "speedup_vs_fp64_gpu": (fp64_ms / ms_per_step) if fp64_ms else None,
-        "roofline": {"bound": "valu", "kernel": "asm_tile_f32", "achieved": round(achieved_tflops, 3),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic("asm_tile_f32", args.config), "avg_ms": round(asm_avg_ms, 4),
-                     "flops_per_launch": FLOP_PER_EVAL * evals},
-        "roofline_ne": {"bound": "hbm", "kernel": "ne_apply_part", "achieved": ne_gbs and round(ne_gbs, 1),
-                        "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ne_gbs and round(ne_gbs / PEAK_HBM_GBS, 4),
-                        "bytes_per_launch": ne_bytes, "avg_us": ne_avg_us and round(ne_avg_us, 2),
-                        "avg_us_with_reduce": nt_n and round(nt_ms / nt_n * 1e3, 2),
-                        "traffic": measured_traffic("ne_apply_part", args.config), "launches": ne_n},
-        "cpu_baseline": cpu and {kk: cpu[kk] for kk in ("value", "unit", "cores", "kind", "sample")},
-        "plain": plain,
-        "batching": batching,
-        "large": large,
-        "serving": serving,
+        "config": {"workload": large["workload"], "N": large["N"], "E": large["E"], "M": large["M"],
+                   "rhs": large["rhs"]},
+        "latency_ms": large["latency_ms"],
+        "roofline": large["roofline"],
+        "cpu_baseline": large.get("cpu_baseline") and {kk: large["cpu_baseline"][kk] for kk in (
+            "value", "unit", "cores", "kind", "sample", "extrapolated", "cpu_model")},
     }
+    out.update({kk: v for kk, v in large.items() if kk not in head_keys})
+    out.update(sections)
     print(json.dumps(out), flush=True)
 
 

@@ -79,7 +79,8 @@ int bem_mesh_get_bb(const bem_mesh *m, float *bb6);
  * bounding box that are inside the surface (odd ray-crossing count) and at least
  * threshold away from every node.  The reference reseeds cuRAND per candidate
  * and casts the ray along the same uniforms; here a SplitMix64 stream seeded by
- * `seed` draws the position and an independent ray direction on the sphere. */
+ * `seed` draws the position and three independent ray directions on the
+ * sphere, and the majority of their crossing parities decides. */
 int bem_mesh_gen_chief(bem_mesh *m, int num, float threshold, unsigned long long seed);
 /* Extension: set the CHIEF points explicitly (copies them). */
 int bem_mesh_set_chief(bem_mesh *m, const bem_cart_coord *pnts, int num);
@@ -101,6 +102,16 @@ int bem_mesh_gen_ellipsoid(bem_mesh *m, int subdiv, float ax, float ay, float az
                            int lmax, float amp, unsigned long long seed);
 /* Extension: signed volume (negative for the reference's winding). */
 double bem_mesh_signed_volume(const bem_mesh *m);
+/* Extension: make the winding consistent across every edge-connected part
+ * of the surface and point each part's normals (p1-p3)x(p2-p3) into the body
+ * (negative signed volume), as the reference's meshes are wound; *flipped
+ * (may be NULL) receives the number of elements whose winding changed. */
+int bem_mesh_fix_orientation(bem_mesh *m, int *flipped);
+/* Extension: the CHIEF generator's inside test (inObj, numerical.cu:119-131)
+ * for n points: inside[i] = 1 when the majority of three rays in seeded
+ * random directions cross the surface an odd number of times. */
+int bem_mesh_points_inside(const bem_mesh *m, const bem_cart_coord *pts, int n, unsigned long long seed,
+                           int *inside);
 /* cmptSurfArea (mesh.cu:1091-1117), computed on the host. */
 float bem_mesh_surf_area(const bem_mesh *m);
 
@@ -116,6 +127,19 @@ int bem_quad_get(double *pnts, double *wgts);
 int bem_set_medium(float density, float speed);
 /* Extension: strength q of the incident point source (STRENGTH, numerical.h:128-130). */
 int bem_set_source_strength(float q);
+/* Extension: FP64 near-singular integration.  The reference integrates every
+ * element that does not hold the collocation point with the 6x6 rule
+ * (g_h_c_l_nsgl, mesh.cu:597-672; atomicFuncs.cu:75-76 excludes only the
+ * element's own vertices), whose error reaches 1e-3 for a point a quarter of
+ * the element size away.  With factor > 0, every (row, element) pair whose
+ * distance is below factor times the element's longest edge is integrated in
+ * FP64 by adaptive subdivision (midpoint splits, at most max_level deep) and
+ * its entries of A, the diagonal and the boundary-condition term are
+ * corrected after assembly, in a fixed order per row.  Applies to every later
+ * assembly (FP32 and FP64, collocation matrix and field operator; not to
+ * bem_ext_pressure and bem_solver_field).  factor = 0 (default) keeps the
+ * reference's rule everywhere; max_level default 12. */
+int bem_set_near_field(float factor, int max_level);
 
 /* -------------------------------------------------- reference entry points --- */
 
@@ -141,7 +165,11 @@ int bem_lsq_solve(bem_complex *A, int m, int n, int lda, bem_complex *B, int nrh
                   bem_complex *Q);
 /* GMRES (GMRES.h:104-105, GMRES.cu:145-279): square m x m (lda = m) system,
  * unrestarted, initial guess in x, stops when the residual estimate
- * |err|/||b|| < threshold or after maxIter steps; solution in x. */
+ * |err|/||b|| < threshold or after maxIter steps; solution in x.  The Krylov
+ * basis is kept in HBM in FP64 ((min(maxIter, m) + 1) * m * 16 bytes) and the
+ * packed Hessenberg matrix on the host (about min(maxIter, m)^2 * 8 bytes);
+ * only when the basis would exceed half of the free HBM is the space
+ * restarted at the largest length that fits. */
 int bem_gmres(const bem_complex *A, const bem_complex *b, int m, int maxIter, float threshold,
               bem_complex *x);
 /* genExtPressure (mesh.h:425-426, mesh.cu:1038-1073): pressure at one
@@ -283,7 +311,9 @@ long bem_server_submit(bem_server *s, const bem_mesh *m, float k, int precision,
  * and its record, and forget it. */
 int bem_server_wait(bem_server *s, long id, bem_zcomplex *x, int ldx, bem_request_info *info);
 int bem_server_status(bem_server *s, bem_server_info *info);
-/* Cancel queued requests, let running ones finish, free everything. */
+/* Cancel queued requests (their waiters return EXIT_FAILURE), let running
+ * ones finish, wait until every thread inside bem_server_wait has returned,
+ * then free everything.  No call may start on the server once this one has. */
 void bem_server_destroy(bem_server *s);
 /* HBM bytes a request of these sizes is admitted with (upper bound of what
  * its solve allocates). */

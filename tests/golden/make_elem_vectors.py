@@ -12,6 +12,10 @@ Regular cases integrate over the triangle in (xi1, xi2); singular cases (x at a
 vertex) integrate in polar coordinates centred on the vertex, where the 1/r
 singularity cancels against the area element.
 
+Near-singular cases ("near": x at 0.25 and 0.5 element sizes above the centroid or
+an edge midpoint) pin the opt-in FP64 near-field integration, where the 6x6 rule is
+off by more than 1e-5.
+
 Run: python tests/golden/make_elem_vectors.py
 """
 import json
@@ -131,6 +135,25 @@ def main():
                 r = singular_case(k, v, p)
                 cases.append({"kind": "singular", "k": k, "v": v, "p": p.tolist(),
                               "G": [[z.real, z.imag] for z in r["G"]]})
+    # near-singular points over the two small triangles (drawn after the cases
+    # above so that those stay the same)
+    # (coordinates and k rounded to FP32, the library's input precision, so
+    # that the device's FP64 near-field integrals can be compared at 1e-7)
+    for ti, p in enumerate(tris[:2]):
+        p = p.astype(np.float32).astype(np.float64)
+        cen = p.mean(axis=0)
+        hsz = max(np.linalg.norm(p[0] - p[1]), np.linalg.norm(p[1] - p[2]), np.linalg.norm(p[2] - p[0]))
+        cr = np.cross(p[0] - p[2], p[1] - p[2])
+        n = cr / np.linalg.norm(cr)
+        mid = 0.5 * (p[0] + p[1])
+        for where, base in (("centroid", cen), ("edge", mid)):
+            for dist in (0.25, 0.5):
+                x = (base + dist * hsz * n).astype(np.float32).astype(np.float64)
+                for k in (float(np.float32(0.5 / hsz)), 30.0):
+                    r = regular_case(k, x, p)
+                    cases.append({"kind": "near", "where": where, "k": k, "x": x.tolist(), "p": p.tolist(),
+                                  "dist_over_h": dist, "G": [[z.real, z.imag] for z in r["G"]],
+                                  "H": [[z.real, z.imag] for z in r["H"]], "C": r["C"]})
     with open(os.path.join(HERE, "elem_vectors.json"), "w") as f:
         json.dump({"generator": "tests/golden/make_elem_vectors.py", "cases": cases}, f, indent=1)
     print("wrote %d cases" % len(cases))

@@ -170,6 +170,86 @@ def test_find_bb_and_chief_generator():
         bem.Mesh().icosphere(2, a).find_bb(1e-4).gen_chief(4, 0.5, seed=1)  # impossible threshold
 
 
+def _winding_number(pnts, elems, xs):
+    """Generalised winding number of a closed triangle surface at each x: the
+    sum of the signed solid angles of its triangles (Van Oosterom & Strackee,
+    IEEE Trans. Biomed. Eng. 30, 1983) over 4 pi; +-1 inside, 0 outside.  An
+    inside test that casts no rays."""
+    P = np.asarray(pnts, np.float64)[np.asarray(elems)]
+    out = np.zeros(len(xs))
+    for i, x in enumerate(np.asarray(xs, np.float64)):
+        a, b, c = P[:, 0] - x, P[:, 1] - x, P[:, 2] - x
+        la, lb, lc = (np.linalg.norm(v, axis=1) for v in (a, b, c))
+        num = np.einsum("ij,ij->i", a, np.cross(b, c))
+        den = la * lb * lc + (a * b).sum(1) * lc + (b * c).sum(1) * la + (c * a).sum(1) * lb
+        out[i] = 2.0 * np.arctan2(num, den).sum() / (4.0 * np.pi)
+    return out
+
+
+@pytest.mark.parametrize("case", ["ellipsoid", "sphere.obj"])
+def test_inside_test_matches_winding_number_and_oracle(case):
+    """The CHIEF generator's 3-ray majority inside test against the winding
+    number (no rays) and the oracle's one-ray inObj (numerical.cu:119-131,
+    rayTrnglInt mesh.cu:1184-1189) at 1200 seeded points in the bounding box of
+    a perturbed ellipsoid and of the reference's own sphere.obj.  The
+    reference's triangle test compares normalised edge cross products within
+    EPS = 5e-5 (mesh.cu:1167-1182, numerical.h:124-126), which fails for hits
+    near an edge, where one cross product is short and its direction is set by
+    rounding: its one-ray answer is wrong at about 3 % of such points, the
+    majority of three Moller-Trumbore rays at none."""
+    if case == "ellipsoid":
+        m = bem.Mesh().ellipsoid(3, (0.075, 0.09, 0.11), nbumps=8, lmax=6, amp=0.05, seed=7)
+    else:
+        m = bem.Mesh().read_obj(os.path.join(C.GOLDEN, "sphere.obj"))
+    p, e, _ = C.mesh_arrays(m)
+    lo, hi = p.min(0), p.max(0)
+    rng = np.random.default_rng(11)
+    xs = (lo + (hi - lo) * rng.random((1200, 3))).astype(np.float32)
+    wn = _winding_number(p, e, xs)
+    assert np.all(np.minimum(np.abs(wn), np.abs(np.abs(wn) - 1)) < 1e-6)  # closed surface
+    truth = np.abs(wn) > 0.5
+    got = m.points_inside(xs, seed=3)
+    dirs = C.sphere_points(12, len(xs), 1.0)
+    orc = np.array([O.ray_inside(x, d, p, e) for x, d in zip(xs, dirs)], bool)
+    print("%s: %d of %d inside; library disagrees with the winding number at %d, the one-ray oracle at %d" %
+          (case, truth.sum(), len(xs), (got != truth).sum(), (orc != truth).sum()))
+    assert 300 < truth.sum() < 900
+    assert np.array_equal(got, truth)
+    assert (orc != truth).mean() < 0.06
+
+
+def test_chief_points_are_inside_a_nonconvex_body():
+    """CHIEF points generated on a perturbed ellipsoid (not convex where the
+    bumps dent it) lie inside by the winding number."""
+    m = bem.Mesh().ellipsoid(3, (0.075, 0.09, 0.11), nbumps=8, lmax=6, amp=0.05, seed=7)
+    m.find_bb(1e-4).gen_chief(32, 0.01, seed=9)
+    p, e, _ = C.mesh_arrays(m)
+    assert np.all(np.abs(_winding_number(p, e, m.chief())) > 0.99)
+
+
+def test_fix_orientation():
+    """Randomly flipped windings, and a part wound outward as a whole, are
+    repaired to the reference's winding (normals into the body)."""
+    a = bem.Mesh().icosphere(2, 0.1)
+    p, e0, bc = C.mesh_arrays(a)
+    assert a.fix_orientation() == 0  # already consistent and inward
+    e = e0.copy()
+    flip = np.random.default_rng(4).random(len(e)) < 0.3
+    e[flip] = e[flip][:, [0, 2, 1]]
+    b = bem.Mesh().set(p, e, (0, 1, 0))
+    assert b.fix_orientation() == flip.sum()
+    got = b.elems()["nodes"]
+    assert np.array_equal(got, e0)  # the flipped ones are swapped back exactly
+    assert b.signed_volume() < 0
+    # two spheres, the second wound outward as a whole
+    p2 = np.vstack([p - [0.3, 0, 0], p + [0.3, 0, 0]])
+    e2 = np.vstack([e0, e0[:, [0, 2, 1]] + p.shape[0]])
+    t = bem.Mesh().set(p2, e2, (0, 1, 0))
+    assert t.fix_orientation() == len(e0)
+    assert np.array_equal(t.elems()["nodes"], np.vstack([e0, e0 + p.shape[0]]))
+    assert t.signed_volume() < 0
+
+
 def test_wrt_cplx_mat_golden_format():
     v = np.array([[1.5 + 2.25j, -3.125 + 0.5j]])
     with tempfile.NamedTemporaryFile("r", suffix=".txt", delete=False) as f:
