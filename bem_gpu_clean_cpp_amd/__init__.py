@@ -131,6 +131,7 @@ _SIGS = {
     "bem_lsq_solve": (_I, [_P, _I, _I, _I, _P, _I, _I, _P]),
     "bem_gmres": (_I, [_P, _P, _I, _I, _F, _P]),
     "bem_ext_pressure": (bem_complex, [_F, _P, bem_cart_coord, bem_cart_coord, _P]),
+    "bem_ext_pressure_noalloc": (_I, [_F, _P, bem_cart_coord, _P, _I, _P, _P, _P]),
     "bem_wrt_cplx_mat": (_I, [_P, _I, _I, _I, ctypes.c_char_p]),
     "bem_print_complex_matrix": (None, [_P, _I, _I, _I]),
     "bem_solver_create": (_P, [_P]),
@@ -407,6 +408,14 @@ def ext_pressure(k, mesh, src, x, surf_p):
     if np.isnan(r.x):
         raise BemError("genExtPressure failed: %s" % last_error())
     return complex(r.x, r.y)
+
+
+def ext_pressure_noalloc(k, mesh, src, xs_ptr, P, surf_ptr, out_ptr, stream=None):
+    """bem_ext_pressure_noalloc: the field at P points from device buffers
+    (raw device addresses), nothing allocated; queued on `stream`
+    (a hipStream_t handle, None: the library's stream), no wait."""
+    _check(lib().bem_ext_pressure_noalloc(k, mesh.handle, bem_cart_coord((_F * 3)(*src)), xs_ptr, P, surf_ptr,
+                                          out_ptr, stream), "bem_ext_pressure_noalloc")
 
 
 def wrt_cplx_mat(mat, path):

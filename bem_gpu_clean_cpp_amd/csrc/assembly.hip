@@ -1295,6 +1295,106 @@ static int near_correct(DeviceProblem &dp, double k, double rw, C2 *A, long ldA,
     return EXIT_SUCCESS;
 }
 
+// ------------------------------------------- field without allocations ---
+// The exterior field for callers that cannot allocate device memory (code
+// under HIP graph capture, serving loops): the partial sums, the points in
+// the kernels' float4 layout and the source point live in one file-scope
+// device array of fixed size.  Each device holds its own copy of the array;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// on that device, kept per device below.  Every call on a device shares that
+// device's array, so calls that overlap on one device race.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// the scratch array's device address on the current device (looked up once per device)
+int scratch_address(void **out) {
+    int dev = 0;
+    HIP_CALL(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) BEM_FAIL("device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIP_CALL(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) BEM_FAIL("no device address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return EXIT_SUCCESS;
+}
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+}  // namespace
+
+// 12-byte points (cartCoord) to the kernels' float4 layout, and the
+// incident point source's position next to them
+__global__ void k_pack_field_points(const float *__restrict__ xs, int P, float4 *__restrict__ xs4, float sx, float sy,
+                                    float sz, float *__restrict__ params) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P) xs4[p] = make_float4(xs[3 * p], xs[3 * p + 1], xs[3 * p + 2], 0.f);
+    if (p == 0) {
+        params[0] = sx;
+        params[1] = sy;
+        params[2] = sz;
+    }
+}
+
+size_t field_noalloc_bytes(int E, int P) {
+    const size_t nch = (size_t)(E + kFieldChunk - 1) / kFieldChunk;
+    return round256(sizeof(float4) * (size_t)P) + 256 + sizeof(float2) * nch * (size_t)P;
+}
+
+size_t field_noalloc_capacity() { return kNoAllocScratchBytes; }
+
+int field_fp32_noalloc(const DeviceProblem &dp, float k, const float *xs, int P, const float2 *X, float strength,
+                       const float *src, float2 *out, hipStream_t s) {
+    if (P <= 0) return EXIT_SUCCESS;
+    const size_t need = field_noalloc_bytes(dp.E, P);
+    if (need > kNoAllocScratchBytes)
+        BEM_FAIL("%d field points on %d elements need %zu bytes of scratch, more than its %zu", P, dp.E, need,
+                 kNoAllocScratchBytes);
+    void *base = nullptr;
+    HOST_CALL(scratch_address(&base));
+    unsigned char *b = (unsigned char *)base;
+    float4 *xs4 = (float4 *)b;
+    float *params = (float *)(b + round256(sizeof(float4) * (size_t)P));
+    float2 *part = (float2 *)(b + round256(sizeof(float4) * (size_t)P) + 256);
+    k_pack_field_points<<<(P + 255) / 256, 256, 0, s>>>(xs, P, xs4, src[0], src[1], src[2], params);
+    HIP_CALL(hipGetLastError());
+    const Config &cfg = config();
+    QuadTables qt;
+    quad_tables(qt);
+    FieldArgs a;
+    a.xs = xs4;
+    a.recs = dp.rec32;
+    a.X = X;
+    a.ldx = dp.N;
+    a.part = part;
+    a.P = P;
+    a.E = dp.E;
+    a.R = 1;
+    a.r0 = 0;
+    a.k = k;
+    a.kf = (float)((double)k / (2.0 * kPi));
+    a.rw = (float)((double)cfg.density * (double)(k * cfg.speed));
+    for (int q = 0; q < kNQ; q++) {
+        a.q.xi1[q] = qt.xi1[q];
+        a.q.xi2[q] = qt.xi2[q];
+        a.q.w[q] = qt.w[q];
+    }
+    const int nch = (dp.E + kFieldChunk - 1) / kFieldChunk;
+    field_part<<<dim3((P + 63) / 64, nch), 64, 0, s>>>(a);
+    HIP_CALL(hipGetLastError());
+    field_finalize<<<dim3((P + 255) / 256, 1), 256, 0, s>>>(xs4, P, 1, nch, part, (double)k, BEM_RHS_POINT,
+                                                            (double)strength, params, out, P);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
 // --------------------------------------------------------------- host ---
 // Launch shape of the assembly: qg lanes per row (1 or 4) and P column tiles,
 // P x ceil(M / (64 / qg)) waves in all.  More tiles cost more halo elements

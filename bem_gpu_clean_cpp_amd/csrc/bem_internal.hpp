@@ -229,6 +229,15 @@ int rhs_fp64(DeviceProblem &dp, double k, double2 *B, long ldB, const RhsSpec &r
 int field_fp32(DeviceProblem &dp, float k, const float4 *xs, int P, const float2 *X, long ldx, int R,
                const RhsSpec &rhs, float2 *out, long ldo, hipStream_t s);
 
+// The same field for one right-hand side (an incident point source of the
+// given strength at src) with every temporary in the library's fixed-size
+// static device scratch: no allocation.  xs: P points of 3 floats, X: N nodal
+// values, out: P values, all in device memory; queued on s, no wait.
+int field_fp32_noalloc(const DeviceProblem &dp, float k, const float *xs, int P, const float2 *X, float strength,
+                       const float *src, float2 *out, hipStream_t s);
+size_t field_noalloc_bytes(int E, int P);
+size_t field_noalloc_capacity();
+
 // Exterior field as a GEMM: fdp = mesh with the field points as extra rows.
 int field_matrix_fp32(DeviceProblem &fdp, float k, float2 *F, long ldF, double2 *fbterm, hipStream_t s);
 int field_apply_fp32(DeviceProblem &fdp, double k, const float2 *F, long ldF, const double2 *fbterm, const float2 *X,

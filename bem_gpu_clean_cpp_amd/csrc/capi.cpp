@@ -20,6 +20,7 @@ int stream(hipStream_t *s) {
     *s = c->stream;
     return EXIT_SUCCESS;
 }
+int stream_of_library(hipStream_t *s) { return stream(s); }
 
 int mesh_device(bem_mesh *m, hipStream_t s) {
     if (m->dp_version != m->m.version || m->dp.M == 0) {
@@ -379,6 +380,25 @@ bem_complex bem_ext_pressure(float k, const bem_mesh *mc, bem_cart_coord src, be
     }
     out = {o.x, o.y};
     return out;
+}
+
+int bem_ext_pressure_noalloc(float k, const bem_mesh *mc, bem_cart_coord src, const bem_cart_coord *xs, int P,
+                             const bem_complex *surfPressure, bem_complex *out, void *stream) {
+    if (!mc || !xs || !surfPressure || !out || P <= 0) BEM_FAIL("bad field arguments");
+    std::lock_guard<std::mutex> lk(g_mu);
+    const bem_mesh *m = mc;
+    // the mesh's device records come from an earlier call that could
+    // allocate them (bem_ext_pressure or bem_system on this mesh)
+    if (m->dp.M == 0 || m->dp_version != m->m.version)
+        BEM_FAIL("mesh is not resident on the device: call bem_ext_pressure or bem_system on it once first");
+    const size_t need = field_noalloc_bytes(m->dp.E, P);
+    if (need > field_noalloc_capacity())
+        BEM_FAIL("%d field points on %d elements need %zu bytes of scratch, more than its %zu", P, m->dp.E, need,
+                 field_noalloc_capacity());
+    hipStream_t s = (hipStream_t)stream;
+    if (!s) HOST_CALL(stream_of_library(&s));
+    return field_fp32_noalloc(m->dp, k, (const float *)xs, P, (const float2 *)surfPressure, config().strength,
+                              src.coords, (float2 *)out, s);
 }
 
 int bem_wrt_cplx_mat(const bem_complex *mat, int numRows, int numCols, int ldm, const char *file) {

@@ -177,6 +177,22 @@ int bem_gmres(const bem_complex *A, const bem_complex *b, int m, int maxIter, fl
  * q*G(x, src) - sum_e [sum_j (h_j - (ca/cb) g_j) p_j + (cc/cb) sum_j g_j]. */
 bem_complex bem_ext_pressure(float k, const bem_mesh *m, bem_cart_coord src, bem_cart_coord x,
                              const bem_complex *surfPressure);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): bem_ext_pressure for P points at once with every
+ * temporary in a static device scratch array of 32 MiB per GPU, held until
+ * the process ends; nothing is allocated or freed.  xs (P points), the nodal
+ * surfPressure (N values) and out (P values) are DEVICE pointers; the work is
+ * queued on `stream` (a hipStream_t; NULL: the library's stream) and the call
+ * returns without waiting.  The incident point source at src has the strength
+ * set by bem_set_source_strength.  The mesh must already be resident: call
+ * bem_ext_pressure or bem_system on it once first (those allocate its device
+ * records).  P must satisfy 16 P + 8 P ceil(E / 64) + 512 <= 32 MiB (about
+ * 3200 points at E = 81920), else EXIT_FAILURE with a message.  Every call on
+ * a GPU shares that GPU's scratch: calls that overlap on one GPU race.  Per
+ * point, the result is bit-identical to bem_ext_pressure's. */
+int bem_ext_pressure_noalloc(float k, const bem_mesh *m, bem_cart_coord src, const bem_cart_coord *xs, int P,
+                             const bem_complex *surfPressure, bem_complex *out, void *stream);
 /* wrtCplxMat (numerical.h:193-194, numerical.cu:556-578): "(%f,%f) " rows. */
 int bem_wrt_cplx_mat(const bem_complex *mat, int numRows, int numCols, int ldm, const char *file);
 /* printComplexMatrix (numerical.cu:81-93) on stdout. */
