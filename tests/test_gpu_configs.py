@@ -4,7 +4,7 @@ Batching (64 frequencies 100 Hz - 8 kHz on a seeded perturbed ellipsoid at
 icosphere subdivision 4, 2562 nodes, no CHIEF rows, 16 plane waves per
 frequency; FP32 assembly + lockstep GMRES(100) with MFMA products and FP64
 refinement): three of its frequencies -- the lowest (100 Hz), the middle one
-(1.98 kHz) and the highest (8 kHz, which also needs the most GMRES iterations of
+(926 Hz) and the highest (8 kHz, which also needs the most GMRES iterations of
 the 64, about 300) -- solved through the batch driver and checked against the
 FP64 oracle (assembly + LAPACK) at the lowest and the highest and the all-FP64
 device path at the middle one.  Bar: relative 2-norm error <= 1e-5 per
