@@ -78,6 +78,15 @@ class bem_tune_result(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class bem_product_tune_result(ctypes.Structure):
+    _fields_ = [("rhs_per_wave_n", ctypes.c_int), ("chunks_n", ctypes.c_int), ("rhs_per_wave_h", ctypes.c_int),
+                ("chunks_h", ctypes.c_int), ("best_ms_n", ctypes.c_double), ("default_ms_n", ctypes.c_double),
+                ("best_ms_h", ctypes.c_double), ("default_ms_h", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class bem_server_info(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int), ("running", ctypes.c_int), ("max_running", ctypes.c_int),
                 ("slots", ctypes.c_int), ("budget_bytes", ctypes.c_double), ("in_use_bytes", ctypes.c_double),
@@ -160,6 +169,7 @@ _SIGS = {
     "bem_server_destroy": (None, [_P]),
     "bem_request_footprint": (_D, [_I, _I, _I, _I, _I, _I, _I]),
     "bem_autotune_assembly": (_I, [_P, _F, _I, ctypes.POINTER(bem_tune_result)]),
+    "bem_autotune_products": (_I, [_I, _I, _I, ctypes.POINTER(bem_product_tune_result)]),
     "bem_tune_save": (_I, [ctypes.c_char_p]),
     "bem_tune_load": (_I, [ctypes.c_char_p]),
     "bem_tune_clear": (_I, []),
@@ -609,6 +619,14 @@ def autotune_assembly(mesh, k, precision=PREC_FP32):
     for meshes of its size class.  Returns the choice and both timings."""
     r = bem_tune_result()
     _check(lib().bem_autotune_assembly(mesh.handle, k, precision, ctypes.byref(r)), "autotune_assembly")
+    return r.as_dict()
+
+
+def autotune_products(M, N, nrhs):
+    """Time the multi-right-hand-side product shapes for an M x N matrix and
+    nrhs right-hand sides; the fastest is kept for its size class."""
+    r = bem_product_tune_result()
+    _check(lib().bem_autotune_products(M, N, nrhs, ctypes.byref(r)), "autotune_products")
     return r.as_dict()
 
 

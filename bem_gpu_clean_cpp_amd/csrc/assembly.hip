@@ -1518,12 +1518,17 @@ int autotune_assembly(const Mesh &m, double k, int precision, int *best_qg, int 
     return EXIT_SUCCESS;
 }
 
-// The tuned shapes as text lines "M-bucket E-bucket qg waves" (and back).
+// The tuned shapes as text lines: "M-bucket E-bucket qg waves" for the
+// assembly, then the products' "p ..." lines (gemm.hip), and back.
 int tune_save(const char *file) {
     FILE *f = fopen(file, "w");
     if (!f) BEM_FAIL("cannot write %s", file);
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    for (const auto &kv : g_tune) fprintf(f, "%d %d %d %d\n", kv.first.first, kv.first.second, kv.second.qg, kv.second.waves);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto &kv : g_tune)
+            fprintf(f, "%d %d %d %d\n", kv.first.first, kv.first.second, kv.second.qg, kv.second.waves);
+    }
+    product_tune_save(f);
     fclose(f);
     return EXIT_SUCCESS;
 }
@@ -1531,17 +1536,25 @@ int tune_save(const char *file) {
 int tune_load(const char *file) {
     FILE *f = fopen(file, "r");
     if (!f) BEM_FAIL("cannot read %s", file);
-    int a, b, q, w;
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    while (fscanf(f, "%d %d %d %d", &a, &b, &q, &w) == 4)
-        if ((q == 1 || q == 4) && w > 0) g_tune[{a, b}] = TileShape{q, w};
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        int a, b, q, w;
+        if (product_tune_parse(line)) continue;
+        if (sscanf(line, "%d %d %d %d", &a, &b, &q, &w) == 4 && (q == 1 || q == 4) && w > 0) {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[{a, b}] = TileShape{q, w};
+        }
+    }
     fclose(f);
     return EXIT_SUCCESS;
 }
 
 void tune_clear() {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune.clear();
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune.clear();
+    }
+    product_tune_clear();
 }
 
 // Upper bound of the HBM bytes device_problem_build allocates for a mesh of

@@ -211,6 +211,11 @@ int autotune_assembly(const Mesh &m, double k, int precision, int *best_qg, int 
 int tune_save(const char *file);
 int tune_load(const char *file);
 void tune_clear();
+// multi-right-hand-side product tuning (gemm.hip)
+int autotune_products(long M, long N, int R, int *shape, double *ms, hipStream_t s);
+void product_tune_save(FILE *f);
+bool product_tune_parse(const char *line);
+void product_tune_clear();
 
 struct RhsSpec {
     int kind = BEM_RHS_POINT;

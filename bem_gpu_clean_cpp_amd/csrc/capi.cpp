@@ -858,6 +858,27 @@ int bem_autotune_assembly(const bem_mesh *m, float k, int precision, bem_tune_re
     if (out) *out = r;
     return EXIT_SUCCESS;
 }
+int bem_autotune_products(int M, int N, int nrhs, bem_product_tune_result *out) {
+    if (M < N || N <= 0 || nrhs <= 0 || nrhs > kMaxRhs) BEM_FAIL("bad product sizes %d x %d, %d right-hand sides", M, N,
+                                                                   nrhs);
+    std::lock_guard<std::mutex> lk(g_mu);
+    hipStream_t s;
+    HOST_CALL(stream(&s));
+    int shape[4] = {0, 0, 0, 0};
+    double ms[4] = {0, 0, 0, 0};
+    HOST_CALL(autotune_products(M, N, nrhs, shape, ms, s));
+    if (out) {
+        out->rhs_per_wave_n = shape[0];
+        out->chunks_n = shape[1];
+        out->rhs_per_wave_h = shape[2];
+        out->chunks_h = shape[3];
+        out->best_ms_n = ms[0];
+        out->default_ms_n = ms[1];
+        out->best_ms_h = ms[2];
+        out->default_ms_h = ms[3];
+    }
+    return EXIT_SUCCESS;
+}
 int bem_tune_save(const char *file) {
     if (!file) BEM_FAIL("null file");
     return tune_save(file);

@@ -24,6 +24,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
+#include <map>
+#include <mutex>
 
 #include "bem_internal.hpp"
 
@@ -279,6 +283,190 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
     }
 }
 
+// ------------------------------------------- FP32, 32 right-hand sides ---
+// The same products for a group of up to 32 right-hand sides on
+// v_mfma_f32_32x32x2f32 (32 x 32 output, 2 steps of k per instruction).
+// Against the 16x16x4 kernels a wave covers twice the rows (or columns) for
+// the same operand loads, so the right-hand-side block is read half as often
+// per flop, and each lane holds its share of the 32 x 32 output in 16
+// registers per accumulator.  The operands are prefetched one k step at a
+// time into a ring of kRing buffers (each buffer: one 16-byte load of A and
+// one load of the right-hand-side block per lane), so several steps of loads
+// are in flight while the oldest is multiplied.
+//
+// Output layout of v_mfma_f32_32x32x2f32 (checked on the device with an
+// in-bounds one-hot probe): lane l, register v holds D[8 (v / 4) + 4 (l / 32)
+// + v % 4][l % 32]; operands A[i][k] from lane i + 32 k, B[k][j] from lane
+// j + 32 k.
+typedef float v16f __attribute__((ext_vector_type(16)));
+constexpr int kRingN = 4;   // Y = A X: buffers in the ring
+constexpr int kWavesN = 3;  // waves per SIMD it is built for
+constexpr int kRingH = 2;   // Z = A^H T
+constexpr int kStepsH = 4;  // k steps per buffer of the adjoint product
+constexpr int kWavesH = 2;
+
+__device__ __forceinline__ int d32_row(int lane, int v) { return 8 * (v / 4) + 4 * (lane >> 5) + (v & 3); }
+
+// part[(chunk * R + n) * ldp + i] = sum_{j in chunk} A[i, j] X[j, n] for
+// n in [rg0, rg0 + 32): wave = 64 complex rows.  Lane (i = l % 32, k = l / 32)
+// loads complex rows row0 + 2i, row0 + 2i + 1 of column j + k (four real rows:
+// one A operand for each of four MFMAs) and X[j + k][rg0 + i] (the B operand,
+// real and imaginary part for two MFMAs).  MFMA t accumulates real row 4 i' + t
+// of the tile; Re/Im of Y combine lane-locally:
+//   Y[2i'] = (P0 - Q1, Q0 + P1), Y[2i' + 1] = (P2 - Q3, Q2 + P3)
+// with P_t against Re X, Q_t against Im X.  Needs lda >= roundup(M, 64).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesN, kWavesN))) void gemm_n_mfma32(
+    const float2 *__restrict__ A, long lda, int M, int N, const float2 *__restrict__ X, long ldx, int R, int rg0,
+    int kc, float2 *__restrict__ part, long ldp) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = (blockIdx.x * 4 + wave) * 64;
+    if (row0 >= M) return;
+    const int chunk = blockIdx.y;
+    const int j0 = chunk * kc, j1 = min(N, j0 + kc);
+    const int i = lane & 31, k = lane >> 5;
+    v16f acc[4][2];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        acc[t][0] = (v16f){0};
+        acc[t][1] = (v16f){0};
+    }
+    const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * i);
+    const long lda4 = lda / 2;
+    // right-hand sides past R read column R - 1 (their results are not written)
+    const float2 *Xr = X + (size_t)min(rg0 + i, R - 1) * ldx;
+    struct Buf {
+        float4 a;
+        float2 x;
+    };
+    auto load = [&](int jb, Buf &B) {
+        const int Jc = min(jb + k, j1 - 1);
+        B.a = Ab[(size_t)Jc * lda4];
+        B.x = Xr[Jc];
+    };
+    auto load_masked = [&](int jb, Buf &B) {
+        const int J = jb + k;
+        const bool in = J < j1;
+        const int Jc = min(J, j1 - 1);
+        B.a = keep4(in, in, Ab[(size_t)Jc * lda4]);
+        B.x = keep2(in, Xr[Jc]);
+    };
+    auto mul = [&](const Buf &B) {
+        const float av[4] = {B.a.x, B.a.y, B.a.z, B.a.w};
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.x, acc[t][0], 0, 0, 0);
+            acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.y, acc[t][1], 0, 0, 0);
+        }
+    };
+    Buf b[kRingN];
+    const int jend = j0 + (j1 - j0) / (2 * kRingN) * (2 * kRingN);  // end of the full rounds of the ring
+#pragma unroll
+    for (int w = 0; w < kRingN - 1; w++) load(j0 + 2 * w, b[w]);
+    for (int jb = j0; jb < jend; jb += 2 * kRingN) {
+#pragma unroll
+        for (int w = 0; w < kRingN; w++) {
+            load(jb + 2 * (w + kRingN - 1), b[(w + kRingN - 1) % kRingN]);
+            __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
+            mul(b[w]);
+        }
+    }
+    for (int jb = jend; jb < j1; jb += 2) {
+        load_masked(jb, b[0]);
+        mul(b[0]);
+    }
+    const int n = rg0 + i;
+    if (n >= R) return;
+    float2 *pc = part + ((size_t)chunk * R + n) * ldp;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int r = row0 + 2 * d32_row(lane, v);
+        if (r < M) pc[r] = make_float2(acc[0][0][v] - acc[1][1][v], acc[0][1][v] + acc[1][0][v]);
+        if (r + 1 < M) pc[r + 1] = make_float2(acc[2][0][v] - acc[3][1][v], acc[2][1][v] + acc[3][0][v]);
+    }
+}
+
+// part[(chunk * R + n) * ldp + j] = sum_{i in chunk} conj(A[i, j]) T[i, n]
+// for n in [rg0, rg0 + 32): wave = 32 columns.  Lane (j = l % 32, k = l / 32)
+// loads complex rows r, r + 1 (r = ib + 4u + 2k) of column c0 + j and of
+// right-hand side rg0 + j: the k index runs over real rows, with
+//   Re Z = sum Ar Tr + Ai Ti,  Im Z = sum Ar Ti - Ai Tr.
+// Rows read in pairs: row r + 1 may be M, which needs lda, ldt > M for odd M.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesH, kWavesH))) void gemm_h_mfma32(
+    const float2 *__restrict__ A, long lda, int M, int N, const float2 *__restrict__ Tv, long ldt, int R, int rg0,
+    int kr, float2 *__restrict__ part, long ldp) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c0 = (blockIdx.x * 4 + wave) * 32;
+    if (c0 >= N) return;
+    const int chunk = blockIdx.y;
+    const int i0 = chunk * kr, i1 = min(M, i0 + kr);
+    const int j = lane & 31, k = lane >> 5;
+    v16f zr = (v16f){0}, zi = (v16f){0};
+    // columns past N and right-hand sides past R read the last valid one
+    const float2 *col = A + (size_t)min(c0 + j, N - 1) * lda;
+    const float2 *Tn = Tv + (size_t)min(rg0 + j, R - 1) * ldt;
+    const int rlast = (i1 - 1) & ~1;  // last even row of the chunk
+    struct Buf {
+        float4 a[kStepsH], t[kStepsH];
+    };
+    auto load = [&](int ib, Buf &B) {
+#pragma unroll
+        for (int u = 0; u < kStepsH; u++) {
+            const int rc = min(ib + 4 * u + 2 * k, rlast);
+            B.a[u] = *reinterpret_cast<const float4 *>(col + rc);
+            B.t[u] = *reinterpret_cast<const float4 *>(Tn + rc);
+        }
+    };
+    auto load_masked = [&](int ib, Buf &B) {
+#pragma unroll
+        for (int u = 0; u < kStepsH; u++) {
+            const int row = ib + 4 * u + 2 * k;
+            const bool in0 = row < i1, in1 = row + 1 < i1;
+            const int rc = min(row, rlast);
+            B.a[u] = keep4(in0, in1, *reinterpret_cast<const float4 *>(col + rc));
+            B.t[u] = keep4(in0, in1, *reinterpret_cast<const float4 *>(Tn + rc));
+        }
+    };
+    auto mul = [&](const Buf &B) {
+#pragma unroll
+        for (int u = 0; u < kStepsH; u++) {
+            const float4 a = B.a[u], t = B.t[u];
+            zr = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, t.x, zr, 0, 0, 0);
+            zi = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, t.y, zi, 0, 0, 0);
+            zr = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, t.y, zr, 0, 0, 0);
+            zi = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, -t.x, zi, 0, 0, 0);
+            zr = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, t.z, zr, 0, 0, 0);
+            zi = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, t.w, zi, 0, 0, 0);
+            zr = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, t.w, zr, 0, 0, 0);
+            zi = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, -t.z, zi, 0, 0, 0);
+        }
+    };
+    Buf b[kRingH];
+    const int step = 4 * kStepsH;  // complex rows per buffer
+    const int iend = i0 + (i1 - i0) / (step * kRingH) * (step * kRingH);
+#pragma unroll
+    for (int w = 0; w < kRingH - 1; w++) load(i0 + w * step, b[w]);
+    for (int ib = i0; ib < iend; ib += step * kRingH) {
+#pragma unroll
+        for (int w = 0; w < kRingH; w++) {
+            load(ib + (w + kRingH - 1) * step, b[(w + kRingH - 1) % kRingH]);
+            __builtin_amdgcn_sched_barrier(0);
+            mul(b[w]);
+        }
+    }
+    for (int ib = iend; ib < i1; ib += step) {
+        load_masked(ib, b[0]);
+        mul(b[0]);
+    }
+    const int n = rg0 + j;
+    if (n >= R) return;
+    float2 *pc = part + ((size_t)chunk * R + n) * ldp;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int jc = c0 + d32_row(lane, v);
+        if (jc < N) pc[jc] = make_float2(zr[v], zi[v]);
+    }
+}
+
 // Y[n][i] = (b[n][i] -) sum_chunks part[(chunk * R + n) * ldp + i]  (fixed order)
 // (partials of type MF<T>::c2, output OC2 accumulated in S)
 template <typename T, typename S, typename OC2>
@@ -339,22 +527,78 @@ static void gemm_h_shape(long M, long N, int &nch, int &kr, bool long_chunks) {
     nch = (int)((M + kr - 1) / kr);
 }
 
+// the same rules for the 32-right-hand-side kernels, whose waves cover 64
+// rows (Y = A X) or 32 columns (Z = A^H T)
+static void gemm_n_shape32(long M, long N, int &nch, int &kc, bool long_chunks) {
+    const long tiles = (M + 63) / 64;
+    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
+    kc = (int)std::max(32L, ((N + target - 1) / target + 31) / 32 * 32);
+    nch = (int)((N + kc - 1) / kc);
+}
+static void gemm_h_shape32(long M, long N, int &nch, int &kr, bool long_chunks) {
+    const long tiles = (N + 31) / 32;
+    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
+    kr = (int)std::max(64L, ((M + target - 1) / target + 63) / 64 * 64);
+    nch = (int)((M + kr - 1) / kr);
+}
+
+// the 32-right-hand-side kernels take every FP32 product of more than 16
+static bool use32(int R) { return R > 16 && !getenv("BEM_GEMM_16X16"); }
+
+// ---------------------------------------------------------- tile tuner ---
+// The FP32 products a refinement follows (the Krylov and CG operators) can
+// take a tuned shape per size class instead of the rules above: the kernel
+// family (16 or 32 right-hand sides per wave) and the number of split-K
+// chunks, chosen by timing every candidate on the device
+// (bem_autotune_products).  Classes: the product, M and N in half-octave
+// buckets, R in steps of 16.  Chunk counts stay <= kMaxTunedChunks, which the
+// workspace sizes allow for.
+constexpr int kMaxTunedChunks = 16;
+namespace {
+struct PShape {
+    int wide = 0, chunks = 0;
+};
+std::mutex g_pt_mu;
+std::map<std::array<int, 4>, PShape> g_ptune;
+thread_local const PShape *t_pforce = nullptr;  // candidate under test
+int p_half_octave(long v) { return (int)std::lround(2.0 * std::log2((double)std::max(1L, v))); }
+std::array<int, 4> pkey(int which, long M, long N, int R) {
+    return {which, p_half_octave(M), p_half_octave(N), (R + 15) / 16};
+}
+bool tuned_product(int which, long M, long N, int R, PShape &out) {
+    if (t_pforce) {
+        out = *t_pforce;
+        return true;
+    }
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    auto it = g_ptune.find(pkey(which, M, N, R));
+    if (it == g_ptune.end()) return false;
+    out = it->second;
+    return true;
+}
+}  // namespace
+
 // partials of the FP32 adjoint product alone (Z = A^H T, N x R)
 size_t ws_gemm_h_f32(long M, long N, int R) {
-    int nch, kr;
+    int nch, kr, n32, k32;
     gemm_h_shape(M, N, nch, kr, false);
-    return (size_t)nch * (size_t)((N + 63) / 64 * 64) * (size_t)std::max(1, R) * sizeof(float2);
+    gemm_h_shape32(M, N, n32, k32, false);
+    return (size_t)std::max({nch, n32, kMaxTunedChunks}) * (size_t)((N + 63) / 64 * 64) * (size_t)std::max(1, R) *
+           sizeof(float2);
 }
 
 // partials of either product at FP64 for R right-hand sides
 size_t ws_gemm(long M, long N, int R) {
     int nch = 0;
     for (bool lc : {false, true}) {
-        int a, b, c, d;
+        int a, b, c, d, e, f, g, h;
         gemm_n_shape(M, N, a, b, lc);
         gemm_h_shape(M, N, c, d, lc);
-        nch = std::max({nch, a, c});
+        gemm_n_shape32(M, N, e, f, lc);
+        gemm_h_shape32(M, N, g, h, lc);
+        nch = std::max({nch, a, c, e, g});
     }
+    nch = std::max(nch, kMaxTunedChunks);
     const long ldp = (std::max(M, N) + 63) / 64 * 64;
     return (size_t)nch * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
 }
@@ -365,19 +609,35 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
                        bool long_chunks) {
     if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
     if (lda % 2 || lda < (M + 31) / 32 * 32) BEM_FAIL("gemm_n needs an even lda >= roundup(M, 32)");
+    PShape ps;
+    const bool tuned = sizeof(T) == 4 && long_chunks && !getenv("BEM_GEMM_16X16") && tuned_product(0, M, N, R, ps);
+    const bool w32 = sizeof(T) == 4 && (tuned ? ps.wide != 0 : use32(R));
+    if (w32 && lda < (M + 63) / 64 * 64) BEM_FAIL("gemm_n needs lda >= roundup(M, 64) for more than 16 right-hand sides");
     int nch, kc;
-    gemm_n_shape(M, N, nch, kc, long_chunks);
+    if (tuned) {
+        kc = (int)std::max(32L, ((N + ps.chunks - 1) / ps.chunks + 31) / 32 * 32);
+        nch = (int)((N + kc - 1) / kc);
+    } else if (w32) {
+        gemm_n_shape32(M, N, nch, kc, long_chunks);
+    } else {
+        gemm_n_shape(M, N, nch, kc, long_chunks);
+    }
     const long ldp = (M + 63) / 64 * 64;
     auto *part = (typename MF<T>::c2 *)ws;
     TimedRegion tr;
     timing_begin(tr, name, s);
     const unsigned gx = (unsigned)((M + 127) / 128);
     for (int rg = 0; rg < R; rg += 32) {
-        dim3 grid(gx, nch);
-        if (R - rg > 16)
-            gemm_n_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
-        else
-            gemm_n_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
+        if (w32) {
+            gemm_n_mfma32<<<dim3((unsigned)((M + 255) / 256), nch), 256, 0, s>>>(
+                A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc, (float2 *)part, ldp);
+        } else {
+            dim3 grid(gx, nch);
+            if (R - rg > 16)
+                gemm_n_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
+            else
+                gemm_n_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
+        }
         HIP_CALL(hipGetLastError());
     }
     timing_end(tr, s);
@@ -392,19 +652,34 @@ static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename
                        long ldz, int R, void *ws, const char *name, hipStream_t s, bool long_chunks) {
     if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
     if (lda % 2) BEM_FAIL("gemm_h needs an even lda");
+    PShape ps;
+    const bool tuned = sizeof(T) == 4 && long_chunks && !getenv("BEM_GEMM_16X16") && tuned_product(1, M, N, R, ps);
+    const bool w32 = sizeof(T) == 4 && (tuned ? ps.wide != 0 : use32(R));
     int nch, kr;
-    gemm_h_shape(M, N, nch, kr, long_chunks);
+    if (tuned) {
+        kr = (int)std::max(64L, ((M + ps.chunks - 1) / ps.chunks + 63) / 64 * 64);
+        nch = (int)((M + kr - 1) / kr);
+    } else if (w32) {
+        gemm_h_shape32(M, N, nch, kr, long_chunks);
+    } else {
+        gemm_h_shape(M, N, nch, kr, long_chunks);
+    }
     const long ldp = (N + 63) / 64 * 64;
     auto *part = (typename MF<T>::c2 *)ws;
     TimedRegion tr;
     timing_begin(tr, name, s);
     const unsigned gx = (unsigned)((N + 63) / 64);
     for (int rg = 0; rg < R; rg += 32) {
-        dim3 grid(gx, nch);
-        if (R - rg > 16)
-            gemm_h_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
-        else
-            gemm_h_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+        if (w32) {
+            gemm_h_mfma32<<<dim3((unsigned)((N + 127) / 128), nch), 256, 0, s>>>(
+                A, lda, (int)M, (int)N, (const float2 *)Tv, ldt, R, rg, kr, (float2 *)part, ldp);
+        } else {
+            dim3 grid(gx, nch);
+            if (R - rg > 16)
+                gemm_h_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+            else
+                gemm_h_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+        }
         HIP_CALL(hipGetLastError());
     }
     timing_end(tr, s);
@@ -450,6 +725,118 @@ int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long 
 int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
             void *ws, hipStream_t s) {
     return gemm_h_impl<double>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "mh_f64acc", s, true);
+}
+
+// ------------------------------------------------------ tuner driver ---
+// values in [-0.5, 0.5) from a counter hash: the tuner's operands
+__global__ void k_fill_tune(float2 *a, size_t n, unsigned long long seed) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        unsigned long long x = 2 * i + seed;
+        float v[2];
+        for (int h = 0; h < 2; h++, x++) {
+            unsigned long long z = x + 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            v[h] = (float)((z >> 40) * (1.0 / 16777216.0)) - 0.5f;
+        }
+        a[i] = make_float2(v[0], v[1]);
+    }
+}
+
+// Times both FP32 products of an M x N matrix with R right-hand sides for
+// every candidate shape (1 warm-up + 3 timed launches with the reduce, HIP
+// events on s) and keeps the fastest per product for the size class.
+// shape[0..1]: Y = A X (rhs per wave, chunks), shape[2..3]: Z = A^H T;
+// ms[0..3]: best and default time of Y = A X, then of Z = A^H T.
+int autotune_products(long M, long N, int R, int *shape, double *ms, hipStream_t s) {
+    if (M < N || N <= 0 || R <= 0 || R > kMaxRhs) BEM_FAIL("bad product sizes %ld x %ld, %d right-hand sides", M, N, R);
+    const long lda = (M + 63) / 64 * 64;
+    float2 *A = nullptr, *X = nullptr, *Y = nullptr;
+    char *ws = nullptr;
+    HOST_CALL(salloc(&A, (size_t)lda * N, s));
+    HOST_CALL(salloc(&X, (size_t)lda * R, s));
+    HOST_CALL(salloc(&Y, (size_t)lda * R, s));
+    HOST_CALL(salloc(&ws, ws_gemm(M, N, R), s));
+    k_fill_tune<<<2048, 256, 0, s>>>(A, (size_t)lda * N, 11);
+    k_fill_tune<<<256, 256, 0, s>>>(X, (size_t)lda * R, 12);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = EXIT_SUCCESS;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = EXIT_FAILURE;
+    std::vector<PShape> cand;
+    for (int wide : {0, 1})
+        for (int c : {1, 2, 3, 4, 6, 8, 12, 16}) cand.push_back(PShape{wide, c});
+    for (int which = 0; which < 2 && rc == EXIT_SUCCESS; which++) {
+        // the rules' own shape (no tuned entry) first: the default time
+        {
+            std::lock_guard<std::mutex> lk(g_pt_mu);
+            g_ptune.erase(pkey(which, M, N, R));
+        }
+        PShape best{};
+        double bms = 1e300, dms = -1;
+        for (int ci = -1; ci < (int)cand.size() && rc == EXIT_SUCCESS; ci++) {
+            t_pforce = ci < 0 ? nullptr : &cand[ci];
+            float t = 0;
+            for (int rep = 0; rep < 4 && rc == EXIT_SUCCESS; rep++) {
+                if (rep == 1) (void)hipEventRecord(e0, s);
+                rc = which == 0 ? gemm_n_f32(A, lda, M, N, X, lda, Y, lda, R, ws, s, true)
+                                : gemm_h_f32(A, lda, M, N, X, lda, Y, lda, R, ws, s, true);
+            }
+            t_pforce = nullptr;
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
+            const double avg = t / 3.0;
+            if (ci < 0) {
+                dms = avg;
+            } else if (avg < bms) {
+                bms = avg;
+                best = cand[ci];
+            }
+        }
+        if (rc != EXIT_SUCCESS) break;
+        {
+            std::lock_guard<std::mutex> lk(g_pt_mu);
+            g_ptune[pkey(which, M, N, R)] = best;
+        }
+        if (shape) {
+            shape[2 * which] = best.wide ? 32 : 16;
+            shape[2 * which + 1] = best.chunks;
+        }
+        if (ms) {
+            ms[2 * which] = bms;
+            ms[2 * which + 1] = dms;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    sfree(A, s);
+    sfree(X, s);
+    sfree(Y, s);
+    sfree(ws, s);
+    (void)hipStreamSynchronize(s);
+    HOST_CALL(rc);
+    return EXIT_SUCCESS;
+}
+
+// the tuned product shapes as text lines "p which Mb Nb Rb wide chunks"
+void product_tune_save(FILE *f) {
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    for (const auto &kv : g_ptune)
+        fprintf(f, "p %d %d %d %d %d %d\n", kv.first[0], kv.first[1], kv.first[2], kv.first[3], kv.second.wide,
+                kv.second.chunks);
+}
+bool product_tune_parse(const char *line) {
+    int w, a, b, r, wide, c;
+    if (sscanf(line, " p %d %d %d %d %d %d", &w, &a, &b, &r, &wide, &c) != 6) return false;
+    if ((w != 0 && w != 1) || c < 1 || c > kMaxTunedChunks || (wide != 0 && wide != 1)) return false;
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    g_ptune[{w, a, b, r}] = PShape{wide, c};
+    return true;
+}
+void product_tune_clear() {
+    std::lock_guard<std::mutex> lk(g_pt_mu);
+    g_ptune.clear();
 }
 
 // ------------------------------------------------ QR rank-nb update ---

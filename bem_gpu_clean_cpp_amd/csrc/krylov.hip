@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <memory>
 #include <cstdio>
 #include <cstdlib>
@@ -313,6 +315,8 @@ __global__ void kb_cvt(const float2 *__restrict__ in, long istride, double2 *__r
 
 // ---------------------------------------------------------------- driver ---
 constexpr int kMaxChunk = 8;  // Arnoldi steps between reads of the Hessenberg columns
+constexpr size_t kMaxGraphs = 512;  // captured Arnoldi chunks kept per workspace
+constexpr int kGraphAfter = 2;       // eager launches of a chunk before it is captured
 
 struct KrylovWS {
     int n = 0, m = 0, R = 0, restart = 0, nblk = 0;
@@ -514,7 +518,23 @@ struct GmresCache {
     // the update coefficients (device-to-host reads of pageable memory are
     // staged, and each one stalls the host loop)
     double2 *pin = nullptr;
+    // Arnoldi chunks captured as HIP graphs, by (first step, steps, second
+    // Gram-Schmidt pass): the kernels' arguments depend only on those and on
+    // the workspace and operator pointers, so a chunk replays in later cycles
+    // and later solves of the same shape.  A chunk is captured on its third
+    // launch (a one-off chunk costs more to capture than to launch).  Valid
+    // for the operator recorded with them.
+    std::map<std::array<int, 3>, hipGraphExec_t> graphs;
+    std::map<std::array<int, 3>, int> uses;  // eager launches of each chunk so far
+    const void *graph_op = nullptr;
+    long graph_lda = 0;
+    void clear_graphs() {
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+        uses.clear();
+    }
     ~GmresCache() {
+        clear_graphs();
         if (pin) (void)hipHostFree(pin);
     }
     bool matches(int N, int M, int R, int restart, bool ne_, hipStream_t s) const {
@@ -565,6 +585,15 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
         }
     }
     KrylovWS &ws = gc->ws;
+    // Arnoldi chunks replay as captured graphs when the workspace is kept
+    // between solves (not while kernel timing records events around launches)
+    const void *opp = A32 ? (const void *)A32 : (const void *)A64;
+    if (gc->graph_op != opp || gc->graph_lda != lda) {
+        gc->clear_graphs();
+        gc->graph_op = opp;
+        gc->graph_lda = lda;
+    }
+    const bool use_graphs = cache && !timing_on() && !getenv("BEM_NO_GRAPH");
     Operator op;
     op.A32 = A32;
     op.A64 = A64;
@@ -696,6 +725,7 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
             } else if (j == 0 && cycles == 1) {
                 c = std::min(c, 4);
             }
+            auto launch_chunk = [&](int j, int c) -> int {
             for (int t = 0; t < c; t++) {
                 const int jj = j + t;
                 HOST_CALL(op.apply(ws, jj, s));
@@ -735,6 +765,31 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
                                                ws.vstride, op.A32 ? ws.v32 : nullptr, ldv, N);
                 HIP_CALL(hipGetLastError());
                 }
+            }
+            return EXIT_SUCCESS;
+            };
+            if (use_graphs) {
+                const std::array<int, 3> key = {j, c, cgs2 ? 1 : 0};
+                auto it = gc->graphs.find(key);
+                hipGraphExec_t exec = it == gc->graphs.end() ? nullptr : it->second;
+                // a chunk is captured once it has recurred: capturing and
+                // instantiating costs more than launching it a few times
+                if (!exec && gc->graphs.size() < kMaxGraphs && gc->uses[key]++ >= kGraphAfter) {
+                    hipGraph_t graph = nullptr;
+                    HIP_CALL(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    const int rc = launch_chunk(j, c);
+                    hipError_t e = hipStreamEndCapture(s, &graph);
+                    HOST_CALL(rc);
+                    HIP_CALL(e);
+                    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    HIP_CALL(e);
+                    gc->graphs[key] = exec;
+                }
+                if (exec) HIP_CALL(hipGraphLaunch(exec, s));
+                else HOST_CALL(launch_chunk(j, c));
+            } else {
+                HOST_CALL(launch_chunk(j, c));
             }
             HIP_CALL(hipMemcpyAsync(hch, ws.hch, sizeof(double2) * hcs * R, hipMemcpyDeviceToHost, s));
             HIP_CALL(hipStreamSynchronize(s));

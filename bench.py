@@ -526,6 +526,7 @@ def bench_precision(bem, C, steps, warmup=2, fp64=True):
             st64 = solver64.solve(bem.SOLVE_GMRES, restart=30, tol=1e-12, maxit=5000, refine_tol=0, max_refine=0)
             bem.sync()
             t64.append((time.perf_counter() - ts) * 1e3)
+        x64 = solver64.solution()
         # the yardstick's assembly kernel against the FP64 peak (same nominal count)
         bem.timing_reset()
         bem.timing_enable(True)
@@ -534,7 +535,6 @@ def bench_precision(bem, C, steps, warmup=2, fp64=True):
         bem.timing_enable(False)
         a64_ms, a64_n = bem.timing_get("asm_tile_f64")
         bem.timing_reset()
-        x64 = solver64.solution()
         solver64.close()
         a64 = a64_ms / max(a64_n, 1)
         nominal64 = FLOP_PER_EVAL * evals / (a64 * 1e-3) / 1e12

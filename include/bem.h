@@ -350,6 +350,20 @@ typedef struct bem_tune_result {
     double default_ms;   /* assembly time with the default shape */
 } bem_tune_result;
 int bem_autotune_assembly(const bem_mesh *m, float k, int precision, bem_tune_result *out);
+/* The multi-right-hand-side MFMA products the Krylov and CG solvers run
+ * (Y = A X and Z = A^H T, FP32, refined): for an M x N matrix and nrhs
+ * right-hand sides, times every candidate shape -- the kernel family (16 or
+ * 32 right-hand sides per wave) and the split-K chunk count (1-16) -- on the
+ * device with a synthetic matrix, and keeps the fastest per product for the
+ * size class (M and N in half-octave buckets, nrhs in steps of 16) in this
+ * process; saved and loaded with the assembly shapes by bem_tune_save /
+ * bem_tune_load. */
+typedef struct bem_product_tune_result {
+    int rhs_per_wave_n, chunks_n;  /* Y = A X */
+    int rhs_per_wave_h, chunks_h;  /* Z = A^H T */
+    double best_ms_n, default_ms_n, best_ms_h, default_ms_h;
+} bem_product_tune_result;
+int bem_autotune_products(int M, int N, int nrhs, bem_product_tune_result *out);
 int bem_tune_save(const char *file);
 int bem_tune_load(const char *file);
 int bem_tune_clear(void);

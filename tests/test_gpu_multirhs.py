@@ -75,6 +75,31 @@ def test_multi_rhs_gmres_matches_oracle(nchief, R):
     assert err.max() < 1e-5
 
 
+def test_autotuned_products_keep_parity():
+    """The product tile tuner (kernel family and split-K chunks per size
+    class) on the s3 CHIEF system with 20 right-hand sides: the solve with the
+    tuned shapes in force still matches the oracle at the 1e-5 bar."""
+    m = _mesh(3, 8)
+    M, N = m.num_pnts + m.num_chief, m.num_pnts
+    k = 31.3
+    d = _dirs(20)
+    try:
+        r = bem.autotune_products(M, N, 20)
+        print("case: product tuner, M %d N %d R 20: %s" % (M, N, r))
+        assert r["rhs_per_wave_n"] in (16, 32) and 1 <= r["chunks_n"] <= 16
+        assert r["rhs_per_wave_h"] in (16, 32) and 1 <= r["chunks_h"] <= 16
+        assert 0 < r["best_ms_n"] and 0 < r["best_ms_h"]
+        s = bem.Solver(m)
+        s.assemble(k, bem.PREC_FP32)
+        s.set_rhs(bem.RHS_PLANE, d, 1.0)
+        st = s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+        err = _relcols(s.solution(), _oracle_lstsq(m, k, d))
+        print("case: tuned solve stats %s worst err %.3g" % (st, err.max()))
+        assert st["converged"] == 1 and err.max() < 1e-5
+    finally:
+        bem.tune_clear()
+
+
 @pytest.mark.parametrize("nchief", [0, 8])
 def test_multi_rhs_equals_single_rhs(nchief):
     m = _mesh(3, nchief)
@@ -164,6 +189,31 @@ def test_cgls_matches_oracle(R, prec):
     print("R %d prec %d stats %s worst err %.3g" % (R, prec, st, err.max()))
     assert st["converged"] == 1
     assert err.max() < (1e-5 if prec == bem.PREC_FP32 else 1e-8)
+
+
+def test_gmres_graph_replays_equal_eager(monkeypatch):
+    """GMRES Arnoldi chunks captured as HIP graphs (one per first step, step
+    count and Gram-Schmidt mode, on a chunk's third launch) and replayed in
+    later cycles and later solves of the same solver, with the host's Givens rotations, the FP64
+    residuals and the solution updates between replays: three solves in a row
+    at different wavenumbers return the same bits as eager launches, and the
+    oracle's solution."""
+    m = _mesh(3, 0)
+    s = bem.Solver(m)
+    d = _dirs(16, seed=5)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    for k in (20.0, 31.3, 42.0):
+        s.assemble(k, bem.PREC_FP32)
+        monkeypatch.delenv("BEM_NO_GRAPH", raising=False)
+        st = s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+        xg = s.solution()
+        monkeypatch.setenv("BEM_NO_GRAPH", "1")
+        s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
+        xe = s.solution()
+        print("case: GMRES graphs, k %.1f, stats %s" % (k, st))
+        assert st["converged"] == 1
+        assert np.array_equal(xg, xe), k
+        assert _relcols(xg, _oracle_lstsq(m, k, d)).max() < 1e-5
 
 
 def test_cgls_graph_replays_equal_eager(monkeypatch):

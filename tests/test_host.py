@@ -290,14 +290,18 @@ def test_request_footprint_bounds_the_allocations():
 
 def test_tune_table_roundtrip(tmp_path):
     src = tmp_path / "tune.txt"
-    src.write_text("26 29 1 16384\n22 25 4 8192\nbad line\n")
+    src.write_text("26 29 1 16384\n22 25 4 8192\nbad line\np 0 31 31 2 1 3\np 1 23 23 1 0 12\n"
+                   "p 0 31 31 2 1 99\n")  # last: chunk count past the limit, ignored
     bem.tune_clear()
     try:
         bem.tune_load(src)
         out = tmp_path / "out.txt"
         bem.tune_save(out)
-        rows = sorted(tuple(map(int, l.split())) for l in out.read_text().splitlines())
+        lines = out.read_text().splitlines()
+        rows = sorted(tuple(map(int, l.split())) for l in lines if not l.startswith("p"))
         assert rows == [(22, 25, 4, 8192), (26, 29, 1, 16384)]
+        prods = sorted(tuple(map(int, l.split()[1:])) for l in lines if l.startswith("p"))
+        assert prods == [(0, 31, 31, 2, 1, 3), (1, 23, 23, 1, 0, 12)]
         with pytest.raises(bem.BemError):
             bem.tune_load(tmp_path / "missing.txt")
     finally:
