@@ -290,6 +290,20 @@ int gemm_n_f32z(const float2 *A, long lda, long M, long N, const float2 *X, long
                 void *ws, hipStream_t s);
 int gemm_h_f32z(const float2 *A, long lda, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R,
                 void *ws, hipStream_t s);
+// Adjoint products through a conjugate-transposed copy AH = A^H (N x M,
+// column-major, ldah = adjoint_copy_ld(N)): Z = A^H T then runs on the Y = A X
+// kernel, whose wave reads whole column segments of its operand.
+bool adjoint_copy_pays(long M, long N, int R);
+long adjoint_copy_ld(long N);
+size_t adjoint_copy_bytes(long M, long N);
+// allocates *AH and fills it when the copy pays and the HBM is free (else *AH
+// stays null); adjoint_copy_fill refreshes an allocated copy
+int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s);
+int adjoint_copy_fill(const float2 *A, long lda, long M, long N, float2 *AH, hipStream_t s);
+int gemm_hc_f32(const float2 *AH, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R, void *ws,
+                hipStream_t s);
+int gemm_hc_f32z(const float2 *AH, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R, void *ws,
+                 hipStream_t s);
 size_t ws_mres(long M, long N, int R);
 // C -= V W (QR trailing update, V with 32 columns)
 int rank_update_f32(const float2 *V, long ldv, int M, const float2 *W, int ldw, int nc, float2 *C, long ldc,

@@ -14,7 +14,8 @@
 // A^H q only ever sees the correction.
 //
 // The products are the MFMA multi-RHS kernels for an FP32 A (the GEMV for one
-// right-hand side); the recurrence vectors and scalars are FP64 and stay on
+// right-hand side), the adjoint one through a conjugate-transposed copy of A
+// made at the start of the solve when the HBM allows; the recurrence vectors and scalars are FP64 and stay on
 // the device (alpha and beta are formed by the update kernels from device
 // scalars), so an iteration needs no host round trip; the host reads the
 // per-system activity flags every 8 steps, and those 8 steps are one captured
@@ -147,12 +148,13 @@ struct CglsWS {
     hipStream_t st = nullptr;
     double2 *rr = nullptr, *s64 = nullptr, *d = nullptr, *p = nullptr, *q64 = nullptr, *w64 = nullptr;
     float2 *p32 = nullptr, *q32 = nullptr, *w32 = nullptr;
+    float2 *AH = nullptr;  // conjugate-transposed copy of an FP32 A (null: adjoint products read A)
     double *gam = nullptr, *del = nullptr, *g0 = nullptr, *thr = nullptr;
     int *act = nullptr, *done = nullptr;
     char *gws = nullptr;
     ~CglsWS() {
         sfree(rr, st); sfree(s64, st); sfree(d, st); sfree(p, st); sfree(q64, st); sfree(w64, st);
-        sfree(p32, st); sfree(q32, st); sfree(w32, st);
+        sfree(p32, st); sfree(q32, st); sfree(w32, st); sfree(AH, st);
         sfree(gam, st); sfree(del, st); sfree(g0, st); sfree(thr, st); sfree(act, st); sfree(done, st); sfree(gws, st);
     }
 };
@@ -189,6 +191,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     HOST_CALL(salloc(&w.act, R, s));
     HOST_CALL(salloc(&w.done, R, s));
     HOST_CALL(salloc(&w.gws, std::max(std::max(ws_gemv_n(M, N, 1), ws_gemm(M, N, R)), ws_mres(M, N, R)), s));
+    if (f32 && R > 1) HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &w.AH, s));
     const int bs = 256;
     const dim3 gN((N + bs - 1) / bs, R);
 
@@ -238,6 +241,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     auto prod_h = [&]() -> int {
         if (f32) {
             if (R == 1) HOST_CALL(gemv_h_f32(A32, lda, M, N, w.q32, 0, w.w32, 0, 1, s));
+            else if (w.AH) HOST_CALL(gemm_hc_f32(w.AH, M, N, w.q32, ld, w.w32, ld, R, w.gws, s));
             else HOST_CALL(gemm_h_f32(A32, lda, M, N, w.q32, ld, w.w32, ld, R, w.gws, s, true));
         } else {
             for (int q = 0; q < R; q++)

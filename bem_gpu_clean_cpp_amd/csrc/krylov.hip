@@ -329,6 +329,9 @@ struct KrylovWS {
     float2 *Ar = nullptr;
     long ldr = 0;
     char *nws = nullptr;
+    // conjugate-transposed copy of an FP32 A for the adjoint products of the
+    // normal equations (several right-hand sides); refreshed every solve
+    float2 *AH = nullptr;
     double2 *hch = nullptr;  // Hessenberg columns of an Arnoldi chunk
     double *npart = nullptr;  // per system: |w|^2 partials of the 256-row workgroups
     long nstride = 0;
@@ -336,7 +339,7 @@ struct KrylovWS {
     ~KrylovWS() {
         sfree(V, st); sfree(w64, st); sfree(t64, st); sfree(r, st); sfree(h, st); sfree(part, st);
         sfree(v32, st); sfree(w32, st); sfree(t32, st); sfree(gws, st); sfree(Ar, st); sfree(nws, st);
-        sfree(hch, st); sfree(npart, st);
+        sfree(AH, st); sfree(hch, st); sfree(npart, st);
     }
     int init(int n_, int m_, int R_, int restart_, hipStream_t s) {
         st = s;
@@ -400,6 +403,7 @@ static int bnorm2(KrylovWS &ws, const double2 *w, long wstride, int n, double2 *
 // The operator of the Krylov iteration: square A, or A^H A for M > N.
 struct Operator {
     const float2 *A32 = nullptr;
+    const float2 *AH = nullptr;  // conjugate-transposed copy of A32, or null
     const double2 *A64 = nullptr;
     long lda = 0;
     int M = 0, N = 0;
@@ -429,7 +433,8 @@ struct Operator {
                     HOST_CALL(gemm_n_f32z(A32, lda, M, N, ws.v32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
                 } else {
                     HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.t32, ws.ldv, R, ws.gws, s, true));
-                    HOST_CALL(gemm_h_f32z(A32, lda, M, N, ws.t32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
+                    if (AH) HOST_CALL(gemm_hc_f32z(AH, M, N, ws.t32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
+                    else HOST_CALL(gemm_h_f32z(A32, lda, M, N, ws.t32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
                 }
             }
         } else {
@@ -586,20 +591,28 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     }
     KrylovWS &ws = gc->ws;
     // Arnoldi chunks replay as captured graphs when the workspace is kept
-    // between solves (not while kernel timing records events around launches)
+    // between solves and BEM_GMRES_GRAPH is set (not while kernel timing
+    // records events around launches)
     const void *opp = A32 ? (const void *)A32 : (const void *)A64;
     if (gc->graph_op != opp || gc->graph_lda != lda) {
         gc->clear_graphs();
         gc->graph_op = opp;
         gc->graph_lda = lda;
     }
-    const bool use_graphs = cache && !timing_on() && !getenv("BEM_NO_GRAPH");
+    // opt-in: with several solvers sharing the GPU (the batch driver's
+    // workers) graph launches measured slower than the eager chunks
+    const bool use_graphs = cache && !timing_on() && getenv("BEM_GMRES_GRAPH") && !getenv("BEM_NO_GRAPH");
     Operator op;
     op.A32 = A32;
     op.A64 = A64;
     op.lda = lda;
     op.M = M;
     op.N = N;
+    if (A32 && M > N && R > 1) {
+        if (ws.AH) HOST_CALL(adjoint_copy_fill(A32, lda, M, N, ws.AH, s));
+        else HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &ws.AH, s));
+        op.AH = ws.AH;
+    }
     const int bs = 256;
     const long ldv = ws.ldv;
     // ||rhs|| per system: the residual at x = 0.  With the row-major copy, the

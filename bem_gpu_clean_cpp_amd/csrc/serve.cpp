@@ -87,6 +87,8 @@ double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int m
         b += sizeof(double2) * (K + 2) * (2 + ((size_t)std::max(M, N) + 1023) / 1024) * R;
         if (method == BEM_SOLVE_GMRES && precision != BEM_PREC_FP64 && nrhs == 1 && M > N && ne_fused_fits(M, N))
             b += sizeof(float2) * (size_t)M * ne_ldr(N) + std::max(ws_ne_resid(M, N), ws_ne_rhs(M, N));  // row-major copy of A
+        if (precision != BEM_PREC_FP64 && nrhs > 1 && M > N && adjoint_copy_pays(M, N, nrhs))
+            b += adjoint_copy_bytes(M, N);  // conjugate-transposed copy of A
     }
     b += sizeof(float2) * R * ld;  // right-hand-side partials
     return (double)b * 1.05 + (1 << 20);

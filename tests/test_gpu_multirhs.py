@@ -193,7 +193,8 @@ def test_cgls_matches_oracle(R, prec):
 
 def test_gmres_graph_replays_equal_eager(monkeypatch):
     """GMRES Arnoldi chunks captured as HIP graphs (one per first step, step
-    count and Gram-Schmidt mode, on a chunk's third launch) and replayed in
+    count and Gram-Schmidt mode, on a chunk's third launch; opt-in through
+    BEM_GMRES_GRAPH) and replayed in
     later cycles and later solves of the same solver, with the host's Givens rotations, the FP64
     residuals and the solution updates between replays: three solves in a row
     at different wavenumbers return the same bits as eager launches, and the
@@ -202,12 +203,13 @@ def test_gmres_graph_replays_equal_eager(monkeypatch):
     s = bem.Solver(m)
     d = _dirs(16, seed=5)
     s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    monkeypatch.delenv("BEM_NO_GRAPH", raising=False)
     for k in (20.0, 31.3, 42.0):
         s.assemble(k, bem.PREC_FP32)
-        monkeypatch.delenv("BEM_NO_GRAPH", raising=False)
+        monkeypatch.setenv("BEM_GMRES_GRAPH", "1")
         st = s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
         xg = s.solution()
-        monkeypatch.setenv("BEM_NO_GRAPH", "1")
+        monkeypatch.delenv("BEM_GMRES_GRAPH")
         s.solve(bem.SOLVE_GMRES, restart=30, tol=1e-6, maxit=3000, refine_tol=1e-10, max_refine=50)
         xe = s.solution()
         print("case: GMRES graphs, k %.1f, stats %s" % (k, st))
@@ -238,3 +240,34 @@ def test_cgls_graph_replays_equal_eager(monkeypatch):
         assert np.array_equal(xg, xe), seed
         xo = _oracle_lstsq(m, k, d)
         assert _relcols(xg, xo).max() < 1e-5
+
+
+@pytest.mark.parametrize("method", [bem.SOLVE_CGLS, bem.SOLVE_GMRES])
+def test_adjoint_copy_matches_adjoint_kernel(monkeypatch, method):
+    """Adjoint products through the conjugate-transposed copy of A (taken for
+    more than 16 right-hand sides) against the adjoint kernel on A itself
+    (BEM_NO_ADJOINT_COPY), on an s3 sphere with 7 CHIEF rows (M = 649: odd, so
+    the copy's last row pair and its last tiles are ragged) and 24 right-hand
+    sides.  After FP64 refinement to 1e-10 the two solutions agree to 1e-8, and
+    the copy's matches the oracle's least-squares solution at 1e-5."""
+    m = _mesh(3, 7)
+    assert m.num_pnts + m.num_chief == 649
+    k = 31.3
+    d = _dirs(24, seed=17)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    kw = dict(restart=30, tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=50)
+    monkeypatch.delenv("BEM_NO_ADJOINT_COPY", raising=False)
+    st = s.solve(method, **kw)
+    xc = s.solution()
+    monkeypatch.setenv("BEM_NO_ADJOINT_COPY", "1")
+    st2 = s.solve(method, **kw)
+    xk = s.solution()
+    rel = _relcols(xc, xk).max()
+    err = _relcols(xc, _oracle_lstsq(m, k, d)).max()
+    print("case: adjoint copy, method %d: stats %s / %s, copy vs kernel %.3g, vs oracle %.3g" %
+          (method, st, st2, rel, err))
+    assert st["converged"] == 1 and st2["converged"] == 1
+    assert rel < 1e-8
+    assert err < 1e-5
