@@ -24,6 +24,7 @@ RHS_PLANE = 1
 SOLVE_QR = 0
 SOLVE_GMRES = 1
 SOLVE_CGLS = 2
+SOLVE_BLOCK_CG = 3
 
 _lib = None
 

@@ -329,11 +329,12 @@ void gmres_cache_free(GmresCache *&c);
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
                  int max_cycles, bem_solve_stats *st, hipStream_t s, GmresCache **cache = nullptr);
-// CGLS (conjugate gradients on A^H A x = A^H b) for R systems in lockstep,
+// CGLS (conjugate gradients on A^H A x = A^H b) for R systems in lockstep
+// (block = false) or as one block CG over all R (block = true, R <= 64),
 // FP64 residual refinement; X is overwritten (the solve starts from zero)
 int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                 double2 *X, long ldx, int R, double tol, int maxit, double outer_tol, int max_cycles,
-                bem_solve_stats *st, hipStream_t s);
+                bem_solve_stats *st, hipStream_t s, bool block = false);
 int qr_lsq_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, hipStream_t s);
 int qr_lsq_f64(double2 *A, long lda, int M, int N, double2 *B, long ldb, int R, hipStream_t s);
 int qr_lsq_with_q_f32(float2 *A, long lda, int M, int N, float2 *B, long ldb, int R, float2 *Q, long ldq,

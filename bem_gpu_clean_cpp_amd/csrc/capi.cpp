@@ -568,7 +568,8 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
     if (!sv) BEM_FAIL("null solver");
     if (!sv->prec) BEM_FAIL("solver not assembled");
     if (sv->rhs.nrhs <= 0) BEM_FAIL("no right-hand sides set");
-    if (method != BEM_SOLVE_QR && method != BEM_SOLVE_GMRES && method != BEM_SOLVE_CGLS)
+    if (method != BEM_SOLVE_QR && method != BEM_SOLVE_GMRES && method != BEM_SOLVE_CGLS &&
+        method != BEM_SOLVE_BLOCK_CG)
         BEM_FAIL("unknown method %d", method);
     std::lock_guard<std::mutex> lk(sv->mu);
     hipStream_t s = sv->stream;
@@ -591,12 +592,12 @@ int bem_solver_solve(bem_solver *sv, int method, int restart, float tol, int max
     HOST_CALL(rhs_fp64(dp, (double)sv->k, sv->B64, ld, sv->rhs, s));
     bem_solve_stats st{};
     st.method = method;
-    if (method == BEM_SOLVE_CGLS) {
+    if (method == BEM_SOLVE_CGLS || method == BEM_SOLVE_BLOCK_CG) {
         const double outer = refine_tol > 0 ? refine_tol : (double)tol;
         const int cycles = max_refine > 0 ? max_refine : (1 << 20);
         if (maxit <= 0) maxit = 100000;
         HOST_CALL(cgls_device(sv->A32, sv->A64, ld, dp.M, dp.N, sv->B64, ld, sv->X64, ld, R, (double)tol, maxit,
-                              outer, cycles, &st, s));
+                              outer, cycles, &st, s, method == BEM_SOLVE_BLOCK_CG));
     } else if (method == BEM_SOLVE_GMRES) {
         HIP_CALL(hipMemsetAsync(sv->X64, 0, sizeof(double2) * ld * R, s));
         const double outer = refine_tol > 0 ? refine_tol : (double)tol;

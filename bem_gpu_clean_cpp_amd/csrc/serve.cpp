@@ -89,6 +89,8 @@ double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int m
             b += sizeof(float2) * (size_t)M * ne_ldr(N) + std::max(ws_ne_resid(M, N), ws_ne_rhs(M, N));  // row-major copy of A
         if (precision != BEM_PREC_FP64 && nrhs > 1 && M > N && adjoint_copy_pays(M, N, nrhs))
             b += adjoint_copy_bytes(M, N);  // conjugate-transposed copy of A
+        if (method == BEM_SOLVE_BLOCK_CG)  // block vectors, Gram partials
+            b += (6 * sizeof(double2) + sizeof(float2)) * ldv * R + sizeof(double2) * 4096 * ((N + 255) / 256 + 5);
     }
     b += sizeof(float2) * R * ld;  // right-hand-side partials
     return (double)b * 1.05 + (1 << 20);

@@ -60,8 +60,9 @@ def parse():
     ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--large-method", choices=["cgls", "gmres"], default="cgls",
-                    help="solver of the headline workload (gmres: BA-GMRES(100) on the normal equations)")
+    ap.add_argument("--large-method", choices=["cgls", "gmres", "block"], default="block",
+                    help="solver of the headline workload (cgls: CG per right-hand side; block: block CG over all "
+                         "32; gmres: BA-GMRES(100) on the normal equations)")
     ap.add_argument("--precision-steps", type=int, default=10)
     ap.add_argument("--plain-subdiv", type=int, default=3, help="icosphere subdivision of the plain workload")
     ap.add_argument("--plain-reps", type=int, default=20)
@@ -71,6 +72,8 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-method", choices=sorted(SERVE_METHODS), default="block",
+                    help="solver of the serving requests (cgls: CG per right-hand side; block: block CG)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (scale for FETCH/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -326,15 +329,15 @@ def bench_plain(bem, C, reps=20, subdiv=3):
     }
 
 
-def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="cgls"):
+def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block"):
     """Headline: seeded perturbed ellipsoid at icosphere subdivision 6 (40962
     nodes, 81920 triangles, 13.4 GB complex64 matrix), rigid, 12 kHz, 32 CHIEF
     points, 32 point sources 1 cm outside the surface.  One step = FP32
-    assembly + the least-squares solve of the 32 right-hand sides (CG on the
-    normal equations with FP32 MFMA products, FP64 refinement to a
-    normal-equation residual of 1e-11).  Timed steps run without per-launch
-    events (the CG iterations replay a captured HIP graph); kernel durations
-    for the rooflines come from one instrumented step after them."""
+    assembly + the least-squares solve of the 32 right-hand sides (block CG on
+    the normal equations over all 32 by default, FP32 MFMA products, FP64
+    refinement to a normal-equation residual of 1e-11).  Timed steps run
+    without per-launch events; kernel durations for the rooflines come from
+    one instrumented step after them."""
     c, rho = 343.0, 1.21
     m, src = C.large_case(bem)
     N, E, Cn = m.num_pnts, m.num_elems, m.num_chief
@@ -343,6 +346,8 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="cgls"):
     k = 2 * np.pi * 12000.0 / c
     if method == "gmres":
         kw = dict(method=bem.SOLVE_GMRES, restart=100, tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=20)
+    elif method == "block":
+        kw = dict(method=bem.SOLVE_BLOCK_CG, tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=10)
     else:
         kw = dict(method=bem.SOLVE_CGLS, tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=10)
     sv = bem.Solver(m)
@@ -392,8 +397,9 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="cgls"):
         "workload": "large: perturbed ellipsoid s6 (N=%d, E=%d, M=%d, A %.1f GB), rigid, 12 kHz, 32 CHIEF, "
                     "32 point sources 1 cm outside; FP32 assembly + %s (FP32 MFMA products, R=32) with FP64 "
                     "refinement to 1e-11" % (N, E, M, 8.0 * M * N / 1e9,
-                                             "BA-GMRES(100) on the normal equations" if method == "gmres"
-                                             else "CG on the normal equations"),
+                                             {"gmres": "BA-GMRES(100) on the normal equations",
+                                              "block": "block CG on the normal equations"}.get(
+                                                 method, "CG on the normal equations")),
         "N": N, "E": E, "M": M, "rhs": R,
         "value": R / (ms_per_step / 1e3), "ms_per_step": ms_per_step,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2)},
@@ -590,15 +596,19 @@ def serving_requests(bem, C, n, seed):
     return reqs
 
 
+SERVE_METHODS = {"cgls": lambda bem: bem.SOLVE_CGLS, "block": lambda bem: bem.SOLVE_BLOCK_CG}
+
+
 def bench_serving(bem, C, n, rate, slots, seed=104, method=None):
     """Poisson request stream into the request server; latency = completion -
-    arrival (queueing included).  The CHIEF systems are solved with CG on the
-    normal equations (FP32 MFMA products, FP64 refinement to 1e-8)."""
+    arrival (queueing included).  The CHIEF systems are solved with block CG
+    (or CG per right-hand side) on the normal equations (FP32 MFMA products,
+    FP64 refinement to 1e-8)."""
     import threading
     reqs = serving_requests(bem, C, n, seed)
     gaps = np.random.default_rng(seed + 1).exponential(1.0 / rate, n)
     arrivals = np.cumsum(gaps) - gaps[0]
-    kw = dict(method=bem.SOLVE_CGLS if method is None else method, restart=30, tol=1e-6, maxit=6000,
+    kw = dict(method=bem.SOLVE_BLOCK_CG if method is None else method, restart=30, tol=1e-6, maxit=6000,
               refine_tol=1e-8, max_refine=50)
     srv = bem.Server(slots=slots)
     # warm-up: one small request
@@ -653,8 +663,9 @@ def bench_serving(bem, C, n, rate, slots, seed=104, method=None):
         sizes[r["s"]] = sizes.get(r["s"], 0) + 1
     return {
         "workload": "serving: %d Poisson requests (rate %.3g/s, seed %d), sphere/perturbed ellipsoid s3-s6, "
-                    "ka U[0.5,20], 1-32 plane-wave or point-source RHS, 16 CHIEF; FP32 assembly + CG on the normal "
-                    "equations with FP64 refinement; %d slots, HBM-footprint admission" % (n, rate, seed, slots),
+                    "ka U[0.5,20], 1-32 plane-wave or point-source RHS, 16 CHIEF; FP32 assembly + %s on the normal "
+                    "equations with FP64 refinement; %d slots, HBM-footprint admission" %
+                    (n, rate, seed, "block CG" if kw["method"] == bem.SOLVE_BLOCK_CG else "CG", slots),
         "requests": n, "rhs_columns": cols, "wall_s": round(wall, 3),
         "solves_per_s": round(cols / wall, 2), "requests_per_s": round(n / wall, 3),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
@@ -699,7 +710,8 @@ def main():
             res = bench_large(bem, C, args.steps, args.warmup, fp64=fp64, cpu=not args.no_cpu_baseline,
                               method=args.large_method)
         else:
-            res = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
+            res = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots,
+                                method=SERVE_METHODS[args.serve_method](bem))
         print(json.dumps({args.only: res}), flush=True)
         return
 
@@ -717,7 +729,8 @@ def main():
         sections["batching"] = bench_batching(bem, C, args.workers, fp64=fp64)
         log("batching", sections["batching"])
     if not args.no_serving:
-        sections["serving"] = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots)
+        sections["serving"] = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots,
+                                            method=SERVE_METHODS[args.serve_method](bem))
         log("serving", sections["serving"])
 
     head_keys = ("value", "ms_per_step", "latency_ms", "roofline", "cpu_baseline", "workload", "N", "E", "M", "rhs")

@@ -205,10 +205,10 @@ typedef struct bem_solver bem_solver;
 
 enum { BEM_PREC_FP32 = 32, BEM_PREC_FP64 = 64 };
 enum { BEM_RHS_POINT = 0, BEM_RHS_PLANE = 1 };
-enum { BEM_SOLVE_QR = 0, BEM_SOLVE_GMRES = 1, BEM_SOLVE_CGLS = 2 };
+enum { BEM_SOLVE_QR = 0, BEM_SOLVE_GMRES = 1, BEM_SOLVE_CGLS = 2, BEM_SOLVE_BLOCK_CG = 3 };
 
 typedef struct bem_solve_stats {
-    int method;           /* BEM_SOLVE_QR, BEM_SOLVE_GMRES or BEM_SOLVE_CGLS */
+    int method;           /* BEM_SOLVE_QR, BEM_SOLVE_GMRES, BEM_SOLVE_CGLS or BEM_SOLVE_BLOCK_CG */
     int iterations;       /* total inner Krylov iterations (all refinement steps) */
     int refinements;      /* outer FP64 refinement steps */
     int converged;        /* 1 when the requested tolerance was reached */
@@ -237,7 +237,12 @@ int bem_solver_set_rhs(bem_solver *s, int kind, const float *params, int nrhs, f
  * method BEM_SOLVE_QR: blocked Householder least squares (FP32 path also
  * refined in FP64).  method BEM_SOLVE_CGLS: conjugate gradients on the
  * normal equations (restart unused), for the overdetermined CHIEF systems,
- * with the same FP64 refinement. */
+ * with the same FP64 refinement, each right-hand side on its own.
+ * method BEM_SOLVE_BLOCK_CG: the same normal equations solved by block
+ * conjugate gradients over all right-hand sides at once (one shared block
+ * Krylov space, orthonormalised residual block), with the same refinement;
+ * falls back to BEM_SOLVE_CGLS's iteration when the block breaks down (e.g.
+ * two equal right-hand sides).  iterations counts block steps. */
 int bem_solver_solve(bem_solver *s, int method, int restart, float tol, int maxit, double refine_tol,
                      int max_refine, bem_solve_stats *stats);
 /* Copy the nodal solution (N x nrhs, ldx) to the host. */
@@ -276,7 +281,7 @@ int bem_solver_release(bem_solver *s);
 /* Solve settings shared by the batch and server calls (the arguments of
  * bem_solver_solve). */
 typedef struct bem_solve_params {
-    int method;         /* BEM_SOLVE_GMRES, BEM_SOLVE_QR or BEM_SOLVE_CGLS */
+    int method;         /* BEM_SOLVE_GMRES, BEM_SOLVE_QR, BEM_SOLVE_CGLS or BEM_SOLVE_BLOCK_CG */
     int restart;
     float tol;
     int maxit;

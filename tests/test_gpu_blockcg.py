@@ -1,0 +1,136 @@
+"""Block conjugate gradients on the CHIEF normal equations (BEM_SOLVE_BLOCK_CG).
+
+All right-hand sides share one block Krylov space (O'Leary 1980, residual
+block orthonormalised by CholeskyQR2 as in Dubrulle 2001), with FP64 residual
+refinement.  Checks:
+  * each column against the FP64 oracle's least-squares solution (relative
+    2-norm error <= 1e-5 for the FP32 matrix, <= 1e-8 for the FP64 matrix), for
+    2, 20 and 40 plane waves and the maximum block of 64 point sources;
+  * the same solution as the lockstep CG after refinement to 1e-10 (1e-8);
+  * fewer block steps than lockstep CG steps for 20 right-hand sides;
+  * two equal right-hand sides (a rank-deficient block): the block cannot
+    start, the solve falls back to the lockstep CG and still meets the bar.
+"""
+import numpy as np
+import pytest
+
+import bem_gpu_clean_cpp_amd as bem
+import bemcases as C
+import oracle_py as O
+
+pytestmark = pytest.mark.gpu
+
+RHO, SPEED = float(np.float32(1.21)), float(np.float32(343.0))
+KW = dict(tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=20)
+
+
+@pytest.fixture(autouse=True, scope="module")
+def _setup():
+    bem.quad_init(6)
+    O.quad_init(6)
+    bem.set_medium(RHO, SPEED)
+    yield
+    bem.set_medium(1.2041, 343.21)
+
+
+def _mesh(s, nchief, r=0.1, seed=7):
+    m = bem.Mesh().icosphere(s, r)
+    m.set_chief(C.ball_points(seed, nchief, r / 2))
+    return m
+
+
+def _oracle(m, k, kind, params):
+    p, e, bc = C.mesh_arrays(m)
+    pts = np.vstack([p, m.chief()]).astype(np.float64)
+    q = np.asarray(params, np.float64)
+    if kind == bem.RHS_PLANE:
+        binit = np.exp(-1j * k * pts @ q.T)
+    else:
+        r = np.linalg.norm(pts[:, None, :] - q[None, :, :], axis=2)
+        binit = np.exp(-1j * k * r) / (4 * np.pi * r)
+    A, B = O.assemble(k, pts, p.shape[0], e, bc, binit=binit, dens=RHO, speed=SPEED)
+    return np.linalg.lstsq(A, B, rcond=None)[0]
+
+
+def _relcols(x, y):
+    return np.linalg.norm(x - y, axis=0) / np.linalg.norm(y, axis=0)
+
+
+@pytest.mark.parametrize("R", [2, 20, 40])
+def test_block_cg_matches_oracle(R):
+    m = _mesh(3, 8)
+    k = 31.3
+    d = C.sphere_points(11, R, 1.0)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    st = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+    err = _relcols(s.solution(), _oracle(m, k, bem.RHS_PLANE, d))
+    print("case: block CG, R %d: stats %s, worst err %.3g" % (R, st, err.max()), flush=True)
+    assert st["converged"] == 1 and st["method"] == bem.SOLVE_BLOCK_CG
+    assert err.max() < 1e-5
+
+
+def test_block_cg_fp64_matrix():
+    m = _mesh(2, 6)
+    k = 20.0
+    d = C.sphere_points(2, 9, 1.0)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP64)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    st = s.solve(bem.SOLVE_BLOCK_CG, tol=1e-12, maxit=5000, refine_tol=1e-12, max_refine=20)
+    err = _relcols(s.solution(), _oracle(m, k, bem.RHS_PLANE, d))
+    print("case: block CG FP64, stats %s, worst err %.3g" % (st, err.max()), flush=True)
+    assert st["converged"] == 1
+    assert err.max() < 1e-8
+
+
+def test_block_cg_max_block_point_sources():
+    """64 point sources inside the body (the largest block)."""
+    m = _mesh(2, 4)
+    k = 12.0
+    src = C.ball_points(3, 64, 0.04)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_POINT, src, 1.0)
+    st = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+    err = _relcols(s.solution(), _oracle(m, k, bem.RHS_POINT, src))
+    print("case: block CG, 64 point sources: stats %s, worst err %.3g" % (st, err.max()), flush=True)
+    assert st["converged"] == 1
+    assert err.max() < 1e-5
+
+
+def test_block_cg_equals_lockstep_cg():
+    m = _mesh(3, 8)
+    k = 31.3
+    d = C.sphere_points(5, 20, 1.0)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    sb = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+    xb = s.solution()
+    sl = s.solve(bem.SOLVE_CGLS, **KW)
+    xl = s.solution()
+    rel = _relcols(xb, xl).max()
+    print("case: block vs lockstep CG: %s / %s, difference %.3g" % (sb, sl, rel), flush=True)
+    assert sb["converged"] == 1 and sl["converged"] == 1
+    assert rel < 1e-8
+    assert sb["iterations"] < sl["iterations"]
+
+
+def test_block_cg_equal_right_hand_sides_fall_back():
+    """Columns 0 and 3 are the same plane wave: the block is rank deficient."""
+    m = _mesh(3, 8)
+    k = 25.0
+    d = C.sphere_points(9, 6, 1.0)
+    d[3] = d[0]
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    st = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+    x = s.solution()
+    err = _relcols(x, _oracle(m, k, bem.RHS_PLANE, d))
+    print("case: block CG with equal columns: stats %s, worst err %.3g" % (st, err.max()), flush=True)
+    assert st["converged"] == 1
+    assert err.max() < 1e-5
+    assert np.array_equal(x[:, 0], x[:, 3])

@@ -12,7 +12,7 @@ right-hand side.
 
 Serving (requests on seeded perturbed spheres and ellipsoids at subdivision
 3-6, ka up to 20, 1-32 plane-wave or point-source right-hand sides, 16 CHIEF
-rows each, CG on the normal equations with FP64 refinement): a mix that holds
+rows each, block CG on the normal equations with FP64 refinement): a mix that holds
 an s5 request (sphere, 32 plane waves, ka = 20) and an s6 request (ellipsoid,
 32 point sources 1 cm outside, ka = 19) besides small ones; every result is
 bit-identical alone (one slot) and in the mix (four slots); the s5 and s6
@@ -128,7 +128,7 @@ def _serving_requests():
 
 def test_serving_workload_sizes():
     reqs = _serving_requests()
-    kw = dict(method=bem.SOLVE_CGLS, restart=30, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
+    kw = dict(method=bem.SOLVE_BLOCK_CG, restart=30, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
     alone = []
     srv = bem.Server(slots=1)
     for q, (m, k, kind, params) in enumerate(reqs):

@@ -5,9 +5,12 @@ Checks (an FP64 oracle of the whole system would take hours on the CPU):
   * 2048 rows (8 seeded blocks of 256) of the FP32 and FP64 device matrices
     against the FP64 oracle: max |dA| <= 2e-5 max |A| (FP32) and
     <= 1e-10 max |A| (FP64);
-  * the FP32 solve (conjugate gradients on the normal equations, FP32 MFMA
-    products, FP64 refinement) against the all-FP64 device solve of the same
-    system for 2 of the sources: relative 2-norm error <= 1e-5;
+  * the FP32 solve (block conjugate gradients on the normal equations over
+    all 32 sources, FP32 MFMA products, FP64 refinement) against the all-FP64
+    device solve (conjugate gradients per source) of the same system for 2 of
+    the sources: relative 2-norm error <= 1e-5;
+  * the same FP32 solve with conjugate gradients per source: the two agree to
+    1e-7 after refinement;
   * every source's refined FP64 normal-equation residual <= 1e-11, and the
     residual of the FP32 solution on the sampled rows evaluated with the
     oracle's rows equals the one with the device rows to 1e-6.
@@ -45,9 +48,16 @@ def test_large_system_rows_and_solution():
     s32.assemble(k, bem.PREC_FP32)
     s32.set_rhs(bem.RHS_POINT, src, 1.0)
     st = s32.solve(bem.SOLVE_CGLS, tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=10)
-    print("fp32 stats", st, flush=True)
+    print("fp32 lockstep CG stats", st, flush=True)
+    assert st["converged"] == 1 and st["residual"] <= 1e-11
+    xl = s32.solution()
+    st = s32.solve(bem.SOLVE_BLOCK_CG, tol=1e-6, maxit=6000, refine_tol=1e-11, max_refine=10)
+    print("fp32 block CG stats", st, flush=True)
     assert st["converged"] == 1 and st["residual"] <= 1e-11
     x32 = s32.solution()
+    dbl = (np.linalg.norm(x32 - xl, axis=0) / np.linalg.norm(xl, axis=0)).max()
+    print("block vs lockstep CG: %.3g" % dbl, flush=True)
+    assert dbl <= 1e-7
     starts = np.sort(np.random.default_rng(1053).choice(M // 256, 8, replace=False)) * 256
     rows = np.concatenate([np.arange(r, min(r + 256, M)) for r in starts])
     a32 = s32.rows(rows)
