@@ -161,22 +161,29 @@ __global__ void kc_act_init(int R, const double *__restrict__ g0, const int *__r
 
 // ------------------------------------------------------------ block CG ---
 constexpr int kMaxBlock = 64;    // right-hand sides of one block (Rb <= 64)
-constexpr int kGramRows = 256;   // rows of one Gram partial
+constexpr int kGramRows = 64;    // rows of one Gram partial
 constexpr int kGramSub = 32;     // rows staged in LDS at a time
+constexpr int kGramSegs = 16;    // first-level groups of partials in the sum
 constexpr int kMulCols = 8;      // output columns per thread of the block product
 
 // part[blk][a + Rb b] = sum over the block's rows i of conj(U[i, a]) V[i, b]
 // (U, V: n x Rb, column-major, ld); FP64, each entry summed in row order.
-// Thread (a = t % 64, g = t / 64) owns the entries (a, g + 4 u), u < 16.
+// Thread t owns the entries e = t + 256 u (a = e % Rb, b = e / Rb).
 __global__ __launch_bounds__(256) void kc_gram_part(const double2 *__restrict__ U, const double2 *__restrict__ V,
                                                     long ld, int n, int Rb, double2 *__restrict__ part) {
     __shared__ double2 Us[kGramSub * kMaxBlock], Vs[kGramSub * kMaxBlock];  // [row][column]
+    const int RR = Rb * Rb;
     const int i0 = blockIdx.x * kGramRows;
-    const int a = threadIdx.x & 63, g = threadIdx.x >> 6;
-    constexpr int kPer = kMaxBlock / 4;
+    constexpr int kPer = kMaxBlock * kMaxBlock / 256;
     double2 acc[kPer];
+    int ia[kPer], ib[kPer];
 #pragma unroll
-    for (int u = 0; u < kPer; u++) acc[u] = make_double2(0, 0);
+    for (int u = 0; u < kPer; u++) {
+        acc[u] = make_double2(0, 0);
+        const int e = min((int)threadIdx.x + 256 * u, RR - 1);
+        ia[u] = e % Rb;
+        ib[u] = e / Rb;
+    }
     for (int sub = 0; sub < kGramRows; sub += kGramSub) {
         for (int e = threadIdx.x; e < kGramSub * Rb; e += 256) {
             const int r = e % kGramSub, c = e / kGramSub;
@@ -190,38 +197,47 @@ __global__ __launch_bounds__(256) void kc_gram_part(const double2 *__restrict__ 
             Vs[r * Rb + c] = v;
         }
         __syncthreads();
-        if (a < Rb) {
-#pragma unroll 2
-            for (int r = 0; r < kGramSub; r++) {
-                const double2 x = Us[r * Rb + a];
+        for (int r = 0; r < kGramSub; r++) {
 #pragma unroll
-                for (int u = 0; u < kPer; u++) {
-                    const int b = g + 4 * u;
-                    if (b < Rb) {
-                        const double2 y = Vs[r * Rb + b];
-                        acc[u].x = fma(x.x, y.x, fma(x.y, y.y, acc[u].x));
-                        acc[u].y = fma(x.x, y.y, fma(-x.y, y.x, acc[u].y));
-                    }
+            for (int u = 0; u < kPer; u++) {
+                if ((int)threadIdx.x + 256 * u < RR) {
+                    const double2 x = Us[r * Rb + ia[u]], y = Vs[r * Rb + ib[u]];
+                    acc[u].x = fma(x.x, y.x, fma(x.y, y.y, acc[u].x));
+                    acc[u].y = fma(x.x, y.y, fma(-x.y, y.x, acc[u].y));
                 }
             }
         }
         __syncthreads();
     }
-    if (a >= Rb) return;
 #pragma unroll
     for (int u = 0; u < kPer; u++) {
-        const int b = g + 4 * u;
-        if (b < Rb) part[(size_t)blockIdx.x * Rb * Rb + a + (size_t)b * Rb] = acc[u];
+        const int e = threadIdx.x + 256 * u;
+        if (e < RR) part[(size_t)blockIdx.x * RR + e] = acc[u];
     }
 }
 
-// G[e] = sum_blk part[blk][e], blocks in order
-__global__ void kc_gram_sum(const double2 *__restrict__ part, int nblk, int RR, double2 *__restrict__ G) {
+// seg[g][e] = sum of part[blk][e] over the g-th run of blocks, in order
+__global__ void kc_gram_sum1(const double2 *__restrict__ part, int nblk, int RR, double2 *__restrict__ seg) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+    if (e >= RR) return;
+    const int per = (nblk + kGramSegs - 1) / kGramSegs;
+    const int b1 = min(nblk, (g + 1) * per);
+    double2 sacc = make_double2(0, 0);
+    for (int b = g * per; b < b1; b++) {
+        const double2 v = part[(size_t)b * RR + e];
+        sacc.x += v.x;
+        sacc.y += v.y;
+    }
+    seg[(size_t)g * RR + e] = sacc;
+}
+// G[e] = sum_g seg[g][e], in order
+__global__ void kc_gram_sum2(const double2 *__restrict__ seg, int RR, double2 *__restrict__ G) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= RR) return;
     double2 sacc = make_double2(0, 0);
-    for (int b = 0; b < nblk; b++) {
-        const double2 v = part[(size_t)b * RR + e];
+#pragma unroll
+    for (int g = 0; g < kGramSegs; g++) {
+        const double2 v = seg[(size_t)g * RR + e];
         sacc.x += v.x;
         sacc.y += v.y;
     }
@@ -354,7 +370,7 @@ struct CglsWS {
     // staging slots and their device copies); active column indices
     double2 *bQ = nullptr, *bS = nullptr, *bP = nullptr, *bP2 = nullptr, *bW = nullptr, *bX = nullptr;
     float2 *bP32 = nullptr;
-    double2 *gpart = nullptr, *gsum = nullptr, *dcm = nullptr;
+    double2 *gpart = nullptr, *gseg = nullptr, *gsum = nullptr, *dcm = nullptr;
     double2 *hgram = nullptr, *hcm = nullptr;
     int *bidx = nullptr;
     ~CglsWS() {
@@ -362,7 +378,7 @@ struct CglsWS {
         sfree(p32, st); sfree(q32, st); sfree(w32, st); sfree(AH, st);
         sfree(gam, st); sfree(del, st); sfree(g0, st); sfree(thr, st); sfree(act, st); sfree(done, st); sfree(gws, st);
         sfree(bQ, st); sfree(bS, st); sfree(bP, st); sfree(bP2, st); sfree(bW, st); sfree(bX, st); sfree(bP32, st);
-        sfree(gpart, st); sfree(gsum, st); sfree(dcm, st); sfree(bidx, st);
+        sfree(gpart, st); sfree(gseg, st); sfree(gsum, st); sfree(dcm, st); sfree(bidx, st);
         if (hgram || hcm) (void)hipStreamSynchronize(st);  // the pinned slots may feed queued copies
         if (hgram) (void)hipHostFree(hgram);
         if (hcm) (void)hipHostFree(hcm);
@@ -410,6 +426,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         for (double2 **b : {&w.bQ, &w.bS, &w.bP, &w.bP2, &w.bW, &w.bX}) HOST_CALL(salloc(b, nv, s));
         if (f32) HOST_CALL(salloc(&w.bP32, nv, s));
         HOST_CALL(salloc(&w.gpart, (size_t)gblk * kMaxBlock * kMaxBlock, s));
+        HOST_CALL(salloc(&w.gseg, (size_t)kGramSegs * kMaxBlock * kMaxBlock, s));
         HOST_CALL(salloc(&w.gsum, (size_t)kMaxBlock * kMaxBlock, s));
         HOST_CALL(salloc(&w.dcm, (size_t)kCSlots * kMaxBlock * kMaxBlock, s));
         HOST_CALL(salloc(&w.bidx, R, s));
@@ -510,7 +527,8 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     // G (Rb x Rb, host) = U^H V over the N rows
     auto gram = [&](const double2 *U, const double2 *V, int Rb, Small &G) -> int {
         kc_gram_part<<<gblk, 256, 0, s>>>(U, V, ld, N, Rb, w.gpart);
-        kc_gram_sum<<<(Rb * Rb + 255) / 256, 256, 0, s>>>(w.gpart, gblk, Rb * Rb, w.gsum);
+        kc_gram_sum1<<<dim3((Rb * Rb + 63) / 64, kGramSegs), 64, 0, s>>>(w.gpart, gblk, Rb * Rb, w.gseg);
+        kc_gram_sum2<<<(Rb * Rb + 255) / 256, 256, 0, s>>>(w.gseg, Rb * Rb, w.gsum);
         HIP_CALL(hipGetLastError());
         HIP_CALL(hipMemcpyAsync(w.hgram, w.gsum, sizeof(double2) * Rb * Rb, hipMemcpyDeviceToHost, s));
         HIP_CALL(hipStreamSynchronize(s));

@@ -90,7 +90,7 @@ double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int m
         if (precision != BEM_PREC_FP64 && nrhs > 1 && M > N && adjoint_copy_pays(M, N, nrhs))
             b += adjoint_copy_bytes(M, N);  // conjugate-transposed copy of A
         if (method == BEM_SOLVE_BLOCK_CG)  // block vectors, Gram partials
-            b += (6 * sizeof(double2) + sizeof(float2)) * ldv * R + sizeof(double2) * 4096 * ((N + 255) / 256 + 5);
+            b += (6 * sizeof(double2) + sizeof(float2)) * ldv * R + sizeof(double2) * 4096 * ((N + 63) / 64 + 21);
     }
     b += sizeof(float2) * R * ld;  // right-hand-side partials
     return (double)b * 1.05 + (1 << 20);

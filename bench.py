@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--only", choices=["plain", "precision", "batching", "large", "serving"],
                     help="run one workload alone and print its section (not the benchmark line)")
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
+    ap.add_argument("--batch-method", choices=["gmres", "block", "cgls"], default="gmres",
+                    help="solver of the batching workload")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -188,7 +190,7 @@ def cpu_baseline_large(mesh, k, rho, c, R, products, threads):
     }
 
 
-def bench_batching(bem, C, workers, fp64=True):
+def bench_batching(bem, C, workers, fp64=True, method=1):
     """Multi-frequency, multi-right-hand-side batch: seeded perturbed ellipsoid
     (semi-axes 0.075/0.09/0.11 m, 8 spherical-harmonic bumps up to degree 6,
     amplitude <= 5%) at icosphere subdivision 4 (2562 nodes), rigid, 64
@@ -205,7 +207,7 @@ def bench_batching(bem, C, workers, fp64=True):
     freqs = 100.0 * 80.0 ** (np.arange(nf) / (nf - 1.0))
     ks = (2 * np.pi * freqs / c).astype(np.float32)
     dirs = np.stack([C.sphere_points(1030 + f, R, 1.0) for f in range(nf)]).astype(np.float32)
-    kw = dict(restart=100, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
+    kw = dict(method=method, restart=100, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
     bem.batch_solve(mesh, ks[-workers:], bem.RHS_PLANE, dirs[-workers:], workers=workers, **kw)  # warm-up
     bem.sync()
     t0 = time.perf_counter()
@@ -249,8 +251,11 @@ def bench_batching(bem, C, workers, fp64=True):
     fp64_rate = 1.0 / float(np.mean(t64)) if t64 else None
     return {
         "workload": "batching: perturbed ellipsoid s4 (N=%d, E=%d), rigid, 64 freqs 100 Hz-8 kHz x 16 plane waves; "
-                    "FP32 assembly + lockstep GMRES(100,1e-6) with MFMA multi-RHS products, FP64 refinement to 1e-8, "
-                    "%d concurrent workers" % (N, E, workers),
+                    "FP32 assembly + %s with MFMA multi-RHS products, FP64 refinement to 1e-8, "
+                    "%d concurrent workers" % (N, E, {bem.SOLVE_GMRES: "lockstep GMRES(100,1e-6)",
+                                                      bem.SOLVE_BLOCK_CG: "block CG (normal equations, 1e-6)",
+                                                      bem.SOLVE_CGLS: "lockstep CG (normal equations, 1e-6)"}[method],
+                                               workers),
         "value": round(value, 2), "unit": "solves/s", "wall_s": round(wall, 4), "solves": nf * R,
         "iterations": {"min": int(min(its)), "median": int(np.median(its)), "max": int(max(its))},
         "converged": conv,
@@ -596,7 +601,8 @@ def serving_requests(bem, C, n, seed):
     return reqs
 
 
-SERVE_METHODS = {"cgls": lambda bem: bem.SOLVE_CGLS, "block": lambda bem: bem.SOLVE_BLOCK_CG}
+SERVE_METHODS = {"cgls": lambda bem: bem.SOLVE_CGLS, "block": lambda bem: bem.SOLVE_BLOCK_CG,
+                 "gmres": lambda bem: bem.SOLVE_GMRES}
 
 
 def bench_serving(bem, C, n, rate, slots, seed=104, method=None):
@@ -705,7 +711,7 @@ def main():
         elif args.only == "precision":
             res = bench_precision(bem, C, args.precision_steps, fp64=fp64)
         elif args.only == "batching":
-            res = bench_batching(bem, C, args.workers, fp64=fp64)
+            res = bench_batching(bem, C, args.workers, fp64=fp64, method=SERVE_METHODS[args.batch_method](bem))
         elif args.only == "large":
             res = bench_large(bem, C, args.steps, args.warmup, fp64=fp64, cpu=not args.no_cpu_baseline,
                               method=args.large_method)
@@ -726,7 +732,8 @@ def main():
         sections["plain"] = bench_plain(bem, C, args.plain_reps, args.plain_subdiv)
         log("plain", sections["plain"])
     if not args.no_batching:
-        sections["batching"] = bench_batching(bem, C, args.workers, fp64=fp64)
+        sections["batching"] = bench_batching(bem, C, args.workers, fp64=fp64,
+                                              method=SERVE_METHODS[args.batch_method](bem))
         log("batching", sections["batching"])
     if not args.no_serving:
         sections["serving"] = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots,
