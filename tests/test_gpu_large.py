@@ -10,7 +10,9 @@ Checks (an FP64 oracle of the whole system would take hours on the CPU):
     device solve (conjugate gradients per source) of the same system for 2 of
     the sources: relative 2-norm error <= 1e-5;
   * the same FP32 solve with conjugate gradients per source: the two agree to
-    1e-7 after refinement;
+    1e-7 after refinement to 1e-11;
+  * the block solve refined to 1e-10 only (the benchmark's setting): against
+    the FP64 solve at 1e-5 as well, and within 1e-6 of the 1e-11 solution;
   * every source's refined FP64 normal-equation residual <= 1e-11, and the
     residual of the FP32 solution on the sampled rows evaluated with the
     oracle's rows equals the one with the device rows to 1e-6.
@@ -58,6 +60,11 @@ def test_large_system_rows_and_solution():
     dbl = (np.linalg.norm(x32 - xl, axis=0) / np.linalg.norm(xl, axis=0)).max()
     print("block vs lockstep CG: %.3g" % dbl, flush=True)
     assert dbl <= 1e-7
+    st10 = s32.solve(bem.SOLVE_BLOCK_CG, tol=1e-6, maxit=6000, refine_tol=1e-10, max_refine=10)
+    x10 = s32.solution()
+    d10 = (np.linalg.norm(x10 - x32, axis=0) / np.linalg.norm(x32, axis=0)).max()
+    print("block CG to 1e-10: stats %s, vs the 1e-11 solution %.3g" % (st10, d10), flush=True)
+    assert st10["converged"] == 1 and d10 <= 1e-6
     starts = np.sort(np.random.default_rng(1053).choice(M // 256, 8, replace=False)) * 256
     rows = np.concatenate([np.arange(r, min(r + 256, M)) for r in starts])
     a32 = s32.rows(rows)
@@ -71,9 +78,10 @@ def test_large_system_rows_and_solution():
     a64 = s64.rows(rows)
     s64.close()
     err = np.linalg.norm(x32[:, :2] - x64, axis=0) / np.linalg.norm(x64, axis=0)
-    print("solution error vs FP64", err, flush=True)
+    err10 = np.linalg.norm(x10[:, :2] - x64, axis=0) / np.linalg.norm(x64, axis=0)
+    print("solution error vs FP64: refined to 1e-11 %s, to 1e-10 %s" % (err, err10), flush=True)
     assert st64["converged"] == 1
-    assert err.max() <= 1e-5
+    assert err.max() <= 1e-5 and err10.max() <= 1e-5
     p, e, bc = C.mesh_arrays(m)
     pts = np.vstack([p, m.chief()]).astype(np.float64)
     ao = np.zeros_like(a32)
