@@ -74,6 +74,8 @@ def parse():
     ap.add_argument("--workers", type=int, default=4, help="concurrent solvers of the batch driver")
     ap.add_argument("--batch-method", choices=["gmres", "block", "cgls"], default="gmres",
                     help="solver of the batching workload")
+    ap.add_argument("--batch-tune", action="store_true",
+                    help="run the product tile tuner for the batching size before the timed batch")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-slots", type=int, default=4)
@@ -193,7 +195,7 @@ def cpu_baseline_large(mesh, k, rho, c, R, products, threads):
     }
 
 
-def bench_batching(bem, C, workers, fp64=True, method=1):
+def bench_batching(bem, C, workers, fp64=True, method=1, tune=False):
     """Multi-frequency, multi-right-hand-side batch: seeded perturbed ellipsoid
     (semi-axes 0.075/0.09/0.11 m, 8 spherical-harmonic bumps up to degree 6,
     amplitude <= 5%) at icosphere subdivision 4 (2562 nodes), rigid, 64
@@ -211,6 +213,11 @@ def bench_batching(bem, C, workers, fp64=True, method=1):
     ks = (2 * np.pi * freqs / c).astype(np.float32)
     dirs = np.stack([C.sphere_points(1030 + f, R, 1.0) for f in range(nf)]).astype(np.float32)
     kw = dict(method=method, restart=100, tol=1e-6, maxit=6000, refine_tol=1e-8, max_refine=50)
+    tuned = None
+    if tune:
+        # the product tile tuner for this size class (outside the timed region)
+        tuned = bem.autotune_products(M, N, R)
+        log("batching: product tuner %s" % tuned)
     bem.batch_solve(mesh, ks[-workers:], bem.RHS_PLANE, dirs[-workers:], workers=workers, **kw)  # warm-up
     bem.sync()
     t0 = time.perf_counter()
@@ -262,6 +269,7 @@ def bench_batching(bem, C, workers, fp64=True, method=1):
         "value": round(value, 2), "unit": "solves/s", "wall_s": round(wall, 4), "solves": nf * R,
         "iterations": {"min": int(min(its)), "median": int(np.median(its)), "max": int(max(its))},
         "converged": conv,
+        "product_tuner": tuned,
         "rel_err_vs_fp64_max": max(errs) if errs else None,
         "fp64_single_rhs_solves_per_s": fp64_rate and round(fp64_rate, 3),
         "speedup_vs_fp64_single_rhs": fp64_rate and round(value / fp64_rate, 2),
@@ -714,7 +722,8 @@ def main():
         elif args.only == "precision":
             res = bench_precision(bem, C, args.precision_steps, fp64=fp64)
         elif args.only == "batching":
-            res = bench_batching(bem, C, args.workers, fp64=fp64, method=SERVE_METHODS[args.batch_method](bem))
+            res = bench_batching(bem, C, args.workers, fp64=fp64, method=SERVE_METHODS[args.batch_method](bem),
+                                 tune=args.batch_tune)
         elif args.only == "large":
             res = bench_large(bem, C, args.steps, args.warmup, fp64=fp64, cpu=not args.no_cpu_baseline,
                               method=args.large_method, refine_tol=args.large_refine_tol)
@@ -736,7 +745,7 @@ def main():
         log("plain", sections["plain"])
     if not args.no_batching:
         sections["batching"] = bench_batching(bem, C, args.workers, fp64=fp64,
-                                              method=SERVE_METHODS[args.batch_method](bem))
+                                              method=SERVE_METHODS[args.batch_method](bem), tune=args.batch_tune)
         log("batching", sections["batching"])
     if not args.no_serving:
         sections["serving"] = bench_serving(bem, C, args.serve_requests, args.serve_rate, args.serve_slots,
