@@ -376,9 +376,11 @@ static int bdot(KrylovWS &ws, const double2 *V, long vstride, int nv, const doub
     dim3 g1(nblk, ws.R, (nv + 3) / 4);
     kb_mdot_part<<<g1, 256, 0, s>>>(V, ws.ldv, vstride, nv, w, wstride, n, ws.part, ws.pstride);
     HIP_CALL(hipGetLastError());
-    dim3 g2((nv + 255) / 256, ws.R);
-    kb_mdot_final<<<g2, 256, 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc,
-                                                        astride ? astride : ostride);
+    // one wave per 64 sums, at most 256 threads per workgroup (long restarts
+    // take several workgroups per system)
+    const int bt = std::min(256, 64 * ((nv + 63) / 64));
+    dim3 g2((nv + bt - 1) / bt, ws.R);
+    kb_mdot_final<<<g2, bt, 0, s>>>(ws.part, ws.pstride, nblk, nv, out, ostride, acc, astride ? astride : ostride);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
