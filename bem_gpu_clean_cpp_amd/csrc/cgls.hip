@@ -55,6 +55,7 @@
 #include <vector>
 
 #include "bem_internal.hpp"
+#include "small_dense.hpp"
 
 namespace bem {
 
@@ -301,60 +302,12 @@ __global__ void kc_scatter_add(int N, const double2 *__restrict__ D, long ldd, c
 namespace {
 
 constexpr int kChunk = 8;  // CG steps between reads of the activity flags
-
-// ---- R x R algebra on the host (column-major, FP64 complex)
-using cd = std::complex<double>;
-using Small = std::vector<cd>;
-// G = L L^H (lower L) from G's lower triangle; false when a pivot is not
-// positive beyond rounding (relative to the largest diagonal entry)
-bool chol_lower(const Small &G, int n, Small &L) {
-    L.assign((size_t)n * n, cd(0));
-    double gmax = 0;
-    for (int j = 0; j < n; j++) gmax = std::max(gmax, G[(size_t)j * n + j].real());
-    if (!(gmax > 0) || !std::isfinite(gmax)) return false;
-    for (int j = 0; j < n; j++) {
-        double d = G[(size_t)j * n + j].real();
-        for (int k = 0; k < j; k++) d -= std::norm(L[(size_t)k * n + j]);
-        if (!(d > 1e-14 * gmax)) return false;
-        const double ljj = std::sqrt(d);
-        L[(size_t)j * n + j] = ljj;
-        for (int i = j + 1; i < n; i++) {
-            cd v = G[(size_t)j * n + i];
-            for (int k = 0; k < j; k++) v -= L[(size_t)k * n + i] * std::conj(L[(size_t)k * n + j]);
-            L[(size_t)j * n + i] = v / ljj;
-        }
-    }
-    return true;
-}
-// inverse of a lower-triangular L (forward substitution per column)
-Small lower_inv(const Small &L, int n) {
-    Small X((size_t)n * n, cd(0));
-    for (int j = 0; j < n; j++) {
-        X[(size_t)j * n + j] = 1.0 / L[(size_t)j * n + j];
-        for (int i = j + 1; i < n; i++) {
-            cd v = 0;
-            for (int k = j; k < i; k++) v += L[(size_t)k * n + i] * X[(size_t)j * n + k];
-            X[(size_t)j * n + i] = -v / L[(size_t)i * n + i];
-        }
-    }
-    return X;
-}
-Small herm(const Small &A, int n) {  // A^H
-    Small B((size_t)n * n);
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < n; i++) B[(size_t)j * n + i] = std::conj(A[(size_t)i * n + j]);
-    return B;
-}
-Small mul(const Small &A, const Small &B, int n) {
-    Small C((size_t)n * n, cd(0));
-    for (int j = 0; j < n; j++)
-        for (int k = 0; k < n; k++) {
-            const cd b = B[(size_t)j * n + k];
-            if (b == cd(0)) continue;
-            for (int i = 0; i < n; i++) C[(size_t)j * n + i] += A[(size_t)k * n + i] * b;
-        }
-    return C;
-}
+using small::cd;
+using small::Small;
+using small::chol_lower;
+using small::herm;
+using small::lower_inv;
+using small::mul;
 
 struct CglsWS {
     long ld = 0;
