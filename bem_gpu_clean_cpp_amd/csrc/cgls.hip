@@ -50,6 +50,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <initializer_list>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -162,19 +163,29 @@ __global__ void kc_act_init(int R, const double *__restrict__ g0, const int *__r
 
 // ------------------------------------------------------------ block CG ---
 constexpr int kMaxBlock = 64;    // right-hand sides of one block (Rb <= 64)
+constexpr double kBlockMinWork = 4e8;  // M N R below which block CG takes the lockstep iteration
 constexpr int kGramRows = 64;    // rows of one Gram partial
 constexpr int kGramSub = 32;     // rows staged in LDS at a time
 constexpr int kGramSegs = 16;    // first-level groups of partials in the sum
 constexpr int kMulCols = 8;      // output columns per thread of the block product
 
-// part[blk][a + Rb b] = sum over the block's rows i of conj(U[i, a]) V[i, b]
-// (U, V: n x Rb, column-major, ld); FP64, each entry summed in row order.
-// Thread t owns the entries e = t + 256 u (a = e % Rb, b = e / Rb).
-__global__ __launch_bounds__(256) void kc_gram_part(const double2 *__restrict__ U, const double2 *__restrict__ V,
-                                                    long ld, int n, int Rb, double2 *__restrict__ part) {
+// Up to four Gram products U_y^H V_y of one launch (grid.y = y)
+constexpr int kMaxPairs = 4;
+struct GramPairs {
+    const double2 *U[kMaxPairs];
+    const double2 *V[kMaxPairs];
+};
+
+// part[y][blk][a + Rb b] = sum over the block's rows i of conj(U_y[i, a])
+// V_y[i, b] (U_y, V_y: n x Rb, column-major, ld); FP64, each entry summed in
+// row order.  Thread t owns the entries e = t + 256 u (a = e % Rb, b = e / Rb).
+__global__ __launch_bounds__(256) void kc_gram_part(GramPairs gp, long ld, int n, int Rb, double2 *__restrict__ part) {
     __shared__ double2 Us[kGramSub * kMaxBlock], Vs[kGramSub * kMaxBlock];  // [row][column]
     const int RR = Rb * Rb;
     const int i0 = blockIdx.x * kGramRows;
+    const double2 *__restrict__ U = gp.U[blockIdx.y];
+    const double2 *__restrict__ V = gp.V[blockIdx.y];
+    part += (size_t)blockIdx.y * gridDim.x * RR;
     constexpr int kPer = kMaxBlock * kMaxBlock / 256;
     double2 acc[kPer];
     int ia[kPer], ib[kPer];
@@ -217,10 +228,12 @@ __global__ __launch_bounds__(256) void kc_gram_part(const double2 *__restrict__ 
     }
 }
 
-// seg[g][e] = sum of part[blk][e] over the g-th run of blocks, in order
+// seg[y][g][e] = sum of part[y][blk][e] over the g-th run of blocks, in order
 __global__ void kc_gram_sum1(const double2 *__restrict__ part, int nblk, int RR, double2 *__restrict__ seg) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
     if (e >= RR) return;
+    part += (size_t)blockIdx.z * nblk * RR;
+    seg += (size_t)blockIdx.z * kGramSegs * RR;
     const int per = (nblk + kGramSegs - 1) / kGramSegs;
     const int b1 = min(nblk, (g + 1) * per);
     double2 sacc = make_double2(0, 0);
@@ -231,10 +244,12 @@ __global__ void kc_gram_sum1(const double2 *__restrict__ part, int nblk, int RR,
     }
     seg[(size_t)g * RR + e] = sacc;
 }
-// G[e] = sum_g seg[g][e], in order
+// G[y][e] = sum_g seg[y][g][e], in order
 __global__ void kc_gram_sum2(const double2 *__restrict__ seg, int RR, double2 *__restrict__ G) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= RR) return;
+    seg += (size_t)blockIdx.y * kGramSegs * RR;
+    G += (size_t)blockIdx.y * RR;
     double2 sacc = make_double2(0, 0);
 #pragma unroll
     for (int g = 0; g < kGramSegs; g++) {
@@ -338,6 +353,10 @@ struct CglsWS {
     }
 };
 constexpr int kCSlots = 4;  // staging slots of R x R matrices (at most 2 uploads between two host waits)
+// then the matrices of one block step: area 0 (4, after the step's Gram
+// products) and area 1 (2, after a second QR pass); each area is rewritten
+// only after a host wait that follows its last upload
+constexpr int kStepMats = 6;
 
 }  // namespace
 
@@ -373,19 +392,26 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     HOST_CALL(salloc(&w.done, R, s));
     HOST_CALL(salloc(&w.gws, std::max(std::max(ws_gemv_n(M, N, 1), ws_gemm(M, N, R)), ws_mres(M, N, R)), s));
     if (f32 && R > 1) HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &w.AH, s));
-    bool block_on = block && R > 1;
+    // Block steps pay three dozen small launches, a host wait and the R x R
+    // algebra on the host (about 0.1 ms at R = 16, 0.3 ms at R = 32); below
+    // about 4e8 of M N R a lockstep CG step is that cheap that the block's
+    // 2x fewer steps do not repay it (s4 with 16 right-hand sides: 7.9 ms
+    // against 4.8 ms; s5 with 32: 33.5 ms against 51.3 ms).  BEM_BLOCK_CG_ALWAYS
+    // takes the block at every size.
+    bool block_on = block && R > 1 &&
+                    ((double)M * (double)N * (double)R >= kBlockMinWork || getenv("BEM_BLOCK_CG_ALWAYS"));
     const int gblk = (N + kGramRows - 1) / kGramRows;
     if (block_on) {
         for (double2 **b : {&w.bQ, &w.bS, &w.bP, &w.bP2, &w.bW, &w.bX}) HOST_CALL(salloc(b, nv, s));
         if (f32) HOST_CALL(salloc(&w.bP32, nv, s));
-        HOST_CALL(salloc(&w.gpart, (size_t)gblk * kMaxBlock * kMaxBlock, s));
-        HOST_CALL(salloc(&w.gseg, (size_t)kGramSegs * kMaxBlock * kMaxBlock, s));
-        HOST_CALL(salloc(&w.gsum, (size_t)kMaxBlock * kMaxBlock, s));
-        HOST_CALL(salloc(&w.dcm, (size_t)kCSlots * kMaxBlock * kMaxBlock, s));
+        const size_t mm = (size_t)kMaxBlock * kMaxBlock;
+        HOST_CALL(salloc(&w.gpart, (size_t)kMaxPairs * gblk * mm, s));
+        HOST_CALL(salloc(&w.gseg, (size_t)kMaxPairs * kGramSegs * mm, s));
+        HOST_CALL(salloc(&w.gsum, (size_t)kMaxPairs * mm, s));
+        HOST_CALL(salloc(&w.dcm, (size_t)(kCSlots + kStepMats) * mm, s));
         HOST_CALL(salloc(&w.bidx, R, s));
-        HIP_CALL(hipHostMalloc((void **)&w.hgram, sizeof(double2) * kMaxBlock * kMaxBlock, hipHostMallocDefault));
-        HIP_CALL(hipHostMalloc((void **)&w.hcm, sizeof(double2) * kCSlots * kMaxBlock * kMaxBlock,
-                               hipHostMallocDefault));
+        HIP_CALL(hipHostMalloc((void **)&w.hgram, sizeof(double2) * kMaxPairs * mm, hipHostMallocDefault));
+        HIP_CALL(hipHostMalloc((void **)&w.hcm, sizeof(double2) * (kCSlots + kStepMats) * mm, hipHostMallocDefault));
     }
     const int bs = 256;
     const dim3 gN((N + bs - 1) / bs, R);
@@ -477,17 +503,27 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
 
     // ---- block CG pieces
     const bool trace = getenv("BEM_TRACE_SOLVE") != nullptr;
-    // G (Rb x Rb, host) = U^H V over the N rows
-    auto gram = [&](const double2 *U, const double2 *V, int Rb, Small &G) -> int {
-        kc_gram_part<<<gblk, 256, 0, s>>>(U, V, ld, N, Rb, w.gpart);
-        kc_gram_sum1<<<dim3((Rb * Rb + 63) / 64, kGramSegs), 64, 0, s>>>(w.gpart, gblk, Rb * Rb, w.gseg);
-        kc_gram_sum2<<<(Rb * Rb + 255) / 256, 256, 0, s>>>(w.gseg, Rb * Rb, w.gsum);
+    // G[y] (Rb x Rb, host) = U_y^H V_y over the N rows for npair pairs: one
+    // launch of the partials, one wait
+    auto grams = [&](const GramPairs &gp, int npair, int Rb, Small *G) -> int {
+        const int RR = Rb * Rb;
+        kc_gram_part<<<dim3(gblk, npair), 256, 0, s>>>(gp, ld, N, Rb, w.gpart);
+        kc_gram_sum1<<<dim3((RR + 63) / 64, kGramSegs, npair), 64, 0, s>>>(w.gpart, gblk, RR, w.gseg);
+        kc_gram_sum2<<<dim3((RR + 255) / 256, npair), 256, 0, s>>>(w.gseg, RR, w.gsum);
         HIP_CALL(hipGetLastError());
-        HIP_CALL(hipMemcpyAsync(w.hgram, w.gsum, sizeof(double2) * Rb * Rb, hipMemcpyDeviceToHost, s));
+        HIP_CALL(hipMemcpyAsync(w.hgram, w.gsum, sizeof(double2) * RR * npair, hipMemcpyDeviceToHost, s));
         HIP_CALL(hipStreamSynchronize(s));
-        G.resize((size_t)Rb * Rb);
-        for (int e = 0; e < Rb * Rb; e++) G[e] = cd(w.hgram[e].x, w.hgram[e].y);
+        for (int y = 0; y < npair; y++) {
+            G[y].resize((size_t)RR);
+            for (int e = 0; e < RR; e++) G[y][e] = cd(w.hgram[(size_t)y * RR + e].x, w.hgram[(size_t)y * RR + e].y);
+        }
         return EXIT_SUCCESS;
+    };
+    auto gram = [&](const double2 *U, const double2 *V, int Rb, Small &G) -> int {
+        GramPairs gp{};
+        gp.U[0] = U;
+        gp.V[0] = V;
+        return grams(gp, 1, Rb, &G);
     };
     // out = Z (or 0) + U C for an Rb x Rb host matrix C
     int cslot = 0;
@@ -498,6 +534,28 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         cslot = (cslot + 1) % kCSlots;
         for (int e = 0; e < Rb * Rb; e++) hc[e] = make_double2(Cm[e].real(), Cm[e].imag());
         HIP_CALL(hipMemcpyAsync(dc, hc, sizeof(double2) * Rb * Rb, hipMemcpyHostToDevice, s));
+        kc_block_mul<<<dim3((N + 255) / 256, (Rb + kMulCols - 1) / kMulCols), 256, 0, s>>>(U, dc, Rb, Z, out,
+                                                                                           out32, ld, N);
+        HIP_CALL(hipGetLastError());
+        return EXIT_SUCCESS;
+    };
+    // the R x R matrices of a block step, uploaded together into a step area;
+    // *dc receives the device copy (matrix q at dc + q Rb^2)
+    auto upload = [&](int area, std::initializer_list<const Small *> mats, int Rb, const double2 **dc) -> int {
+        const size_t mm = (size_t)kMaxBlock * kMaxBlock, RR = (size_t)Rb * Rb;
+        const size_t off = (kCSlots + (size_t)area * 4) * mm;
+        double2 *h = w.hcm + off;
+        size_t q = 0;
+        for (const Small *m : mats) {
+            for (size_t e = 0; e < RR; e++) h[q * RR + e] = make_double2((*m)[e].real(), (*m)[e].imag());
+            q++;
+        }
+        HIP_CALL(hipMemcpyAsync(w.dcm + off, h, sizeof(double2) * RR * q, hipMemcpyHostToDevice, s));
+        *dc = w.dcm + off;
+        return EXIT_SUCCESS;
+    };
+    auto bmul_dev = [&](const double2 *U, const double2 *dc, int Rb, const double2 *Z, double2 *out,
+                        float2 *out32) -> int {
         kc_block_mul<<<dim3((N + 255) / 256, (Rb + kMulCols - 1) / kMulCols), 256, 0, s>>>(U, dc, Rb, Z, out,
                                                                                            out32, ld, N);
         HIP_CALL(hipGetLastError());
@@ -557,37 +615,127 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         Small eye((size_t)Rb * Rb, cd(0));
         for (int c = 0; c < Rb; c++) eye[(size_t)c * Rb + c] = 1.0;
         HOST_CALL(bmul(w.bQ, eye, Rb, nullptr, w.bP, f32 ? w.bP32 : nullptr));  // P = Q
+        const int RR = Rb * Rb;
+        // residual of system c is |xi[:, c]|
+        auto converged = [&](const Small &x, double &worst) {
+            bool all = true;
+            worst = 0;
+            for (int c = 0; c < Rb; c++) {
+                double nrm = 0;
+                for (int r = 0; r < Rb; r++) nrm += std::norm(x[(size_t)c * Rb + r]);
+                nrm = std::sqrt(nrm);
+                worst = std::max(worst, thr[c] > 0 ? nrm / thr[c] : 0.0);
+                all = all && nrm <= thr[c];
+            }
+            return all;
+        };
+        auto fro2 = [&](const Small &x) {
+            double t = 0;
+            for (const cd &v : x) t += std::norm(v);
+            return t;
+        };
         while (total < maxit) {
             HOST_CALL(bprod(Rb));
-            Small D, Ld;
-            HOST_CALL(gram(w.bP, w.bW, Rb, D));  // P^H A^H A P, Hermitian positive definite
-            if (!chol_lower(D, Rb, Ld)) {
+            // one launch, one wait: D = P^H W, E = Q^H W, F = W^H W, Q^H Q
+            GramPairs gp{};
+            gp.U[0] = w.bP; gp.V[0] = w.bW;
+            gp.U[1] = w.bQ; gp.V[1] = w.bW;
+            gp.U[2] = w.bW; gp.V[2] = w.bW;
+            gp.U[3] = w.bQ; gp.V[3] = w.bQ;
+            Small Gs[4];
+            HOST_CALL(grams(gp, 4, Rb, Gs));
+            Small Ld;
+            if (!chol_lower(Gs[0], Rb, Ld)) {  // D = P^H A^H A P, Hermitian positive definite
                 broke = true;
                 break;
             }
             const Small Li = lower_inv(Ld, Rb);
-            Small alpha = mul(herm(Li, Rb), Li, Rb);  // D^-1
-            HOST_CALL(bmul(w.bP, mul(alpha, xi, Rb), Rb, w.bX, w.bX, nullptr));  // X += P alpha xi
-            for (cd &v : alpha) v = -v;
-            HOST_CALL(bmul(w.bW, alpha, Rb, w.bQ, w.bS, nullptr));  // S = Q - W alpha
-            HOST_CALL(cholqr2(Rb, zeta, ok));
+            const Small alpha = mul(herm(Li, Rb), Li, Rb);  // D^-1
+            const Small axi = mul(alpha, xi, Rb);
+            Small malpha = alpha;
+            for (cd &v : malpha) v = -v;
+            // S^H S for S = Q - W alpha from the Gram products:
+            //   Q^H Q - E alpha - (E alpha)^H + alpha^H F alpha
+            const Small Ea = mul(Gs[1], alpha, Rb);
+            const Small aFa = mul(herm(alpha, Rb), mul(Gs[2], alpha, Rb), Rb);
+            Small G1((size_t)RR);
+            for (int j = 0; j < Rb; j++)
+                for (int i = 0; i < Rb; i++)
+                    G1[(size_t)j * Rb + i] = Gs[3][(size_t)j * Rb + i] - Ea[(size_t)j * Rb + i] -
+                                             std::conj(Ea[(size_t)i * Rb + j]) + aFa[(size_t)j * Rb + i];
+            const double gterms = std::sqrt(fro2(Gs[3])) + 2 * std::sqrt(fro2(Ea)) + std::sqrt(fro2(aFa));
+            Small L1;
+            const bool alg = chol_lower(G1, Rb, L1);
             total++;
             its++;
-            if (!ok) {
-                broke = true;
-                break;
-            }
-            xi = mul(zeta, xi, Rb);
-            HOST_CALL(bmul(w.bP, herm(zeta, Rb), Rb, w.bQ, w.bP2, f32 ? w.bP32 : nullptr));  // P = Q + P zeta^H
-            std::swap(w.bP, w.bP2);
-            bool all = true;
             double worst = 0;
-            for (int c = 0; c < Rb; c++) {
-                double nrm = 0;
-                for (int r = 0; r < Rb; r++) nrm += std::norm(xi[(size_t)c * Rb + r]);
-                nrm = std::sqrt(nrm);
-                worst = std::max(worst, thr[c] > 0 ? nrm / thr[c] : 0.0);
-                all = all && nrm <= thr[c];
+            bool all = false;
+            if (alg) {
+                const Small L1i = lower_inv(L1, Rb), L1ih = herm(L1i, Rb);
+                // Q' = S L1^-H is orthonormal up to the rounding of the Gram
+                // terms times |L1^-1|^2 (FP64 sums of N products); one pass
+                // when that stays below 1e-9, else a second CholeskyQR pass on
+                // an explicit Gram product
+                const bool one_pass = 1e-14 * gterms * fro2(L1i) <= 1e-9;
+                const double2 *dc = nullptr;
+                if (one_pass) {
+                    zeta = herm(L1, Rb);
+                    const Small zh = herm(zeta, Rb);
+                    const Small xin = mul(zeta, xi, Rb);
+                    all = converged(xin, worst);
+                    HOST_CALL(upload(0, {&axi, &malpha, &L1ih, &zh}, Rb, &dc));
+                    HOST_CALL(bmul_dev(w.bP, dc, Rb, w.bX, w.bX, nullptr));  // X += P alpha xi
+                    if (!all) {
+                        HOST_CALL(bmul_dev(w.bW, dc + RR, Rb, w.bQ, w.bS, nullptr));         // S = Q - W alpha
+                        HOST_CALL(bmul_dev(w.bS, dc + 2 * RR, Rb, nullptr, w.bQ, nullptr));  // Q = S L1^-H
+                        HOST_CALL(bmul_dev(w.bP, dc + 3 * RR, Rb, w.bQ, w.bP2, f32 ? w.bP32 : nullptr));
+                        std::swap(w.bP, w.bP2);  // P = Q + P zeta^H
+                    }
+                    xi = xin;
+                } else {
+                    HOST_CALL(upload(0, {&axi, &malpha, &L1ih}, Rb, &dc));
+                    HOST_CALL(bmul_dev(w.bP, dc, Rb, w.bX, w.bX, nullptr));
+                    HOST_CALL(bmul_dev(w.bW, dc + RR, Rb, w.bQ, w.bS, nullptr));
+                    HOST_CALL(bmul_dev(w.bS, dc + 2 * RR, Rb, nullptr, w.bQ, nullptr));  // Q1
+                    Small G2, L2;
+                    HOST_CALL(gram(w.bQ, w.bQ, Rb, G2));
+                    if (!chol_lower(G2, Rb, L2)) {
+                        broke = true;
+                        break;
+                    }
+                    zeta = mul(herm(L2, Rb), herm(L1, Rb), Rb);
+                    const Small zh = herm(zeta, Rb), L2ih = herm(lower_inv(L2, Rb), Rb);
+                    const Small xin = mul(zeta, xi, Rb);
+                    all = converged(xin, worst);
+                    if (!all) {
+                        HOST_CALL(upload(1, {&L2ih, &zh}, Rb, &dc));
+                        HOST_CALL(bmul_dev(w.bQ, dc, Rb, nullptr, w.bS, nullptr));
+                        std::swap(w.bQ, w.bS);  // Q = Q1 L2^-H
+                        HOST_CALL(bmul_dev(w.bP, dc + RR, Rb, w.bQ, w.bP2, f32 ? w.bP32 : nullptr));
+                        std::swap(w.bP, w.bP2);
+                    }
+                    xi = xin;
+                }
+            } else {
+                // S^H S from the terms is not positive definite beyond rounding:
+                // form S on the device and take CholeskyQR2 of it
+                const double2 *dc = nullptr;
+                HOST_CALL(upload(0, {&axi, &malpha}, Rb, &dc));
+                HOST_CALL(bmul_dev(w.bP, dc, Rb, w.bX, w.bX, nullptr));
+                HOST_CALL(bmul_dev(w.bW, dc + RR, Rb, w.bQ, w.bS, nullptr));
+                bool ok = false;
+                HOST_CALL(cholqr2(Rb, zeta, ok));
+                if (!ok) {
+                    broke = true;
+                    break;
+                }
+                const Small xin = mul(zeta, xi, Rb);
+                all = converged(xin, worst);
+                if (!all) {
+                    HOST_CALL(bmul(w.bP, herm(zeta, Rb), Rb, w.bQ, w.bP2, f32 ? w.bP32 : nullptr));
+                    std::swap(w.bP, w.bP2);
+                }
+                xi = xin;
             }
             if (trace) fprintf(stderr, "block cg step %d: worst |res| / threshold %.3e\n", total, worst);
             if (all) break;

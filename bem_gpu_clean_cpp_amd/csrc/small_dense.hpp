@@ -87,21 +87,36 @@ inline Small herm(const Small &A, int n) {
     return B;
 }
 
-// A B
+// A B.  The real and imaginary parts of a column of the product accumulate
+// in separate arrays (no aliasing, contiguous lanes), then interleave.
 inline Small mul(const Small &A, const Small &B, int n) {
-    Small C((size_t)n * n, cd(0));
-    const double *a = dp(A), *b = dp(B);
-    double *c = dp(C);
+    Small C((size_t)n * n);
+    const double *__restrict__ a = dp(A);
+    const double *__restrict__ b = dp(B);
+    double *__restrict__ c = dp(C);
+    std::vector<double> ar((size_t)n * n), ai((size_t)n * n);
+    for (size_t e = 0; e < (size_t)n * n; e++) {
+        ar[e] = a[2 * e];
+        ai[e] = a[2 * e + 1];
+    }
+    std::vector<double> cr(n), ci(n);
     for (int j = 0; j < n; j++) {
-        double *cj = c + 2 * (size_t)j * n;
+        for (int i = 0; i < n; i++) cr[i] = ci[i] = 0.0;
         for (int k = 0; k < n; k++) {
             const double br = b[2 * ((size_t)j * n + k)], bi = b[2 * ((size_t)j * n + k) + 1];
             if (br == 0.0 && bi == 0.0) continue;
-            const double *ak = a + 2 * (size_t)k * n;
+            const double *__restrict__ xr = ar.data() + (size_t)k * n;
+            const double *__restrict__ xi = ai.data() + (size_t)k * n;
+            double *__restrict__ yr = cr.data();
+            double *__restrict__ yi = ci.data();
             for (int i = 0; i < n; i++) {
-                cj[2 * i] += ak[2 * i] * br - ak[2 * i + 1] * bi;
-                cj[2 * i + 1] += ak[2 * i] * bi + ak[2 * i + 1] * br;
+                yr[i] += xr[i] * br - xi[i] * bi;
+                yi[i] += xr[i] * bi + xi[i] * br;
             }
+        }
+        for (int i = 0; i < n; i++) {
+            c[2 * ((size_t)j * n + i)] = cr[i];
+            c[2 * ((size_t)j * n + i) + 1] = ci[i];
         }
     }
     return C;

@@ -242,7 +242,10 @@ int bem_solver_set_rhs(bem_solver *s, int kind, const float *params, int nrhs, f
  * conjugate gradients over all right-hand sides at once (one shared block
  * Krylov space, orthonormalised residual block), with the same refinement;
  * falls back to BEM_SOLVE_CGLS's iteration when the block breaks down (e.g.
- * two equal right-hand sides).  iterations counts block steps. */
+ * two equal right-hand sides), and takes that iteration from the start when
+ * M * N * nrhs < 4e8, where a block step's fixed cost outweighs its fewer
+ * steps (environment BEM_BLOCK_CG_ALWAYS: the block at every size).
+ * iterations counts block steps. */
 int bem_solver_solve(bem_solver *s, int method, int restart, float tol, int maxit, double refine_tol,
                      int max_refine, bem_solve_stats *stats);
 /* Copy the nodal solution (N x nrhs, ldx) to the host. */

@@ -9,7 +9,11 @@ refinement.  Checks:
   * the same solution as the lockstep CG after refinement to 1e-10 (1e-8);
   * fewer block steps than lockstep CG steps for 20 right-hand sides;
   * two equal right-hand sides (a rank-deficient block): the block cannot
-    start, the solve falls back to the lockstep CG and still meets the bar.
+    start, the solve falls back to the lockstep CG and still meets the bar;
+  * below 4e8 of M N nrhs the method takes the lockstep iteration: the result
+    equals BEM_SOLVE_CGLS's bit for bit.
+These systems are small, so every test but the last sets BEM_BLOCK_CG_ALWAYS
+to take the block iteration.
 """
 import numpy as np
 import pytest
@@ -31,6 +35,12 @@ def _setup():
     bem.set_medium(RHO, SPEED)
     yield
     bem.set_medium(1.2041, 343.21)
+
+
+@pytest.fixture(autouse=True)
+def _always_block(request, monkeypatch):
+    if "size_policy" not in request.node.name:
+        monkeypatch.setenv("BEM_BLOCK_CG_ALWAYS", "1")
 
 
 def _mesh(s, nchief, r=0.1, seed=7):
@@ -134,3 +144,22 @@ def test_block_cg_equal_right_hand_sides_fall_back():
     assert st["converged"] == 1
     assert err.max() < 1e-5
     assert np.array_equal(x[:, 0], x[:, 3])
+
+
+def test_block_cg_size_policy_small_system_takes_lockstep(monkeypatch):
+    """s3 with 8 plane waves (M N nrhs = 3.3e6): the block method runs the
+    lockstep CG iteration, so the solution is BEM_SOLVE_CGLS's bit for bit."""
+    monkeypatch.delenv("BEM_BLOCK_CG_ALWAYS", raising=False)
+    m = _mesh(3, 8)
+    k = 31.3
+    d = C.sphere_points(11, 8, 1.0)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    sb = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+    xb = s.solution()
+    sl = s.solve(bem.SOLVE_CGLS, **KW)
+    xl = s.solution()
+    print("case: block CG size policy at s3: %s / %s" % (sb, sl), flush=True)
+    assert sb["iterations"] == sl["iterations"]
+    assert np.array_equal(xb, xl)
