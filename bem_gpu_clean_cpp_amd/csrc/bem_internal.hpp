@@ -305,6 +305,13 @@ int gemm_hc_f32(const float2 *AH, long M, long N, const float2 *T, long ldt, flo
 int gemm_hc_f32z(const float2 *AH, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R, void *ws,
                  hipStream_t s);
 size_t ws_mres(long M, long N, int R);
+// nmat matrices astride elements apart, matrix z on right-hand sides
+// z R / nmat .. (z + 1) R / nmat - 1: the FP32 product with FP64 output and
+// the FP64-accumulated residual b - A X (ws >= ws_gemm(M, N, R))
+int gemm_n_f32z_batched(const float2 *A, long lda, long astride, int nmat, long M, long N, const float2 *X, long ldx,
+                        double2 *Y, long ldy, int R, void *ws, hipStream_t s);
+int mres_f32a_batched(const float2 *A, long lda, long astride, int nmat, long M, long N, const double2 *X, long ldx,
+                      const double2 *b, long ldb, double2 *out, long ldo, int R, void *ws, hipStream_t s);
 // C -= V W (QR trailing update, V with 32 columns)
 int rank_update_f32(const float2 *V, long ldv, int M, const float2 *W, int ldw, int nc, float2 *C, long ldc,
                     hipStream_t s);
@@ -326,9 +333,12 @@ void gmres_cache_free(GmresCache *&c);
 // cache: null to allocate the workspace for this call only; else *cache is
 // reused when its shape matches and replaced when it does not (free it with
 // gmres_cache_free after the stream's work on it has been ordered)
+// astride, nmat > 1: nmat square FP32 matrices astride elements apart, matrix
+// z for systems z R / nmat .. (z + 1) R / nmat - 1 (frequencies in lockstep)
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
-                 int max_cycles, bem_solve_stats *st, hipStream_t s, GmresCache **cache = nullptr);
+                 int max_cycles, bem_solve_stats *st, hipStream_t s, GmresCache **cache = nullptr, long astride = 0,
+                 int nmat = 1);
 // CGLS (conjugate gradients on A^H A x = A^H b) for R systems in lockstep
 // (block = false) or as one block CG over all R (block = true, R <= 64),
 // FP64 residual refinement; X is overwritten (the solve starts from zero)

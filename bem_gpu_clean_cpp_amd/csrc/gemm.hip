@@ -79,15 +79,23 @@ __device__ __forceinline__ C2 keep2(bool k, C2 v) {
 // Loads are unconditional at clamped (in-bounds) addresses and masked to zero
 // afterwards, so the loop body is straight-line and the next block's loads stay
 // in flight while the current block is multiplied (ping-pong buffers).
+//
+// Several matrices at once (grid.z = matrix, astride apart): matrix z takes the
+// right-hand sides z rper .. z rper + rper - 1 (rg0 counts within them); a
+// single matrix has astride 0 and rper = R.
 template <typename T, int G>
 __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A, long lda, int M, int N,
                                                    const typename MF<T>::c2 *__restrict__ X, long ldx, int R, int rg0,
-                                                   int kc, typename MF<T>::c2 *__restrict__ part, long ldp) {
+                                                   int kc, typename MF<T>::c2 *__restrict__ part, long ldp,
+                                                   long astride, int rper) {
     typedef typename MF<T>::v4 v4;
     typedef typename MF<T>::c2 c2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row0 = (blockIdx.x * 4 + wave) * 32;
     if (row0 >= M) return;
+    A += (size_t)blockIdx.z * astride;
+    rg0 += blockIdx.z * rper;
+    const int rend = min(R, (int)(blockIdx.z + 1) * rper);  // end of this matrix's right-hand sides
     const int chunk = blockIdx.y;
     const int j0 = chunk * kc, j1 = min(N, j0 + kc);
     const int m = lane & 15, kk = lane >> 4;
@@ -100,10 +108,11 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
             for (int t = 0; t < 4; t++) acc[g][p][t] = (v4){0, 0, 0, 0};
     const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * m);
     const long lda4 = lda / 2;
-    // right-hand sides past R read column R - 1 (their results are not written)
+    // right-hand sides past this matrix's last read that one (their results
+    // are not written)
     const c2 *Xg[G];
 #pragma unroll
-    for (int g = 0; g < G; g++) Xg[g] = X + (size_t)min(rg0 + g * 16 + m, R - 1) * ldx;
+    for (int g = 0; g < G; g++) Xg[g] = X + (size_t)min(rg0 + g * 16 + m, rend - 1) * ldx;
     struct Buf {
         float4 a[kU];
         c2 x[kU][G];
@@ -163,7 +172,7 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int n = rg0 + g * 16 + m;
-        if (n >= R) continue;
+        if (n >= rend) continue;
         c2 *pc = part + ((size_t)chunk * R + n) * ldp;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
@@ -606,15 +615,21 @@ size_t ws_gemm(long M, long N, int R) {
     return (size_t)nch * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
 }
 
+// nmat > 1: nmat matrices astride elements apart, matrix z applied to the
+// right-hand sides z R / nmat .. (z + 1) R / nmat - 1 (16 x 16 x 4 kernels)
 template <typename T, typename S = T, typename OC2 = typename MF<T>::c2>
 static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename MF<T>::c2 *X, long ldx,
                        const OC2 *b, long ldb, OC2 *Y, long ldy, int R, void *ws, const char *name, hipStream_t s,
-                       bool long_chunks) {
+                       bool long_chunks, long astride = 0, int nmat = 1) {
     if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
     if (lda % 2 || lda < (M + 31) / 32 * 32) BEM_FAIL("gemm_n needs an even lda >= roundup(M, 32)");
+    if (nmat < 1 || R % nmat || (nmat > 1 && astride < lda * N) || nmat > 65535)
+        BEM_FAIL("gemm_n: %d matrices %ld apart for %d right-hand sides", nmat, astride, R);
+    const int rper = R / nmat;
     PShape ps;
-    const bool tuned = sizeof(T) == 4 && long_chunks && !getenv("BEM_GEMM_16X16") && tuned_product(0, M, N, R, ps);
-    const bool w32 = sizeof(T) == 4 && (tuned ? ps.wide != 0 : use32(R));
+    const bool tuned = nmat == 1 && sizeof(T) == 4 && long_chunks && !getenv("BEM_GEMM_16X16") &&
+                       tuned_product(0, M, N, R, ps);
+    const bool w32 = nmat == 1 && sizeof(T) == 4 && (tuned ? ps.wide != 0 : use32(R));
     if (w32 && lda < (M + 63) / 64 * 64) BEM_FAIL("gemm_n needs lda >= roundup(M, 64) for more than 16 right-hand sides");
     int nch, kc;
     if (tuned) {
@@ -630,16 +645,18 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
     TimedRegion tr;
     timing_begin(tr, name, s);
     const unsigned gx = (unsigned)((M + 127) / 128);
-    for (int rg = 0; rg < R; rg += 32) {
+    for (int rg = 0; rg < rper; rg += 32) {
         if (w32) {
             gemm_n_mfma32<<<dim3((unsigned)((M + 255) / 256), nch), 256, 0, s>>>(
                 A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc, (float2 *)part, ldp);
         } else {
-            dim3 grid(gx, nch);
-            if (R - rg > 16)
-                gemm_n_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
+            dim3 grid(gx, nch, nmat);
+            if (rper - rg > 16)
+                gemm_n_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp, astride,
+                                                       rper);
             else
-                gemm_n_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp);
+                gemm_n_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp, astride,
+                                                       rper);
         }
         HIP_CALL(hipGetLastError());
     }
@@ -798,6 +815,18 @@ int gemm_hc_f32z(const float2 *AH, long M, long N, const float2 *T, long ldt, do
                  hipStream_t s) {
     return gemm_n_impl<float, double, double2>(AH, adjoint_copy_ld(N), N, M, T, ldt, (const double2 *)nullptr, 0, Z,
                                                ldz, R, ws, "gemm_hc_f32", s, true);
+}
+
+// several matrices at once (astride apart, R / nmat right-hand sides each):
+// the Krylov product with FP64 output and the FP64 residual b - A X
+int gemm_n_f32z_batched(const float2 *A, long lda, long astride, int nmat, long M, long N, const float2 *X, long ldx,
+                        double2 *Y, long ldy, int R, void *ws, hipStream_t s) {
+    return gemm_n_impl<float, double, double2>(A, lda, M, N, X, ldx, (const double2 *)nullptr, 0, Y, ldy, R, ws,
+                                               "gemm_n_f32", s, true, astride, nmat);
+}
+int mres_f32a_batched(const float2 *A, long lda, long astride, int nmat, long M, long N, const double2 *X, long ldx,
+                      const double2 *b, long ldb, double2 *out, long ldo, int R, void *ws, hipStream_t s) {
+    return gemm_n_impl<double>(A, lda, M, N, X, ldx, b, ldb, out, ldo, R, ws, "mres_f64acc", s, true, astride, nmat);
 }
 
 // out (M x R) = b - A X (b == nullptr: out = A X): FP32 A widened to FP64,

@@ -409,6 +409,9 @@ struct Operator {
     const double2 *A64 = nullptr;
     long lda = 0;
     int M = 0, N = 0;
+    // nmat > 1 square FP32 matrices astride apart, each for R / nmat systems
+    long astride = 0;
+    int nmat = 1;
     bool normal() const { return M > N; }
     // w64_r <- op(V_r[:, j]) for every system (FP32 A reads v32)
     int apply(KrylovWS &ws, int j, hipStream_t s) const {
@@ -431,7 +434,10 @@ struct Operator {
                 HIP_CALL(hipGetLastError());
             } else {
                 // the products' fixed-order reduce writes the FP64 vector directly
-                if (!normal()) {
+                if (nmat > 1) {
+                    HOST_CALL(gemm_n_f32z_batched(A32, lda, astride, nmat, M, N, ws.v32, ws.ldv, ws.w64, ws.ldv, R,
+                                                  ws.gws, s));
+                } else if (!normal()) {
                     HOST_CALL(gemm_n_f32z(A32, lda, M, N, ws.v32, ws.ldv, ws.w64, ws.ldv, R, ws.gws, s));
                 } else {
                     HOST_CALL(gemm_n_f32(A32, lda, M, N, ws.v32, ws.ldv, ws.t32, ws.ldv, R, ws.gws, s, true));
@@ -484,6 +490,8 @@ struct Operator {
                     HOST_CALL(resid_n_f32a(A32, lda, M, N, X, 0, B, 0, ws.t64, 0, 1, ws.gws, s));
                     HOST_CALL(gemv_h_f32a(A32, lda, M, N, ws.t64, 0, r, 0, 1, s));
                 }
+            } else if (nmat > 1) {
+                HOST_CALL(mres_f32a_batched(A32, lda, astride, nmat, M, N, X, ldx, B, ldb, r, ws.ldv, R, ws.gws, s));
             } else if (!normal()) {
                 HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, r, ws.ldv, R, ws.gws, s));
             } else {
@@ -556,9 +564,11 @@ void gmres_cache_free(GmresCache *&c) {
 
 int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, const double2 *B, long ldb,
                  double2 *X, long ldx, bool x_zero, int R, int restart, double tol, int maxit, double outer_tol,
-                 int max_cycles, bem_solve_stats *st, hipStream_t s, GmresCache **cache) {
+                 int max_cycles, bem_solve_stats *st, hipStream_t s, GmresCache **cache, long astride, int nmat) {
     if (M < N) BEM_FAIL("GMRES needs M >= N (got %d x %d)", M, N);
     if (R <= 0) return EXIT_SUCCESS;
+    if (nmat > 1 && (!A32 || M != N || R % nmat || R / nmat < 2))
+        BEM_FAIL("GMRES over %d matrices needs square FP32 matrices and at least 2 right-hand sides each", nmat);
     restart = std::max(1, std::min(restart, N));
     const int K = restart;
     // Hessenberg column slots of a chunk: per system kMaxChunk slots of K + 2
@@ -610,6 +620,8 @@ int gmres_device(const float2 *A32, const double2 *A64, long lda, int M, int N, 
     op.lda = lda;
     op.M = M;
     op.N = N;
+    op.astride = astride;
+    op.nmat = std::max(1, nmat);
     if (A32 && M > N && R > 1) {
         if (ws.AH) HOST_CALL(adjoint_copy_fill(A32, lda, M, N, ws.AH, s));
         else HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &ws.AH, s));

@@ -297,9 +297,16 @@ typedef struct bem_solve_params {
  * 3 * nrhs, one per frequency.  X (may be null) receives N x (nrhs * nk) complex128 solutions, column
  * f * nrhs + r at X + (f * nrhs + r) * ldx; stats (nk entries, may be null)
  * each frequency's solve statistics.  `workers` resident solvers (<= 8), each
- * on its own stream and host thread, take frequencies in turn.  sp == NULL:
- * GMRES(30, 1e-6) refined to 1e-8.  Results equal those of one solver running
- * the frequencies one by one. */
+ * on its own stream and host thread, take frequencies in turn, highest
+ * wavenumber first.  sp == NULL: GMRES(30, 1e-6) refined to 1e-8.
+ * Results equal those of one solver running the frequencies one by one.
+ * Environment BEM_BATCH_FUSE = g (2..4): square FP32 systems (no CHIEF points)
+ * solved with GMRES and at least 2 right-hand sides go in groups of g
+ * frequencies (at most 64 systems, and within half the free HBM for the
+ * workers' matrices) that run one lockstep GMRES, every Arnoldi step one
+ * product launch over the group's matrices; a group's stats are those of its
+ * joint solve, and its results agree with the one-by-one solves to the
+ * refinement tolerance, independent of the worker count. */
 int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, int rhs_kind, const float *params,
                     int nrhs, int params_per_freq, float amp, const bem_solve_params *sp, int workers,
                     bem_zcomplex *X, int ldx, bem_solve_stats *stats);
