@@ -51,6 +51,7 @@
 #include <cmath>
 #include <complex>
 #include <initializer_list>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -324,6 +325,42 @@ using small::herm;
 using small::lower_inv;
 using small::mul;
 
+// Pinned host blocks of the block CG's R x R transfers, reused across solves
+// and threads: hipHostFree waits for the whole device, which would stall the
+// other solvers of a batch or server at every solve's end.  The blocks live
+// until the process ends (one per solve running at once).
+class PinnedPool {
+public:
+    static constexpr size_t kBytes = sizeof(double2) * (4 + 4 + 6) * 64 * 64;
+    static int acquire(double2 **p) {
+        {
+            std::lock_guard<std::mutex> lk(mu());
+            if (!free_list().empty()) {
+                *p = free_list().back();
+                free_list().pop_back();
+                return EXIT_SUCCESS;
+            }
+        }
+        HIP_CALL(hipHostMalloc((void **)p, kBytes, hipHostMallocDefault));
+        return EXIT_SUCCESS;
+    }
+    static void release(double2 *p) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(mu());
+        free_list().push_back(p);
+    }
+
+private:
+    static std::mutex &mu() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<double2 *> &free_list() {
+        static std::vector<double2 *> v;
+        return v;
+    }
+};
+
 struct CglsWS {
     long ld = 0;
     hipStream_t st = nullptr;
@@ -339,7 +376,7 @@ struct CglsWS {
     double2 *bQ = nullptr, *bS = nullptr, *bP = nullptr, *bP2 = nullptr, *bW = nullptr, *bX = nullptr;
     float2 *bP32 = nullptr;
     double2 *gpart = nullptr, *gseg = nullptr, *gsum = nullptr, *dcm = nullptr;
-    double2 *hgram = nullptr, *hcm = nullptr;
+    double2 *hgram = nullptr, *hcm = nullptr;  // one PinnedPool block: Gram sums, then R x R uploads
     int *bidx = nullptr;
     ~CglsWS() {
         sfree(rr, st); sfree(s64, st); sfree(d, st); sfree(p, st); sfree(q64, st); sfree(w64, st);
@@ -347,9 +384,10 @@ struct CglsWS {
         sfree(gam, st); sfree(del, st); sfree(g0, st); sfree(thr, st); sfree(act, st); sfree(done, st); sfree(gws, st);
         sfree(bQ, st); sfree(bS, st); sfree(bP, st); sfree(bP2, st); sfree(bW, st); sfree(bX, st); sfree(bP32, st);
         sfree(gpart, st); sfree(gseg, st); sfree(gsum, st); sfree(dcm, st); sfree(bidx, st);
-        if (hgram || hcm) (void)hipStreamSynchronize(st);  // the pinned slots may feed queued copies
-        if (hgram) (void)hipHostFree(hgram);
-        if (hcm) (void)hipHostFree(hcm);
+        if (hgram) {
+            (void)hipStreamSynchronize(st);  // the pinned slots may feed queued copies
+            PinnedPool::release(hgram);
+        }
     }
 };
 constexpr int kCSlots = 4;  // staging slots of R x R matrices (at most 2 uploads between two host waits)
@@ -410,8 +448,10 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         HOST_CALL(salloc(&w.gsum, (size_t)kMaxPairs * mm, s));
         HOST_CALL(salloc(&w.dcm, (size_t)(kCSlots + kStepMats) * mm, s));
         HOST_CALL(salloc(&w.bidx, R, s));
-        HIP_CALL(hipHostMalloc((void **)&w.hgram, sizeof(double2) * kMaxPairs * mm, hipHostMallocDefault));
-        HIP_CALL(hipHostMalloc((void **)&w.hcm, sizeof(double2) * (kCSlots + kStepMats) * mm, hipHostMallocDefault));
+        static_assert(PinnedPool::kBytes >= sizeof(double2) * (kMaxPairs + kCSlots + kStepMats) * kMaxBlock * kMaxBlock,
+                      "pinned block too small");
+        HOST_CALL(PinnedPool::acquire(&w.hgram));
+        w.hcm = w.hgram + kMaxPairs * mm;
     }
     const int bs = 256;
     const dim3 gN((N + bs - 1) / bs, R);
