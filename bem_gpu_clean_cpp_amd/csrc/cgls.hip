@@ -543,6 +543,10 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
 
     // ---- block CG pieces
     const bool trace = getenv("BEM_TRACE_SOLVE") != nullptr;
+    // the second CholeskyQR pass at every step / S formed on the device with an
+    // explicit CholeskyQR2 at every step (both taken by default only when needed)
+    const bool force_two_pass = getenv("BEM_BLOCK_CG_TWO_PASS") != nullptr;
+    const bool force_explicit = getenv("BEM_BLOCK_CG_EXPLICIT") != nullptr;
     // G[y] (Rb x Rb, host) = U_y^H V_y over the N rows for npair pairs: one
     // launch of the partials, one wait
     auto grams = [&](const GramPairs &gp, int npair, int Rb, Small *G) -> int {
@@ -705,18 +709,20 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
                                              std::conj(Ea[(size_t)i * Rb + j]) + aFa[(size_t)j * Rb + i];
             const double gterms = std::sqrt(fro2(Gs[3])) + 2 * std::sqrt(fro2(Ea)) + std::sqrt(fro2(aFa));
             Small L1;
-            const bool alg = chol_lower(G1, Rb, L1);
+            const bool alg = !force_explicit && chol_lower(G1, Rb, L1);
             total++;
             its++;
             double worst = 0;
             bool all = false;
+            bool one_pass_taken = false;
             if (alg) {
                 const Small L1i = lower_inv(L1, Rb), L1ih = herm(L1i, Rb);
                 // Q' = S L1^-H is orthonormal up to the rounding of the Gram
                 // terms times |L1^-1|^2 (FP64 sums of N products); one pass
                 // when that stays below 1e-9, else a second CholeskyQR pass on
                 // an explicit Gram product
-                const bool one_pass = 1e-14 * gterms * fro2(L1i) <= 1e-9;
+                const bool one_pass = !force_two_pass && 1e-14 * gterms * fro2(L1i) <= 1e-9;
+                one_pass_taken = one_pass;
                 const double2 *dc = nullptr;
                 if (one_pass) {
                     zeta = herm(L1, Rb);
@@ -777,7 +783,9 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
                 }
                 xi = xin;
             }
-            if (trace) fprintf(stderr, "block cg step %d: worst |res| / threshold %.3e\n", total, worst);
+            if (trace)
+                fprintf(stderr, "block cg step %d (%s): worst |res| / threshold %.3e\n", total,
+                        !alg ? "explicit QR" : one_pass_taken ? "one QR pass" : "two QR passes", worst);
             if (all) break;
         }
         kc_scatter_add<<<gb, 256, 0, s>>>(N, w.bX, ld, w.bidx, X, ldx);

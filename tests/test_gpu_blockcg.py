@@ -11,7 +11,10 @@ refinement.  Checks:
   * two equal right-hand sides (a rank-deficient block): the block cannot
     start, the solve falls back to the lockstep CG and still meets the bar;
   * below 4e8 of M N nrhs the method takes the lockstep iteration: the result
-    equals BEM_SOLVE_CGLS's bit for bit.
+    equals BEM_SOLVE_CGLS's bit for bit;
+  * the second CholeskyQR pass and the explicit CholeskyQR2 of S, which a step
+    takes only when the Gram terms leave orthogonality in doubt, forced at
+    every step: the same solution to 1e-9.
 These systems are small, so every test but the last sets BEM_BLOCK_CG_ALWAYS
 to take the block iteration.
 """
@@ -163,3 +166,21 @@ def test_block_cg_size_policy_small_system_takes_lockstep(monkeypatch):
     print("case: block CG size policy at s3: %s / %s" % (sb, sl), flush=True)
     assert sb["iterations"] == sl["iterations"]
     assert np.array_equal(xb, xl)
+
+
+@pytest.mark.parametrize("switch", ["BEM_BLOCK_CG_TWO_PASS", "BEM_BLOCK_CG_EXPLICIT"])
+def test_block_cg_qr_variants_agree(monkeypatch, switch):
+    m = _mesh(3, 8)
+    k = 31.3
+    d = C.sphere_points(23, 20, 1.0)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    sd = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+    xd = s.solution()
+    monkeypatch.setenv(switch, "1")
+    sv = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+    xv = s.solution()
+    rel = _relcols(xv, xd).max()
+    print("case: block CG with %s: %s / default %s, difference %.3g" % (switch, sv, sd, rel), flush=True)
+    assert sv["converged"] == 1 and rel < 1e-9
