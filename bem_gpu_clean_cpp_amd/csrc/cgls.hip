@@ -817,8 +817,17 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
             HOST_CALL(norms(beta));
             res_current = true;
         }
-        if (trace)
-            fprintf(stderr, "cgls cycle %d its %d rel[0] %.3e nb[0] %.3e\n", cyc, total, beta[0] / nb[0], nb[0]);
+        if (trace) {
+            double rmax = 0;
+            int above = 0;
+            for (int q = 0; q < R; q++) {
+                const double rq = nb[q] > 0 ? beta[q] / nb[q] : 0.0;
+                rmax = std::max(rmax, rq);
+                above += rq > outer_tol;
+            }
+            fprintf(stderr, "cgls cycle %d its %d rel[0] %.3e max rel %.3e (%d above the tolerance)\n", cyc, total,
+                    beta[0] / nb[0], rmax, above);
+        }
         bool any = false;
         std::vector<int> idx;
         std::vector<double> bthr;

@@ -83,7 +83,7 @@ __device__ __forceinline__ C2 keep2(bool k, C2 v) {
 // Several matrices at once (grid.z = matrix, astride apart): matrix z takes the
 // right-hand sides z rper .. z rper + rper - 1 (rg0 counts within them); a
 // single matrix has astride 0 and rper = R.
-template <typename T, int G>
+template <typename T, int G, bool G3>
 __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A, long lda, int M, int N,
                                                    const typename MF<T>::c2 *__restrict__ X, long ldx, int R, int rg0,
                                                    int kc, typename MF<T>::c2 *__restrict__ part, long ldp,
@@ -99,13 +99,12 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
     const int chunk = blockIdx.y;
     const int j0 = chunk * kc, j1 = min(N, j0 + kc);
     const int m = lane & 15, kk = lane >> 4;
-    v4 acc[G][2][4];
+    constexpr int NA = G3 ? 6 : 8;  // accumulators per group of 16 right-hand sides
+    v4 acc[G][NA];
 #pragma unroll
     for (int g = 0; g < G; g++)
 #pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int t = 0; t < 4; t++) acc[g][p][t] = (v4){0, 0, 0, 0};
+        for (int t = 0; t < NA; t++) acc[g][t] = (v4){0, 0, 0, 0};
     const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * m);
     const long lda4 = lda / 2;
     // right-hand sides past this matrix's last read that one (their results
@@ -144,12 +143,27 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
         for (int u = 0; u < kU; u++) {
             const T av[4] = {(T)B.a[u].x, (T)B.a[u].y, (T)B.a[u].z, (T)B.a[u].w};
 #pragma unroll
-            for (int g = 0; g < G; g++)
+            for (int g = 0; g < G; g++) {
+                const T xr = B.x[u][g].x, xi = B.x[u][g].y;
+                if constexpr (G3) {
+                    // acc[g][3 h + 0..2] = P1, P2, P3 of complex row 2 m' + h
+                    const T xs = xr + xi;
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    acc[g][0][t] = MF<T>::mma(av[t], B.x[u][g].x, acc[g][0][t]);
-                    acc[g][1][t] = MF<T>::mma(av[t], B.x[u][g].y, acc[g][1][t]);
+                    for (int h = 0; h < 2; h++) {
+                        const T ar = av[2 * h], ai = av[2 * h + 1];
+                        acc[g][3 * h] = MF<T>::mma(ar, xs, acc[g][3 * h]);
+                        acc[g][3 * h + 1] = MF<T>::mma(ar + ai, xi, acc[g][3 * h + 1]);
+                        acc[g][3 * h + 2] = MF<T>::mma(ai - ar, xr, acc[g][3 * h + 2]);
+                    }
+                } else {
+                    // acc[g][2 t + 0 / 1]: real row t of the pair against Re X / Im X
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        acc[g][2 * t] = MF<T>::mma(av[t], xr, acc[g][2 * t]);
+                        acc[g][2 * t + 1] = MF<T>::mma(av[t], xi, acc[g][2 * t + 1]);
+                    }
                 }
+            }
         }
     };
     Buf b0, b1;
@@ -167,8 +181,8 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
         load_masked(jb, b1);
         mul(b1);
     }
-    // D_t[m'][n'] (m' = drow(lane / 16, v)) is real row 4 m' + t of the tile:
-    // complex row 2 m' + t / 2, real part for even t
+    // result row m' = drow(lane / 16, v) of the MFMAs belongs to complex rows
+    // 2 m' and 2 m' + 1 of the tile (h = 0, 1)
 #pragma unroll
     for (int g = 0; g < G; g++) {
         const int n = rg0 + g * 16 + m;
@@ -180,8 +194,11 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int i = row0 + 2 * mp + h;
-                if (i < M) pc[i] = c2{acc[g][0][2 * h][v] - acc[g][1][2 * h + 1][v],
-                                      acc[g][0][2 * h + 1][v] + acc[g][1][2 * h][v]};
+                if (i >= M) continue;
+                if constexpr (G3)
+                    pc[i] = c2{acc[g][3 * h][v] - acc[g][3 * h + 1][v], acc[g][3 * h][v] + acc[g][3 * h + 2][v]};
+                else
+                    pc[i] = c2{acc[g][4 * h][v] - acc[g][4 * h + 3][v], acc[g][4 * h + 1][v] + acc[g][4 * h + 2][v]};
             }
         }
     }
@@ -191,7 +208,7 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
 // wave: 16 columns of A x (16 G) right-hand sides x one row chunk (i0 even).
 // Rows read in pairs: row + 1 may be M, which needs lda, ldt > M for odd M
 // (the callers' leading dimensions are M rounded up to 64).
-template <typename T, int G>
+template <typename T, int G, bool G3>
 __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A, long lda, int M, int N,
                                                    const typename MF<T>::c2 *__restrict__ Tv, long ldt, int R, int rg0,
                                                    int kr, typename MF<T>::c2 *__restrict__ part, long ldp) {
@@ -204,12 +221,12 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
     const int i0 = chunk * kr, i1 = min(M, i0 + kr);
     const int m = lane & 15, kk = lane >> 4;
     const int J = c0 + m;
-    v4 accR[G], accI[G];
+    constexpr int NA = G3 ? 3 : 2;  // accumulators per group of 16 right-hand sides
+    v4 acc[G][NA];
 #pragma unroll
-    for (int g = 0; g < G; g++) {
-        accR[g] = (v4){0, 0, 0, 0};
-        accI[g] = (v4){0, 0, 0, 0};
-    }
+    for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int t = 0; t < NA; t++) acc[g][t] = (v4){0, 0, 0, 0};
     // columns past N and right-hand sides past R read the last valid one
     // (their results are not written)
     const float2 *col = A + (size_t)min(J, N - 1) * lda;
@@ -253,13 +270,27 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
             const T av[4] = {(T)B.a[u].x, (T)B.a[u].y, (T)B.a[u].z, (T)B.a[u].w};
 #pragma unroll
             for (int g = 0; g < G; g++) {
-                // Re Z: sum Ar Tr + Ai Ti ; Im Z: sum Ar Ti - Ai Tr
-                const T br[4] = {B.t0[u][g].x, B.t0[u][g].y, B.t1[u][g].x, B.t1[u][g].y};
-                const T bi[4] = {B.t0[u][g].y, -B.t0[u][g].x, B.t1[u][g].y, -B.t1[u][g].x};
+                if constexpr (G3) {
+                    // conj(a) t in the three-product form, with a' = conj(a):
+                    // P1 = Re a' (Re t + Im t), P2 = (Re a' + Im a') Im t,
+                    // P3 = (Im a' - Re a') Re t; Z = (P1 - P2, P1 + P3)
+                    const T tr[2] = {B.t0[u][g].x, B.t1[u][g].x}, ti[2] = {B.t0[u][g].y, B.t1[u][g].y};
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    accR[g] = MF<T>::mma(av[t], br[t], accR[g]);
-                    accI[g] = MF<T>::mma(av[t], bi[t], accI[g]);
+                    for (int h = 0; h < 2; h++) {
+                        const T ar = av[2 * h], ai = av[2 * h + 1];
+                        acc[g][0] = MF<T>::mma(ar, tr[h] + ti[h], acc[g][0]);
+                        acc[g][1] = MF<T>::mma(ar - ai, ti[h], acc[g][1]);
+                        acc[g][2] = MF<T>::mma(-(ai + ar), tr[h], acc[g][2]);
+                    }
+                } else {
+                    // Re Z: sum Ar Tr + Ai Ti ; Im Z: sum Ar Ti - Ai Tr
+                    const T br[4] = {B.t0[u][g].x, B.t0[u][g].y, B.t1[u][g].x, B.t1[u][g].y};
+                    const T bi[4] = {B.t0[u][g].y, -B.t0[u][g].x, B.t1[u][g].y, -B.t1[u][g].x};
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        acc[g][0] = MF<T>::mma(av[t], br[t], acc[g][0]);
+                        acc[g][1] = MF<T>::mma(av[t], bi[t], acc[g][1]);
+                    }
                 }
             }
         }
@@ -287,7 +318,11 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int j = c0 + MF<T>::drow(kk, v);
-            if (j < N) pc[j] = c2{accR[g][v], accI[g][v]};
+            if (j >= N) continue;
+            if constexpr (G3)
+                pc[j] = c2{acc[g][0][v] - acc[g][1][v], acc[g][0][v] + acc[g][2][v]};
+            else
+                pc[j] = c2{acc[g][0][v], acc[g][1][v]};
         }
     }
 }
@@ -318,13 +353,27 @@ __device__ __forceinline__ int d32_row(int lane, int v) { return 8 * (v / 4) + 4
 
 // part[(chunk * R + n) * ldp + i] = sum_{j in chunk} A[i, j] X[j, n] for
 // n in [rg0, rg0 + 32): wave = 64 complex rows.  Lane (i = l % 32, k = l / 32)
-// loads complex rows row0 + 2i, row0 + 2i + 1 of column j + k (four real rows:
-// one A operand for each of four MFMAs) and X[j + k][rg0 + i] (the B operand,
-// real and imaginary part for two MFMAs).  MFMA t accumulates real row 4 i' + t
-// of the tile; Re/Im of Y combine lane-locally:
-//   Y[2i'] = (P0 - Q1, Q0 + P1), Y[2i' + 1] = (P2 - Q3, Q2 + P3)
-// with P_t against Re X, Q_t against Im X.  Needs lda >= roundup(M, 64).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesN, kWavesN))) void gemm_n_mfma32(
+// loads complex rows row0 + 2i, row0 + 2i + 1 of column j + k (four real rows)
+// and X[j + k][rg0 + i] (the B operand).  Two forms of the complex product:
+//
+// G3 = false, four real products per complex multiply-add (8 MFMAs per k
+// step): MFMA t accumulates real row 4 i' + t of the tile against Re X (P_t)
+// and Im X (Q_t), and
+//   Y[2i'] = (P0 - Q1, Q0 + P1), Y[2i' + 1] = (P2 - Q3, Q2 + P3).
+//
+// G3 = true, three real products (6 MFMAs per k step; the 3M method, Higham,
+// "Stability of a method for multiplying complex matrices with three real
+// matrix multiplications", SIAM J. Matrix Anal. Appl. 13 (1992)), in the form
+// where each part of the result is the sum of two products:
+//   P1 = Re A (Re X + Im X), P2 = (Re A + Im A) Im X, P3 = (Im A - Re A) Re X,
+//   Re Y = P1 - P2, Im Y = P1 + P3.
+// The sums are formed per lane from the loaded operands (3 adds per lane per k
+// step); the rounding error stays within a small multiple of the four-product
+// form's, normwise (|A| |X| scale).
+//
+// Needs lda >= roundup(M, 64).
+template <bool G3, int RING = kRingN, int WAVES = kWavesN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void gemm_n_mfma32(
     const float2 *__restrict__ A, long lda, int M, int N, const float2 *__restrict__ X, long ldx, int R, int rg0,
     int kc, float2 *__restrict__ part, long ldp) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -333,12 +382,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesN, kW
     const int chunk = blockIdx.y;
     const int j0 = chunk * kc, j1 = min(N, j0 + kc);
     const int i = lane & 31, k = lane >> 5;
-    v16f acc[4][2];
+    constexpr int NA = G3 ? 6 : 8;  // accumulators
+    v16f acc[NA];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        acc[t][0] = (v16f){0};
-        acc[t][1] = (v16f){0};
-    }
+    for (int t = 0; t < NA; t++) acc[t] = (v16f){0};
     const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * i);
     const long lda4 = lda / 2;
     // right-hand sides past R read column R - 1 (their results are not written)
@@ -360,21 +407,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesN, kW
         B.x = keep2(in, Xr[Jc]);
     };
     auto mul = [&](const Buf &B) {
-        const float av[4] = {B.a.x, B.a.y, B.a.z, B.a.w};
+        if constexpr (G3) {
+            // acc[3 h + 0..2] = P1, P2, P3 of complex row 2 i' + h
+            const float xs = B.x.x + B.x.y;
+            const float ar[2] = {B.a.x, B.a.z}, ai[2] = {B.a.y, B.a.w};
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            acc[t][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.x, acc[t][0], 0, 0, 0);
-            acc[t][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.y, acc[t][1], 0, 0, 0);
+            for (int h = 0; h < 2; h++) {
+                acc[3 * h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h], xs, acc[3 * h], 0, 0, 0);
+                acc[3 * h + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h] + ai[h], B.x.y, acc[3 * h + 1], 0, 0, 0);
+                acc[3 * h + 2] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[h] - ar[h], B.x.x, acc[3 * h + 2], 0, 0, 0);
+            }
+        } else {
+            // acc[2 t + 0 / 1] = P_t / Q_t
+            const float av[4] = {B.a.x, B.a.y, B.a.z, B.a.w};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                acc[2 * t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.x, acc[2 * t], 0, 0, 0);
+                acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.y, acc[2 * t + 1], 0, 0, 0);
+            }
         }
     };
-    Buf b[kRingN];
-    const int jend = j0 + (j1 - j0) / (2 * kRingN) * (2 * kRingN);  // end of the full rounds of the ring
+    Buf b[RING];
+    const int jend = j0 + (j1 - j0) / (2 * RING) * (2 * RING);  // end of the full rounds of the ring
 #pragma unroll
-    for (int w = 0; w < kRingN - 1; w++) load(j0 + 2 * w, b[w]);
-    for (int jb = j0; jb < jend; jb += 2 * kRingN) {
+    for (int w = 0; w < RING - 1; w++) load(j0 + 2 * w, b[w]);
+    for (int jb = j0; jb < jend; jb += 2 * RING) {
 #pragma unroll
-        for (int w = 0; w < kRingN; w++) {
-            load(jb + 2 * (w + kRingN - 1), b[(w + kRingN - 1) % kRingN]);
+        for (int w = 0; w < RING; w++) {
+            load(jb + 2 * (w + RING - 1), b[(w + RING - 1) % RING]);
             __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
             mul(b[w]);
         }
@@ -389,8 +449,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWavesN, kW
 #pragma unroll
     for (int v = 0; v < 16; v++) {
         const int r = row0 + 2 * d32_row(lane, v);
-        if (r < M) pc[r] = make_float2(acc[0][0][v] - acc[1][1][v], acc[0][1][v] + acc[1][0][v]);
-        if (r + 1 < M) pc[r + 1] = make_float2(acc[2][0][v] - acc[3][1][v], acc[2][1][v] + acc[3][0][v]);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            if (r + h >= M) continue;
+            if constexpr (G3)
+                pc[r + h] = make_float2(acc[3 * h][v] - acc[3 * h + 1][v], acc[3 * h][v] + acc[3 * h + 2][v]);
+            else
+                pc[r + h] = make_float2(acc[4 * h][v] - acc[4 * h + 3][v], acc[4 * h + 1][v] + acc[4 * h + 2][v]);
+        }
     }
 }
 
@@ -512,6 +578,26 @@ __global__ __launch_bounds__(256) void gemm_reduce(const typename MF<T>::c2 *__r
     }
 }
 
+// Chunk length for a product over n columns (rows, for A^H T) split into
+// `target` chunks, a multiple of `align`.  FP32 products also cap it at
+// kMaxAccum: the MFMA accumulators sum a chunk in FP32, and their rounding
+// grows with its length, while the chunk partials are summed in the
+// reduce's own precision (FP64 where a refinement follows).  At
+// M = N = 40962, R = 32 (block CG on the normal equations, refined to
+// 1e-10), three chunks of 13.7k columns left the second refinement cycle at
+// 1.3e-10 whatever its inner tolerance, so a third cycle was needed; 32
+// chunks reached 3.7e-11, 64 (640 columns) 1.2e-11 and 128 7.9e-12, and the
+// products ran as fast with 32 or 64 chunks as with 3.
+constexpr long kMaxAccum = 640;
+static int chunk_len(long n, long target, long align, bool fp32) {
+    long len = std::max(align, ((n + target - 1) / target + align - 1) / align * align);
+    if (fp32) len = std::min(len, kMaxAccum);  // a multiple of 32 and of 64
+    return (int)len;
+}
+static long chunk_target(long tiles, bool long_chunks) {
+    return long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
+}
+
 // column chunks: max(3, 1024 / row tiles).  Fewer, longer chunks than for
 // 4096 workgroups: measured Y = A X at R = 16, M = 2562: 25.3 us with 41 chunks,
 // 17.3 with 12; at R = 32, M = 10258: 364 us with 12, 252 with 3; at
@@ -519,35 +605,27 @@ __global__ __launch_bounds__(256) void gemm_reduce(const typename MF<T>::c2 *__r
 // (long_chunks: for products whose rounding a refinement absorbs; the short
 // rule, chunks for 4096 workgroups, keeps FP32 sums short where the product
 // is final, as in the QR updates and the field operator)
-static void gemm_n_shape(long M, long N, int &nch, int &kc, bool long_chunks) {
-    const long tiles = (M + 31) / 32;
-    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
-    kc = (int)std::max(32L, ((N + target - 1) / target + 31) / 32 * 32);
+static void gemm_n_shape(long M, long N, int &nch, int &kc, bool long_chunks, bool fp32) {
+    kc = chunk_len(N, chunk_target((M + 31) / 32, long_chunks), 32, fp32);
     nch = (int)((N + kc - 1) / kc);
 }
 
 // row chunks: max(3, 1024 / column tiles), as for gemm_n (measured Z = A^H T at
 // R = 32: M = 10258, 334 -> 321 us; M = 40994, 5.81 -> 5.29 ms against chunks
 // for 4096 workgroups)
-static void gemm_h_shape(long M, long N, int &nch, int &kr, bool long_chunks) {
-    const long tiles = (N + 15) / 16;
-    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
-    kr = (int)std::max(64L, ((M + target - 1) / target + 63) / 64 * 64);
+static void gemm_h_shape(long M, long N, int &nch, int &kr, bool long_chunks, bool fp32) {
+    kr = chunk_len(M, chunk_target((N + 15) / 16, long_chunks), 64, fp32);
     nch = (int)((M + kr - 1) / kr);
 }
 
 // the same rules for the 32-right-hand-side kernels, whose waves cover 64
-// rows (Y = A X) or 32 columns (Z = A^H T)
+// rows (Y = A X) or 32 columns (Z = A^H T); they are FP32 only
 static void gemm_n_shape32(long M, long N, int &nch, int &kc, bool long_chunks) {
-    const long tiles = (M + 63) / 64;
-    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
-    kc = (int)std::max(32L, ((N + target - 1) / target + 31) / 32 * 32);
+    kc = chunk_len(N, chunk_target((M + 63) / 64, long_chunks), 32, true);
     nch = (int)((N + kc - 1) / kc);
 }
 static void gemm_h_shape32(long M, long N, int &nch, int &kr, bool long_chunks) {
-    const long tiles = (N + 31) / 32;
-    long target = long_chunks ? std::max(3L, 1024 / std::max(1L, tiles)) : std::max(1L, 4096 / std::max(1L, tiles));
-    kr = (int)std::max(64L, ((M + target - 1) / target + 63) / 64 * 64);
+    kr = chunk_len(M, chunk_target((N + 31) / 32, long_chunks), 64, true);
     nch = (int)((M + kr - 1) / kr);
 }
 
@@ -561,7 +639,8 @@ static bool use32(int R) { return R > 16 && !getenv("BEM_GEMM_16X16"); }
 // chunks, chosen by timing every candidate on the device
 // (bem_autotune_products).  Classes: the product, M and N in half-octave
 // buckets, R in steps of 16.  Chunk counts stay <= kMaxTunedChunks, which the
-// workspace sizes allow for.
+// workspace sizes allow for; a tuned shape still keeps chunks within
+// kMaxAccum columns (more chunks than tuned where N is large).
 constexpr int kMaxTunedChunks = 16;
 namespace {
 struct PShape {
@@ -590,7 +669,7 @@ bool tuned_product(int which, long M, long N, int R, PShape &out) {
 // partials of the FP32 adjoint product alone (Z = A^H T, N x R)
 size_t ws_gemm_h_f32(long M, long N, int R) {
     int nch, kr, n32, k32;
-    gemm_h_shape(M, N, nch, kr, false);
+    gemm_h_shape(M, N, nch, kr, false, true);
     gemm_h_shape32(M, N, n32, k32, false);
     return (size_t)std::max({nch, n32, kMaxTunedChunks}) * (size_t)((N + 63) / 64 * 64) * (size_t)std::max(1, R) *
            sizeof(float2);
@@ -602,17 +681,23 @@ size_t ws_gemm(long M, long N, int R) {
     int nch = 0;
     for (bool lc : {false, true}) {
         int a, b, c, d, e, f, g, h, u, v, x, y;
-        gemm_n_shape(M, N, a, b, lc);
-        gemm_h_shape(M, N, c, d, lc);
+        gemm_n_shape(M, N, a, b, lc, true);
+        gemm_h_shape(M, N, c, d, lc, true);
         gemm_n_shape32(M, N, e, f, lc);
         gemm_h_shape32(M, N, g, h, lc);
-        gemm_n_shape(N, M, u, v, lc);
+        gemm_n_shape(N, M, u, v, lc, true);
         gemm_n_shape32(N, M, x, y, lc);
         nch = std::max({nch, a, c, e, g, u, x});
     }
     nch = std::max(nch, kMaxTunedChunks);
     const long ldp = (std::max(M, N) + 63) / 64 * 64;
     return (size_t)nch * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
+}
+
+// complex products in the three-product form (BEM_GEMM_4M=1: four products)
+static bool use_3m() {
+    static const bool on = !getenv("BEM_GEMM_4M");
+    return on;
 }
 
 // nmat > 1: nmat matrices astride elements apart, matrix z applied to the
@@ -633,12 +718,12 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
     if (w32 && lda < (M + 63) / 64 * 64) BEM_FAIL("gemm_n needs lda >= roundup(M, 64) for more than 16 right-hand sides");
     int nch, kc;
     if (tuned) {
-        kc = (int)std::max(32L, ((N + ps.chunks - 1) / ps.chunks + 31) / 32 * 32);
+        kc = chunk_len(N, ps.chunks, 32, true);
         nch = (int)((N + kc - 1) / kc);
     } else if (w32) {
         gemm_n_shape32(M, N, nch, kc, long_chunks);
     } else {
-        gemm_n_shape(M, N, nch, kc, long_chunks);
+        gemm_n_shape(M, N, nch, kc, long_chunks, sizeof(T) == 4);
     }
     const long ldp = (M + 63) / 64 * 64;
     auto *part = (typename MF<T>::c2 *)ws;
@@ -647,16 +732,28 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
     const unsigned gx = (unsigned)((M + 127) / 128);
     for (int rg = 0; rg < rper; rg += 32) {
         if (w32) {
-            gemm_n_mfma32<<<dim3((unsigned)((M + 255) / 256), nch), 256, 0, s>>>(
-                A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc, (float2 *)part, ldp);
+            const dim3 grid((unsigned)((M + 255) / 256), nch);
+            if (use_3m())
+                gemm_n_mfma32<true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc,
+                                                         (float2 *)part, ldp);
+            else
+                gemm_n_mfma32<false><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc,
+                                                          (float2 *)part, ldp);
         } else {
             dim3 grid(gx, nch, nmat);
-            if (rper - rg > 16)
-                gemm_n_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp, astride,
-                                                       rper);
+            const bool two = rper - rg > 16;
+            if (use_3m() && two)
+                gemm_n_mfma<T, 2, true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp,
+                                                             astride, rper);
+            else if (use_3m())
+                gemm_n_mfma<T, 1, true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp,
+                                                             astride, rper);
+            else if (two)
+                gemm_n_mfma<T, 2, false><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp,
+                                                              astride, rper);
             else
-                gemm_n_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp, astride,
-                                                       rper);
+                gemm_n_mfma<T, 1, false><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp,
+                                                              astride, rper);
         }
         HIP_CALL(hipGetLastError());
     }
@@ -677,12 +774,12 @@ static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename
     const bool w32 = sizeof(T) == 4 && (tuned ? ps.wide != 0 : use32(R));
     int nch, kr;
     if (tuned) {
-        kr = (int)std::max(64L, ((M + ps.chunks - 1) / ps.chunks + 63) / 64 * 64);
+        kr = chunk_len(M, ps.chunks, 64, true);
         nch = (int)((M + kr - 1) / kr);
     } else if (w32) {
         gemm_h_shape32(M, N, nch, kr, long_chunks);
     } else {
-        gemm_h_shape(M, N, nch, kr, long_chunks);
+        gemm_h_shape(M, N, nch, kr, long_chunks, sizeof(T) == 4);
     }
     const long ldp = (N + 63) / 64 * 64;
     auto *part = (typename MF<T>::c2 *)ws;
@@ -695,10 +792,15 @@ static int gemm_h_impl(const float2 *A, long lda, long M, long N, const typename
                 A, lda, (int)M, (int)N, (const float2 *)Tv, ldt, R, rg, kr, (float2 *)part, ldp);
         } else {
             dim3 grid(gx, nch);
-            if (R - rg > 16)
-                gemm_h_mfma<T, 2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+            const bool two = R - rg > 16;
+            if (use_3m() && two)
+                gemm_h_mfma<T, 2, true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+            else if (use_3m())
+                gemm_h_mfma<T, 1, true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+            else if (two)
+                gemm_h_mfma<T, 2, false><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
             else
-                gemm_h_mfma<T, 1><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
+                gemm_h_mfma<T, 1, false><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, Tv, ldt, R, rg, kr, part, ldp);
         }
         HIP_CALL(hipGetLastError());
     }
