@@ -321,6 +321,9 @@ int mres_f64(const double2 *A, long lda, long M, long N, const double2 *X, long 
              double2 *out, long ldo, int R, void *ws, hipStream_t s);
 int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
             void *ws, hipStream_t s);
+// mh_f32a through the conjugate-transposed copy AH (adjoint_copy_make)
+int mh_f32a_hc(const float2 *AH, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R, void *ws,
+               hipStream_t s);
 int mh_f64(const double2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
            hipStream_t s);
 

@@ -470,6 +470,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         }
         if (f32) {
             if (R == 1) HOST_CALL(gemv_h_f32a(A32, lda, M, N, w.rr, 0, w.s64, 0, 1, s));
+            else if (w.AH) HOST_CALL(mh_f32a_hc(w.AH, M, N, w.rr, ld, w.s64, ld, R, w.gws, s));
             else HOST_CALL(mh_f32a(A32, lda, M, N, w.rr, ld, w.s64, ld, R, w.gws, s));
         } else {
             if (R == 1) HOST_CALL(gemv_h_f64(A64, lda, M, N, w.rr, 0, w.s64, 0, 1, s));

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "bem_internal.hpp"
 
@@ -333,10 +334,19 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
 // Against the 16x16x4 kernels a wave covers twice the rows (or columns) for
 // the same operand loads, so the right-hand-side block is read half as often
 // per flop, and each lane holds its share of the 32 x 32 output in 16
-// registers per accumulator.  The operands are prefetched one k step at a
-// time into a ring of kRing buffers (each buffer: one 16-byte load of A and
+// registers per accumulator.  The operands are prefetched into a ring of
+// buffers (each buffer: one or two k steps, one 16-byte load of A per step and
 // one load of the right-hand-side block per lane), so several steps of loads
 // are in flight while the oldest is multiplied.
+//
+// The right-hand-side block X is column-major, so one X load touches 32 cache
+// lines (one per right-hand side) against 8 for the 1 KB of A.  At
+// M = N = 40962, R = 32 the texture addresser was busy 69 % of the cycles and
+// the MFMA pipes 62 % (three-product form, one k step per buffer); pairing
+// the steps so that each X load serves two of them took the product from
+// 3.80 to 3.49 ms.  A row-major X, whose loads are 512 contiguous bytes, had
+// measured slower (4.43 against 4.27 ms, four-product form): every wave then
+// reads the same X row at the same time.
 //
 // Output layout of v_mfma_f32_32x32x2f32 (checked on the device with an
 // in-bounds one-hot probe): lane l, register v holds D[8 (v / 4) + 4 (l / 32)
@@ -344,6 +354,7 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
 // j + 32 k.
 typedef float v16f __attribute__((ext_vector_type(16)));
 constexpr int kRingN = 4;   // Y = A X: buffers in the ring
+constexpr int kRingPX = 2;  // (buffers of two k steps, paired X loads)
 constexpr int kWavesN = 3;  // waves per SIMD it is built for
 constexpr int kRingH = 2;   // Z = A^H T
 constexpr int kStepsH = 4;  // k steps per buffer of the adjoint product
@@ -372,7 +383,7 @@ __device__ __forceinline__ int d32_row(int lane, int v) { return 8 * (v / 4) + 4
 // form's, normwise (|A| |X| scale).
 //
 // Needs lda >= roundup(M, 64).
-template <bool G3, int RING = kRingN, int WAVES = kWavesN>
+template <bool G3, bool PX, int RING = kRingN, int WAVES = kWavesN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void gemm_n_mfma32(
     const float2 *__restrict__ A, long lda, int M, int N, const float2 *__restrict__ X, long ldx, int R, int rg0,
     int kc, float2 *__restrict__ part, long ldp) {
@@ -390,56 +401,87 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVE
     const long lda4 = lda / 2;
     // right-hand sides past R read column R - 1 (their results are not written)
     const float2 *Xr = X + (size_t)min(rg0 + i, R - 1) * ldx;
+    // PX: a buffer holds two k steps, lane k taking columns jb + 2k and
+    // jb + 2k + 1, so its X values for both are one 16-byte load (half the X
+    // load instructions, each touching the same 32 cache lines)
+    constexpr int S = PX ? 2 : 1;  // k steps per buffer
     struct Buf {
-        float4 a;
-        float2 x;
+        float4 a[S];
+        typename std::conditional<PX, float4, float2>::type x;
     };
+    const int jx_last = (j1 - 1) & ~1;  // last even column of the chunk
     auto load = [&](int jb, Buf &B) {
-        const int Jc = min(jb + k, j1 - 1);
-        B.a = Ab[(size_t)Jc * lda4];
-        B.x = Xr[Jc];
+        if constexpr (PX) {
+#pragma unroll
+            for (int t = 0; t < 2; t++) B.a[t] = Ab[(size_t)min(jb + 2 * k + t, j1 - 1) * lda4];
+            B.x = *reinterpret_cast<const float4 *>(Xr + min(jb + 2 * k, jx_last));
+        } else {
+            const int Jc = min(jb + k, j1 - 1);
+            B.a[0] = Ab[(size_t)Jc * lda4];
+            B.x = Xr[Jc];
+        }
     };
     auto load_masked = [&](int jb, Buf &B) {
-        const int J = jb + k;
-        const bool in = J < j1;
-        const int Jc = min(J, j1 - 1);
-        B.a = keep4(in, in, Ab[(size_t)Jc * lda4]);
-        B.x = keep2(in, Xr[Jc]);
+        if constexpr (PX) {
+            bool in[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int J = jb + 2 * k + t;
+                in[t] = J < j1;
+                B.a[t] = keep4(in[t], in[t], Ab[(size_t)min(J, j1 - 1) * lda4]);
+            }
+            B.x = keep4(in[0], in[1], *reinterpret_cast<const float4 *>(Xr + min(jb + 2 * k, jx_last)));
+        } else {
+            const int J = jb + k;
+            const bool in = J < j1;
+            const int Jc = min(J, j1 - 1);
+            B.a[0] = keep4(in, in, Ab[(size_t)Jc * lda4]);
+            B.x = keep2(in, Xr[Jc]);
+        }
     };
-    auto mul = [&](const Buf &B) {
+    auto mul1 = [&](const float4 a, const float2 x) {
         if constexpr (G3) {
             // acc[3 h + 0..2] = P1, P2, P3 of complex row 2 i' + h
-            const float xs = B.x.x + B.x.y;
-            const float ar[2] = {B.a.x, B.a.z}, ai[2] = {B.a.y, B.a.w};
+            const float xs = x.x + x.y;
+            const float ar[2] = {a.x, a.z}, ai[2] = {a.y, a.w};
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 acc[3 * h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h], xs, acc[3 * h], 0, 0, 0);
-                acc[3 * h + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h] + ai[h], B.x.y, acc[3 * h + 1], 0, 0, 0);
-                acc[3 * h + 2] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[h] - ar[h], B.x.x, acc[3 * h + 2], 0, 0, 0);
+                acc[3 * h + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h] + ai[h], x.y, acc[3 * h + 1], 0, 0, 0);
+                acc[3 * h + 2] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[h] - ar[h], x.x, acc[3 * h + 2], 0, 0, 0);
             }
         } else {
             // acc[2 t + 0 / 1] = P_t / Q_t
-            const float av[4] = {B.a.x, B.a.y, B.a.z, B.a.w};
+            const float av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                acc[2 * t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.x, acc[2 * t], 0, 0, 0);
-                acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], B.x.y, acc[2 * t + 1], 0, 0, 0);
+                acc[2 * t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], x.x, acc[2 * t], 0, 0, 0);
+                acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], x.y, acc[2 * t + 1], 0, 0, 0);
             }
         }
     };
+    auto mul = [&](const Buf &B) {
+        if constexpr (PX) {
+            mul1(B.a[0], make_float2(B.x.x, B.x.y));
+            mul1(B.a[1], make_float2(B.x.z, B.x.w));
+        } else {
+            mul1(B.a[0], B.x);
+        }
+    };
+    constexpr int W = 2 * S;  // columns per buffer
     Buf b[RING];
-    const int jend = j0 + (j1 - j0) / (2 * RING) * (2 * RING);  // end of the full rounds of the ring
+    const int jend = j0 + (j1 - j0) / (W * RING) * (W * RING);  // end of the full rounds of the ring
 #pragma unroll
-    for (int w = 0; w < RING - 1; w++) load(j0 + 2 * w, b[w]);
-    for (int jb = j0; jb < jend; jb += 2 * RING) {
+    for (int w = 0; w < RING - 1; w++) load(j0 + W * w, b[w]);
+    for (int jb = j0; jb < jend; jb += W * RING) {
 #pragma unroll
         for (int w = 0; w < RING; w++) {
-            load(jb + 2 * (w + RING - 1), b[(w + RING - 1) % RING]);
+            load(jb + W * (w + RING - 1), b[(w + RING - 1) % RING]);
             __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
             mul(b[w]);
         }
     }
-    for (int jb = jend; jb < j1; jb += 2) {
+    for (int jb = jend; jb < j1; jb += W) {
         load_masked(jb, b[0]);
         mul(b[0]);
     }
@@ -694,11 +736,9 @@ size_t ws_gemm(long M, long N, int R) {
     return (size_t)nch * (size_t)ldp * (size_t)std::max(1, R) * sizeof(double2);
 }
 
-// complex products in the three-product form (BEM_GEMM_4M=1: four products)
-static bool use_3m() {
-    static const bool on = !getenv("BEM_GEMM_4M");
-    return on;
-}
+// complex products in the three-product form (BEM_GEMM_4M=1: four products;
+// read at every launch, so a process can compare both)
+static bool use_3m() { return !getenv("BEM_GEMM_4M"); }
 
 // nmat > 1: nmat matrices astride elements apart, matrix z applied to the
 // right-hand sides z R / nmat .. (z + 1) R / nmat - 1 (16 x 16 x 4 kernels)
@@ -733,12 +773,15 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
     for (int rg = 0; rg < rper; rg += 32) {
         if (w32) {
             const dim3 grid((unsigned)((M + 255) / 256), nch);
-            if (use_3m())
-                gemm_n_mfma32<true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc,
-                                                         (float2 *)part, ldp);
-            else
-                gemm_n_mfma32<false><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc,
-                                                          (float2 *)part, ldp);
+            // paired X loads read X[(n) ldx + j .. j + 1] as one 16-byte load
+            const bool px = ldx % 2 == 0 && ((uintptr_t)X & 15) == 0 && (N % 2 == 0 || ldx > N);
+            auto go = [&](auto kern) {
+                kern<<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc, (float2 *)part,
+                                          ldp);
+            };
+            // (an X that cannot be read in pairs takes the four-product form)
+            if (use_3m() && px) go(gemm_n_mfma32<true, true, kRingPX>);
+            else go(gemm_n_mfma32<false, false>);
         } else {
             dim3 grid(gx, nch, nmat);
             const bool two = rper - rg > 16;
@@ -942,6 +985,14 @@ int mres_f32a(const float2 *A, long lda, long M, long N, const double2 *X, long 
 int mh_f32a(const float2 *A, long lda, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R,
             void *ws, hipStream_t s) {
     return gemm_h_impl<double>(A, lda, M, N, T, ldt, Z, ldz, R, ws, "mh_f64acc", s, true);
+}
+// the same through the conjugate-transposed copy, on the Y = A X kernel
+// (M = N = 40962, R = 32: 6.1 ms against 7.6 ms for the A^H kernel, which
+// holds one wave per SIMD at FP64)
+int mh_f32a_hc(const float2 *AH, long M, long N, const double2 *T, long ldt, double2 *Z, long ldz, int R, void *ws,
+               hipStream_t s) {
+    return gemm_n_impl<double>(AH, adjoint_copy_ld(N), N, M, T, ldt, (const double2 *)nullptr, 0, Z, ldz, R, ws,
+                               "mh_f64acc", s, true);
 }
 
 // ------------------------------------------------------ tuner driver ---

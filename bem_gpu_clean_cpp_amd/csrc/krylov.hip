@@ -467,6 +467,7 @@ struct Operator {
                                       hipMemcpyDeviceToDevice, s));
         } else if (A32) {
             if (R == 1) HOST_CALL(gemv_h_f32a(A32, lda, M, N, B, 0, r, 0, 1, s));
+            else if (AH) HOST_CALL(mh_f32a_hc(AH, M, N, B, ldb, r, ws.ldv, R, ws.gws, s));
             else HOST_CALL(mh_f32a(A32, lda, M, N, B, ldb, r, ws.ldv, R, ws.gws, s));
         } else {
             if (R == 1) HOST_CALL(gemv_h_f64(A64, lda, M, N, B, 0, r, 0, 1, s));
@@ -496,7 +497,8 @@ struct Operator {
                 HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, r, ws.ldv, R, ws.gws, s));
             } else {
                 HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, ws.t64, ws.ldv, R, ws.gws, s));
-                HOST_CALL(mh_f32a(A32, lda, M, N, ws.t64, ws.ldv, r, ws.ldv, R, ws.gws, s));
+                if (AH) HOST_CALL(mh_f32a_hc(AH, M, N, ws.t64, ws.ldv, r, ws.ldv, R, ws.gws, s));
+                else HOST_CALL(mh_f32a(A32, lda, M, N, ws.t64, ws.ldv, r, ws.ldv, R, ws.gws, s));
             }
         } else if (R == 1) {
             if (!normal()) {

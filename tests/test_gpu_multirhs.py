@@ -271,3 +271,37 @@ def test_adjoint_copy_matches_adjoint_kernel(monkeypatch, method):
     assert st["converged"] == 1 and st2["converged"] == 1
     assert rel < 1e-8
     assert err < 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nrhs,method", [(24, bem.SOLVE_BLOCK_CG), (24, bem.SOLVE_GMRES), (8, bem.SOLVE_GMRES)])
+def test_three_and_four_product_forms_agree(monkeypatch, nrhs, method):
+    """The complex products in the three-product form (default) against the
+    four-product form (BEM_GEMM_4M): 24 right-hand sides take the 32 x 32
+    kernels (the Krylov product with paired X loads, the adjoint through the
+    conjugate-transposed copy, the FP64 residuals), 8 the 16 x 16 ones.  On an
+    s3 sphere with 7 CHIEF rows (M = 649, ragged tiles), after FP64
+    refinement to 1e-10 the two solutions agree to 1e-8, and the default one
+    matches the oracle's least-squares solution at 1e-5."""
+    m = _mesh(3, 7)
+    k = 27.1
+    d = _dirs(nrhs, seed=23)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    kw = dict(restart=30, tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=50)
+    monkeypatch.delenv("BEM_GEMM_4M", raising=False)
+    if method == bem.SOLVE_BLOCK_CG:
+        monkeypatch.setenv("BEM_BLOCK_CG_ALWAYS", "1")
+    st3 = s.solve(method, **kw)
+    x3 = s.solution()
+    monkeypatch.setenv("BEM_GEMM_4M", "1")
+    st4 = s.solve(method, **kw)
+    x4 = s.solution()
+    rel = _relcols(x3, x4).max()
+    err = _relcols(x3, _oracle_lstsq(m, k, d)).max()
+    print("case: product forms, %d rhs, method %d: stats %s / %s, 3M vs 4M %.3g, vs oracle %.3g" %
+          (nrhs, method, st3, st4, rel, err))
+    assert st3["converged"] == 1 and st4["converged"] == 1
+    assert rel < 1e-8
+    assert err < 1e-5
