@@ -40,6 +40,14 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector and dense MFMA)
 PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 (vector and dense MFMA)
+
+
+def executed_share():
+    """Share of the algorithmic complex-product flops (8 per complex
+    multiply-add) that the MFMA pipes execute: the products' three-product
+    form takes 6 real MFMA flops per complex multiply-add, the four-product
+    form (BEM_GEMM_4M) all 8."""
+    return 1.0 if os.environ.get("BEM_GEMM_4M") else 0.75
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 FLOP_PER_EVAL = 80.0       # nominal FP32 flops per (row, quadrature point) evaluation
 # measured VALU issue rate of the FP32 assembly kernel (wave-instructions per
@@ -277,6 +285,9 @@ def bench_batching(bem, C, workers, fp64=True, method=1, tune=False):
                           "achieved": round(flops / (g_avg * 1e-3) / 1e12, 3), "peak": PEAK_FP32_TFLOPS,
                           "unit": "TFLOP/s", "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                           "hbm_gbs": round(gbytes / (g_avg * 1e-3) / 1e9, 1), "flops_per_launch": flops,
+                          "executed_flops_per_launch": flops * executed_share(),
+                          "executed_frac": round(flops * executed_share() / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                                                 4),
                           "bytes_per_launch": gbytes},
     }
 
@@ -430,6 +441,8 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
                      "frac": round(flops / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                      "traffic": measured_traffic("gemm_n_f32", "large"), "avg_ms": round(g_avg, 3),
                      "launches": t["gemm_n_f32"][1], "flops_per_launch": flops, "bytes_per_launch": gbytes,
+                     "executed_flops_per_launch": flops * executed_share(),
+                     "executed_frac": round(flops * executed_share() / (g_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                      "hbm_gbs": round(8.0 * M * N / (g_avg * 1e-3) / 1e9, 1),
                      "adjoint": {"kernel": hname, "avg_ms": round(h_avg, 3),
                                  "achieved": round(flops / (h_avg * 1e-3) / 1e12, 2),
@@ -442,7 +455,10 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
                                          "achieved": round(flops / (avg["mres_f64acc"] * 1e-3) / 1e12, 2),
                                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                                          "frac": round(flops / (avg["mres_f64acc"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                                                       4)}
+                                                       4),
+                                         "executed_frac": round(flops * executed_share() /
+                                                                (avg["mres_f64acc"] * 1e-3) / 1e12 /
+                                                                PEAK_FP64_TFLOPS, 4)}
     if fp64:
         # 1x yardstick: plain single-RHS FP64 solve (FP64 assembly + FP64 CG),
         # also the reference solution for the first two sources
