@@ -163,6 +163,9 @@ _SIGS = {
     "bem_solver_release": (_I, [_P]),
     "bem_batch_solve": (_I, [_P, _P, _I, _I, _I, _P, _I, _I, _F, ctypes.POINTER(bem_solve_params), _I, _P,
                              _I, _P]),
+    "bem_batch_create": (_P, [_P, _I]),
+    "bem_batch_run": (_I, [_P, _P, _I, _I, _I, _P, _I, _I, _F, ctypes.POINTER(bem_solve_params), _P, _I, _P]),
+    "bem_batch_destroy": (None, [_P]),
     "bem_server_create": (_P, [_I, _D]),
     "bem_server_submit": (ctypes.c_long, [_P, _P, _F, _I, _I, _P, _I, _F, ctypes.POINTER(bem_solve_params)]),
     "bem_server_wait": (_I, [_P, ctypes.c_long, _P, _I, ctypes.POINTER(bem_request_info)]),
@@ -566,6 +569,45 @@ def batch_solve(mesh, ks, kind, params, amp=1.0, precision=PREC_FP32, workers=4,
     _check(lib().bem_batch_solve(mesh.handle, _ptr(k), nk, precision, kind, _ptr(p), R, per_freq, amp,
                                  ctypes.byref(sp), workers, _ptr(X), N, st), "batch_solve")
     return X.transpose(2, 1, 0).copy(), [x.as_dict() for x in st]
+
+
+class Batch:
+    """Batch context (bem_batch): `workers` resident solvers of one mesh kept
+    between runs; run() takes the arguments of batch_solve and gives the same
+    results."""
+
+    def __init__(self, mesh, workers=4):
+        self._h = lib().bem_batch_create(mesh.handle, workers)
+        if not self._h:
+            raise BemError("bem_batch_create failed: %s" % last_error())
+        self.num_pnts = mesh.num_pnts
+
+    def run(self, ks, kind, params, amp=1.0, precision=PREC_FP32, **solve_kw):
+        k = np.ascontiguousarray(ks, np.float32).ravel()
+        nk = k.size
+        p = np.ascontiguousarray(params, np.float32)
+        per_freq = 1 if p.ndim == 3 else 0
+        if per_freq and p.shape[0] != nk:
+            raise BemError("params must be (nrhs, 3) or (nk, nrhs, 3)")
+        R = p.shape[-2]
+        N = self.num_pnts
+        X = np.zeros((nk, R, N), np.complex128)
+        st = (bem_solve_stats * nk)()
+        sp = solve_params(**solve_kw)
+        _check(lib().bem_batch_run(self._h, _ptr(k), nk, precision, kind, _ptr(p), R, per_freq, amp,
+                                   ctypes.byref(sp), _ptr(X), N, st), "batch_run")
+        return X.transpose(2, 1, 0).copy(), [x.as_dict() for x in st]
+
+    def close(self):
+        if self._h:
+            lib().bem_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def request_footprint(M, N, E, nrhs, precision=PREC_FP32, method=SOLVE_GMRES, restart=30):

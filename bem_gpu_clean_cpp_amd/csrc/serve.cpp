@@ -25,6 +25,7 @@
 #include <condition_variable>
 #include <deque>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -72,17 +73,18 @@ int run_one(bem_solver *sv, float k, int precision, int kind, const float *param
 // against 0.142 s for one frequency per worker (a group runs as many steps as
 // its slowest member, and each worker's host loop rotates 4x the systems).
 constexpr int kFuseMax = 4;
+constexpr int kMaxBatchWorkers = 8;
 
-int fuse_width(const bem_mesh *m, int precision, int nrhs, const bem_solve_params &p, int workers) {
+int fuse_width(const Mesh &m, int precision, int nrhs, const bem_solve_params &p, int workers) {
     const char *env = getenv("BEM_BATCH_FUSE");
     const int want = env ? std::max(1, std::min(kFuseMax, atoi(env))) : 1;
-    if (want < 2 || precision != BEM_PREC_FP32 || p.method != BEM_SOLVE_GMRES || nrhs < 2 || m->m.C() > 0) return 1;
+    if (want < 2 || precision != BEM_PREC_FP32 || p.method != BEM_SOLVE_GMRES || nrhs < 2 || m.C() > 0) return 1;
     int fuse = std::max(1, std::min(want, kMaxRhs / nrhs));
     // every worker holds fuse matrices: keep that within half the free HBM
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1;
-    const size_t ld = ((size_t)m->m.N() + 63) / 64 * 64;
-    while (fuse > 1 && (size_t)fuse * workers * sizeof(float2) * ld * (size_t)m->m.N() > free_b / 2) fuse--;
+    const size_t ld = ((size_t)m.N() + 63) / 64 * 64;
+    while (fuse > 1 && (size_t)fuse * workers * sizeof(float2) * ld * (size_t)m.N() > free_b / 2) fuse--;
     return fuse;
 }
 
@@ -198,28 +200,60 @@ double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int m
 }
 
 // ---------------------------------------------------------- batch solve ---
-int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, int rhs_kind, const float *params,
-                    int nrhs, int params_per_freq, float amp, const bem_solve_params *sp, int workers,
-                    bem_zcomplex *X, int ldx, bem_solve_stats *stats) {
-    if (!m || !k || nk <= 0 || !params || nrhs <= 0) BEM_FAIL("bad batch arguments");
-    if (X && ldx < (int)m->m.N()) BEM_FAIL("ldx must be >= N");
-    const bem_solve_params p = sp ? *sp : default_params();
-    workers = std::max(1, std::min(std::min(workers, nk), 8));
-    std::vector<bem_solver *> sv(workers, nullptr);
-    for (int w = 0; w < workers; w++) {
-        sv[w] = bem_solver_create(m);
-        if (!sv[w]) {
-            const std::string e = last_error();
-            for (auto *s : sv) bem_solver_destroy(s);
-            BEM_FAIL("batch: solver creation failed: %s", e.c_str());
-        }
+// A batch context: `workers` resident solvers of one mesh, kept between runs,
+// so each run finds the mesh's device records, the matrices and the solvers'
+// workspaces allocated (at the s4 batching size, creating 4 solvers took
+// 45 ms, their first solves 12.6 ms against 4.9 ms warm, and destroying them
+// 18 ms: a quarter of a 64-frequency run).  One run at a time.
+struct bem_batch {
+    std::vector<bem_solver *> sv;
+    std::mutex mu;
+    ~bem_batch() {
+        for (auto *s : sv) bem_solver_destroy(s);
     }
+};
+
+bem_batch *bem_batch_create(const bem_mesh *m, int workers) {
+    if (!m) {
+        set_error(__FILE__, __LINE__, "null mesh");
+        return nullptr;
+    }
+    if (workers < 1 || workers > kMaxBatchWorkers) {
+        set_error(__FILE__, __LINE__, "batch: %d workers (1 to %d)", workers, kMaxBatchWorkers);
+        return nullptr;
+    }
+    auto *b = new bem_batch();
+    for (int w = 0; w < workers; w++) {
+        bem_solver *s = bem_solver_create(m);
+        if (!s) {
+            const std::string e = last_error();
+            delete b;
+            set_error(__FILE__, __LINE__, "batch: solver creation failed: %s", e.c_str());
+            return nullptr;
+        }
+        b->sv.push_back(s);
+    }
+    return b;
+}
+
+void bem_batch_destroy(bem_batch *b) { delete b; }
+
+int bem_batch_run(bem_batch *b, const float *k, int nk, int precision, int rhs_kind, const float *params, int nrhs,
+                  int params_per_freq, float amp, const bem_solve_params *sp, bem_zcomplex *X, int ldx,
+                  bem_solve_stats *stats) {
+    if (!b || b->sv.empty() || !k || nk <= 0 || !params || nrhs <= 0) BEM_FAIL("bad batch arguments");
+    std::lock_guard<std::mutex> lk(b->mu);
+    const Mesh &mesh = b->sv[0]->mesh;
+    if (X && ldx < (int)mesh.N()) BEM_FAIL("ldx must be >= N");
+    const bem_solve_params p = sp ? *sp : default_params();
+    const int workers = std::min((int)b->sv.size(), nk);
+    const std::vector<bem_solver *> &sv = b->sv;
     // highest wavenumbers first: their solves take the most iterations, so
     // starting them first keeps the workers' finishing times close
     std::vector<int> order(nk);
     for (int f = 0; f < nk; f++) order[f] = f;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return k[a] > k[b]; });
-    const int fuse = fuse_width(m, precision, nrhs, p, workers);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return k[a] > k[c]; });
+    const int fuse = fuse_width(mesh, precision, nrhs, p, workers);
     std::atomic<int> next{0};
     std::atomic<int> failed{0};
     std::string first_error;
@@ -234,7 +268,7 @@ int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, in
                 const int nf = std::min(fuse, nk - q0);
                 if (run_fused(sv[w], fw, k, order.data() + q0, nf, rhs_kind, params, params_per_freq, nrhs, amp, p, X,
                               ldx, stats) != EXIT_SUCCESS) {
-                    std::lock_guard<std::mutex> lk(emu);
+                    std::lock_guard<std::mutex> lk2(emu);
                     if (!failed.exchange(1)) first_error = last_error();
                     break;
                 }
@@ -249,7 +283,7 @@ int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, in
             bem_solve_stats st{};
             const float *pf = params_per_freq ? params + (size_t)f * 3 * nrhs : params;
             if (run_one(sv[w], k[f], precision, rhs_kind, pf, nrhs, amp, p, xf, ldx, &st) != EXIT_SUCCESS) {
-                std::lock_guard<std::mutex> lk(emu);
+                std::lock_guard<std::mutex> lk2(emu);
                 if (!failed.exchange(1)) first_error = last_error();
                 break;
             }
@@ -260,9 +294,19 @@ int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, in
     for (int w = 1; w < workers; w++) th.emplace_back(work, w);
     work(0);
     for (auto &t : th) t.join();
-    for (auto *s : sv) bem_solver_destroy(s);
     if (failed.load()) BEM_FAIL("batch: %s", first_error.c_str());
     return EXIT_SUCCESS;
+}
+
+int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, int rhs_kind, const float *params,
+                    int nrhs, int params_per_freq, float amp, const bem_solve_params *sp, int workers,
+                    bem_zcomplex *X, int ldx, bem_solve_stats *stats) {
+    if (!m || !k || nk <= 0 || !params || nrhs <= 0) BEM_FAIL("bad batch arguments");
+    if (X && ldx < (int)m->m.N()) BEM_FAIL("ldx must be >= N");
+    workers = std::max(1, std::min(std::min(workers, nk), kMaxBatchWorkers));
+    std::unique_ptr<bem_batch> b(bem_batch_create(m, workers));
+    if (!b) return EXIT_FAILURE;
+    return bem_batch_run(b.get(), k, nk, precision, rhs_kind, params, nrhs, params_per_freq, amp, sp, X, ldx, stats);
 }
 
 }  // extern "C"

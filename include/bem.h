@@ -311,6 +311,21 @@ int bem_batch_solve(const bem_mesh *m, const float *k, int nk, int precision, in
                     int nrhs, int params_per_freq, float amp, const bem_solve_params *sp, int workers,
                     bem_zcomplex *X, int ldx, bem_solve_stats *stats);
 
+/* A batch context: the `workers` (1..8) resident solvers of bem_batch_solve,
+ * created once and kept, so repeated batches of one mesh skip the solvers'
+ * set-up (the mesh's device records, the matrices, the solve workspaces).
+ * The mesh is copied.  bem_batch_run takes the arguments of bem_batch_solve
+ * and gives the same results; min(workers, nk) solvers take part; one run at
+ * a time per context (a second caller waits).  bem_batch_solve is
+ * bem_batch_create + bem_batch_run + bem_batch_destroy.  Extension: the
+ * reference solves frequencies one at a time in its driver loop. */
+typedef struct bem_batch bem_batch;
+bem_batch *bem_batch_create(const bem_mesh *m, int workers);
+int bem_batch_run(bem_batch *b, const float *k, int nk, int precision, int rhs_kind, const float *params, int nrhs,
+                  int params_per_freq, float amp, const bem_solve_params *sp, bem_zcomplex *X, int ldx,
+                  bem_solve_stats *stats);
+void bem_batch_destroy(bem_batch *b);
+
 /* Request server: `slots` resident solvers (<= 16) on their own streams take
  * queued requests in arrival order as soon as a slot is free and the request's
  * HBM footprint fits in the budget left by the running ones

@@ -4,6 +4,8 @@ Checks:
   * a multi-frequency batch on several concurrent workers returns exactly the
     bits one solver returns running the frequencies one by one, and matches
     the FP64 oracle at the 1e-5 bar;
+  * a batch context kept over several runs returns what bem_batch_solve
+    returns for each;
   * a resident solver re-pointed at another mesh returns exactly what a fresh
     solver returns;
   * every request solved in a mixed stream on 4 concurrent slots returns
@@ -69,6 +71,30 @@ def test_batch_equals_sequential_and_oracle():
         xo = np.linalg.lstsq(A, B, rcond=None)[0]
         err = np.linalg.norm(X[:, :, f] - xo, axis=0) / np.linalg.norm(xo, axis=0)
         assert err.max() < 1e-5, (f, err)
+
+
+def test_batch_context_reused_equals_batch_solve():
+    """A batch context (bem_batch_create / run / destroy) kept over three runs,
+    with different frequencies, right-hand sides and frequency counts (fewer
+    than its workers in the last), returns exactly the bits bem_batch_solve
+    returns for each."""
+    m = _ellipsoid(3, 11)
+    freqs = 150.0 * 40.0 ** (np.arange(7) / 6.0)
+    ks = (2 * np.pi * freqs / SPEED).astype(np.float32)
+    d1 = C.sphere_points(31, 5, 1.0)
+    d2 = np.stack([C.sphere_points(40 + f, 3, 1.0) for f in range(4)]).astype(np.float32)
+    kw = dict(refine_tol=1e-9)
+    b = bem.Batch(m, workers=3)
+    try:
+        runs = [(ks, d1), (ks[2:6][::-1].copy(), d2), (ks[:2], d1)]
+        for rk, rd in runs:
+            Xb, sb = b.run(rk, bem.RHS_PLANE, rd, **kw)
+            Xs, ss = bem.batch_solve(m, rk, bem.RHS_PLANE, rd, workers=3, **kw)
+            assert all(s["converged"] == 1 for s in sb), sb
+            assert np.array_equal(Xb, Xs)
+            assert [s["iterations"] for s in sb] == [s["iterations"] for s in ss]
+    finally:
+        b.close()
 
 
 def test_solver_set_mesh_matches_fresh_solver():

@@ -124,6 +124,11 @@ def test_reference_entry_points_reject_bad_arguments():
     fails(L.bem_solver_get_solution(None, P(B), 42), "null argument")
     fails(L.bem_batch_solve(m.handle, None, 0, bem.PREC_FP32, bem.RHS_POINT, None, 1, 0, 1.0, None, 1, None, 42,
                             None), "bad batch arguments")
+    assert not L.bem_batch_create(None, 2) and "null mesh" in bem.last_error()
+    assert not L.bem_batch_create(m.handle, 0) and "workers" in bem.last_error()
+    fails(L.bem_batch_run(None, None, 0, bem.PREC_FP32, bem.RHS_POINT, None, 1, 0, 1.0, None, None, 42, None),
+          "bad batch arguments")
+    L.bem_batch_destroy(None)
     fails(L.bem_mesh_get_bb(m.handle, P(np.zeros(6, np.float32))), "bounding box not computed")
     fails(L.bem_mesh_set_chief(m.handle, None, 2), "bad CHIEF arguments")
 
