@@ -383,21 +383,28 @@ __device__ __forceinline__ int d32_row(int lane, int v) { return 8 * (v / 4) + 4
 // form's, normwise (|A| |X| scale).
 //
 // Needs lda >= roundup(M, 64).
-template <bool G3, bool PX, int RING = kRingN, int WAVES = kWavesN>
+template <bool G3, bool PX, int RING = kRingN, int WAVES = kWavesN, int RT = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void gemm_n_mfma32(
     const float2 *__restrict__ A, long lda, int M, int N, const float2 *__restrict__ X, long ldx, int R, int rg0,
     int kc, float2 *__restrict__ part, long ldp) {
+    // RT row tiles of 64 complex rows per wave share every X load
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = (blockIdx.x * 4 + wave) * 64;
+    const int row0 = (blockIdx.x * 4 + wave) * 64 * RT;
     if (row0 >= M) return;
     const int chunk = blockIdx.y;
     const int j0 = chunk * kc, j1 = min(N, j0 + kc);
     const int i = lane & 31, k = lane >> 5;
-    constexpr int NA = G3 ? 6 : 8;  // accumulators
-    v16f acc[NA];
+    constexpr int NA = G3 ? 6 : 8;  // accumulators per row tile
+    v16f acc[RT][NA];
 #pragma unroll
-    for (int t = 0; t < NA; t++) acc[t] = (v16f){0};
-    const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * i);
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+        for (int t = 0; t < NA; t++) acc[r][t] = (v16f){0};
+    // row tiles past M read the last tile's rows (their results are not written)
+    const float4 *Ab[RT];
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+        Ab[r] = reinterpret_cast<const float4 *>(A + min(row0 + 64 * r, (M - 1) / 64 * 64) + 2 * i);
     const long lda4 = lda / 2;
     // right-hand sides past R read column R - 1 (their results are not written)
     const float2 *Xr = X + (size_t)min(rg0 + i, R - 1) * ldx;
@@ -406,18 +413,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVE
     // load instructions, each touching the same 32 cache lines)
     constexpr int S = PX ? 2 : 1;  // k steps per buffer
     struct Buf {
-        float4 a[S];
+        float4 a[S][RT];
         typename std::conditional<PX, float4, float2>::type x;
     };
     const int jx_last = (j1 - 1) & ~1;  // last even column of the chunk
     auto load = [&](int jb, Buf &B) {
         if constexpr (PX) {
 #pragma unroll
-            for (int t = 0; t < 2; t++) B.a[t] = Ab[(size_t)min(jb + 2 * k + t, j1 - 1) * lda4];
+            for (int t = 0; t < 2; t++) {
+                const size_t o = (size_t)min(jb + 2 * k + t, j1 - 1) * lda4;
+#pragma unroll
+                for (int r = 0; r < RT; r++) B.a[t][r] = Ab[r][o];
+            }
             B.x = *reinterpret_cast<const float4 *>(Xr + min(jb + 2 * k, jx_last));
         } else {
             const int Jc = min(jb + k, j1 - 1);
-            B.a[0] = Ab[(size_t)Jc * lda4];
+#pragma unroll
+            for (int r = 0; r < RT; r++) B.a[0][r] = Ab[r][(size_t)Jc * lda4];
             B.x = Xr[Jc];
         }
     };
@@ -428,44 +440,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVE
             for (int t = 0; t < 2; t++) {
                 const int J = jb + 2 * k + t;
                 in[t] = J < j1;
-                B.a[t] = keep4(in[t], in[t], Ab[(size_t)min(J, j1 - 1) * lda4]);
+                const size_t o = (size_t)min(J, j1 - 1) * lda4;
+#pragma unroll
+                for (int r = 0; r < RT; r++) B.a[t][r] = keep4(in[t], in[t], Ab[r][o]);
             }
             B.x = keep4(in[0], in[1], *reinterpret_cast<const float4 *>(Xr + min(jb + 2 * k, jx_last)));
         } else {
             const int J = jb + k;
             const bool in = J < j1;
             const int Jc = min(J, j1 - 1);
-            B.a[0] = keep4(in, in, Ab[(size_t)Jc * lda4]);
+#pragma unroll
+            for (int r = 0; r < RT; r++) B.a[0][r] = keep4(in, in, Ab[r][(size_t)Jc * lda4]);
             B.x = keep2(in, Xr[Jc]);
         }
     };
-    auto mul1 = [&](const float4 a, const float2 x) {
+    auto mul1 = [&](v16f *ac, const float4 a, const float2 x) {
         if constexpr (G3) {
-            // acc[3 h + 0..2] = P1, P2, P3 of complex row 2 i' + h
+            // ac[3 h + 0..2] = P1, P2, P3 of complex row 2 i' + h
             const float xs = x.x + x.y;
             const float ar[2] = {a.x, a.z}, ai[2] = {a.y, a.w};
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                acc[3 * h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h], xs, acc[3 * h], 0, 0, 0);
-                acc[3 * h + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h] + ai[h], x.y, acc[3 * h + 1], 0, 0, 0);
-                acc[3 * h + 2] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[h] - ar[h], x.x, acc[3 * h + 2], 0, 0, 0);
+                ac[3 * h] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h], xs, ac[3 * h], 0, 0, 0);
+                ac[3 * h + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar[h] + ai[h], x.y, ac[3 * h + 1], 0, 0, 0);
+                ac[3 * h + 2] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai[h] - ar[h], x.x, ac[3 * h + 2], 0, 0, 0);
             }
         } else {
-            // acc[2 t + 0 / 1] = P_t / Q_t
+            // ac[2 t + 0 / 1] = P_t / Q_t
             const float av[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                acc[2 * t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], x.x, acc[2 * t], 0, 0, 0);
-                acc[2 * t + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], x.y, acc[2 * t + 1], 0, 0, 0);
+                ac[2 * t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], x.x, ac[2 * t], 0, 0, 0);
+                ac[2 * t + 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[t], x.y, ac[2 * t + 1], 0, 0, 0);
             }
         }
     };
     auto mul = [&](const Buf &B) {
-        if constexpr (PX) {
-            mul1(B.a[0], make_float2(B.x.x, B.x.y));
-            mul1(B.a[1], make_float2(B.x.z, B.x.w));
-        } else {
-            mul1(B.a[0], B.x);
+#pragma unroll
+        for (int r = 0; r < RT; r++) {
+            if constexpr (PX) {
+                mul1(acc[r], B.a[0][r], make_float2(B.x.x, B.x.y));
+                mul1(acc[r], B.a[1][r], make_float2(B.x.z, B.x.w));
+            } else {
+                mul1(acc[r], B.a[0][r], B.x);
+            }
         }
     };
     constexpr int W = 2 * S;  // columns per buffer
@@ -489,15 +507,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVE
     if (n >= R) return;
     float2 *pc = part + ((size_t)chunk * R + n) * ldp;
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int r = row0 + 2 * d32_row(lane, v);
+    for (int rt = 0; rt < RT; rt++) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            if (r + h >= M) continue;
-            if constexpr (G3)
-                pc[r + h] = make_float2(acc[3 * h][v] - acc[3 * h + 1][v], acc[3 * h][v] + acc[3 * h + 2][v]);
-            else
-                pc[r + h] = make_float2(acc[4 * h][v] - acc[4 * h + 3][v], acc[4 * h + 1][v] + acc[4 * h + 2][v]);
+        for (int v = 0; v < 16; v++) {
+            const int r = row0 + 64 * rt + 2 * d32_row(lane, v);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                if (r + h >= M) continue;
+                if constexpr (G3)
+                    pc[r + h] = make_float2(acc[rt][3 * h][v] - acc[rt][3 * h + 1][v],
+                                            acc[rt][3 * h][v] + acc[rt][3 * h + 2][v]);
+                else
+                    pc[r + h] = make_float2(acc[rt][4 * h][v] - acc[rt][4 * h + 3][v],
+                                            acc[rt][4 * h + 1][v] + acc[rt][4 * h + 2][v]);
+            }
         }
     }
 }
@@ -740,6 +763,17 @@ size_t ws_gemm(long M, long N, int R) {
 // read at every launch, so a process can compare both)
 static bool use_3m() { return !getenv("BEM_GEMM_4M"); }
 
+// The 32-column Y = A X kernel with two 64-row tiles per wave: every X load
+// then feeds 12 MFMAs instead of 6 (246 VGPRs, 2 waves per SIMD).  At
+// M = N = 40962, R = 32, 64 chunks: 3.34 ms against 3.44 ms, the same bits
+// (each element's sum runs in the same order).  It halves the waves, so it
+// is taken only while the launch keeps at least 1024 workgroups.
+// BEM_GEMM_ROW_TILES = 1 or 2 forces either form (read at every launch).
+static bool two_row_tiles(long M, int nch) {
+    if (const char *e = getenv("BEM_GEMM_ROW_TILES")) return atoi(e) == 2;
+    return (M + 511) / 512 * (long)nch >= 1024;
+}
+
 // nmat > 1: nmat matrices astride elements apart, matrix z applied to the
 // right-hand sides z R / nmat .. (z + 1) R / nmat - 1 (16 x 16 x 4 kernels)
 template <typename T, typename S = T, typename OC2 = typename MF<T>::c2>
@@ -780,8 +814,15 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
                                           ldp);
             };
             // (an X that cannot be read in pairs takes the four-product form)
-            if (use_3m() && px) go(gemm_n_mfma32<true, true, kRingPX>);
-            else go(gemm_n_mfma32<false, false>);
+            if (use_3m() && px && two_row_tiles(M, nch)) {
+                const dim3 g2((unsigned)((M + 511) / 512), nch);
+                gemm_n_mfma32<true, true, kRingPX, 2, 2><<<g2, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X,
+                                                                           ldx, R, rg, kc, (float2 *)part, ldp);
+            } else if (use_3m() && px) {
+                go(gemm_n_mfma32<true, true, kRingPX>);
+            } else {
+                go(gemm_n_mfma32<false, false>);
+            }
         } else {
             dim3 grid(gx, nch, nmat);
             const bool two = rper - rg > 16;

@@ -305,3 +305,29 @@ def test_three_and_four_product_forms_agree(monkeypatch, nrhs, method):
     assert st3["converged"] == 1 and st4["converged"] == 1
     assert rel < 1e-8
     assert err < 1e-5
+
+
+@pytest.mark.gpu
+def test_product_row_tiles_same_bits(monkeypatch):
+    """The 32-column product with two 64-row tiles per wave (taken for large
+    launches) against one tile per wave (BEM_GEMM_ROW_TILES forces either):
+    every element's sum runs in the same order, so a block CG solve with 24
+    right-hand sides on an s3 sphere with 7 CHIEF rows (ragged row tiles)
+    returns the same bits."""
+    m = _mesh(3, 7)
+    k = 27.1
+    d = _dirs(24, seed=29)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_PLANE, d, 1.0)
+    monkeypatch.setenv("BEM_BLOCK_CG_ALWAYS", "1")
+    kw = dict(tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=50)
+    out = {}
+    for tiles in ("1", "2"):
+        monkeypatch.setenv("BEM_GEMM_ROW_TILES", tiles)
+        st = s.solve(bem.SOLVE_BLOCK_CG, **kw)
+        assert st["converged"] == 1, st
+        out[tiles] = (s.solution(), st["iterations"])
+    print("case: row tiles, iterations %d / %d" % (out["1"][1], out["2"][1]))
+    assert out["1"][1] == out["2"][1]
+    assert np.array_equal(out["1"][0], out["2"][0])
