@@ -229,6 +229,81 @@ __global__ __launch_bounds__(256) void kc_gram_part(GramPairs gp, long ld, int n
     }
 }
 
+// The same partials on MFMA: part[y][blk][a + Rb b] over kGramRowsM rows per
+// workgroup, FP64 v_mfma_f64_16x16x4f64 with the operands read straight from
+// the block vectors.  The Rb x Rb result is cut into 16 x 16 tiles; wave w
+// takes tiles w, w + 4, .. (TPW of them: 1 while Rb <= 32, else 4).  Lane
+// (m, kk) supplies A[m][kk] = U[r + kk, 16 ta + m] and B[kk][m] =
+// V[r + kk, 16 tb + m] for rows r, r + 4, ..; conj(u) v in the three-product
+// form, with u' = conj(u):
+//   P1 = Re u' (Re v + Im v), P2 = (Re u' + Im u') Im v, P3 = (Im u' - Re u') Re v,
+//   Re = P1 - P2, Im = P1 + P3.
+// Result layout: D[l / 16 + 4 v][l % 16] in lane l, register v.  Rows past n
+// and columns past Rb read clamped addresses and are zeroed.
+constexpr int kGramRowsM = 128;
+template <int TPW, int KB>
+__global__ __launch_bounds__(256) void kc_gram_mfma(GramPairs gp, long ld, int n, int Rb, double2 *__restrict__ part) {
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m = lane & 15, kk = lane >> 4;
+    const int nt = (Rb + 15) / 16, T = nt * nt, RR = Rb * Rb;
+    const double2 *__restrict__ U = gp.U[blockIdx.y];
+    const double2 *__restrict__ V = gp.V[blockIdx.y];
+    part += ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * RR;
+    const int i0 = blockIdx.x * kGramRowsM;
+    v4d acc[TPW][3];
+    const double2 *pu[TPW], *pv[TPW];
+    bool ua[TPW], vb[TPW];
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[q][c] = (v4d){0, 0, 0, 0};
+        const int t = min(wave + 4 * q, T - 1);
+        const int ca = 16 * (t % nt) + m, cb = 16 * (t / nt) + m;
+        ua[q] = ca < Rb;
+        vb[q] = cb < Rb;
+        pu[q] = U + (size_t)min(ca, Rb - 1) * ld;
+        pv[q] = V + (size_t)min(cb, Rb - 1) * ld;
+    }
+    for (int r = i0; r < i0 + kGramRowsM; r += 4 * KB) {
+        double2 u[KB][TPW], v[KB][TPW];
+#pragma unroll
+        for (int b = 0; b < KB; b++) {
+            const int i = r + 4 * b + kk, ic = min(i, n - 1);
+#pragma unroll
+            for (int q = 0; q < TPW; q++) {
+                u[b][q] = pu[q][ic];
+                v[b][q] = pv[q][ic];
+                if (i >= n || !ua[q]) u[b][q] = make_double2(0.0, 0.0);
+                if (i >= n || !vb[q]) v[b][q] = make_double2(0.0, 0.0);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < KB; b++)
+#pragma unroll
+            for (int q = 0; q < TPW; q++) {
+                const double ur = u[b][q].x, ui = -u[b][q].y, vr = v[b][q].x, vi = v[b][q].y;
+                acc[q][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur, vr + vi, acc[q][0], 0, 0, 0);
+                acc[q][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ur + ui, vi, acc[q][1], 0, 0, 0);
+                acc[q][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(ui - ur, vr, acc[q][2], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < TPW; q++) {
+        const int t = wave + 4 * q;
+        if (t >= T) continue;
+        const int a0 = 16 * (t % nt), b = 16 * (t / nt) + m;
+        if (b >= Rb) continue;
+#pragma unroll
+        for (int vv = 0; vv < 4; vv++) {
+            const int a = a0 + kk + 4 * vv;
+            if (a < Rb)
+                part[a + (size_t)Rb * b] =
+                    make_double2(acc[q][0][vv] - acc[q][1][vv], acc[q][0][vv] + acc[q][2][vv]);
+        }
+    }
+}
+
 // seg[y][g][e] = sum of part[y][blk][e] over the g-th run of blocks, in order
 __global__ void kc_gram_sum1(const double2 *__restrict__ part, int nblk, int RR, double2 *__restrict__ seg) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
@@ -550,10 +625,21 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     const bool force_explicit = getenv("BEM_BLOCK_CG_EXPLICIT") != nullptr;
     // G[y] (Rb x Rb, host) = U_y^H V_y over the N rows for npair pairs: one
     // launch of the partials, one wait
+    const bool gram_valu = getenv("BEM_GRAM_VALU") != nullptr;
+    const int gblk_m = (N + kGramRowsM - 1) / kGramRowsM;
     auto grams = [&](const GramPairs &gp, int npair, int Rb, Small *G) -> int {
         const int RR = Rb * Rb;
-        kc_gram_part<<<dim3(gblk, npair), 256, 0, s>>>(gp, ld, N, Rb, w.gpart);
-        kc_gram_sum1<<<dim3((RR + 63) / 64, kGramSegs, npair), 64, 0, s>>>(w.gpart, gblk, RR, w.gseg);
+        int nb = gblk;
+        if (gram_valu) {
+            kc_gram_part<<<dim3(gblk, npair), 256, 0, s>>>(gp, ld, N, Rb, w.gpart);
+        } else if (Rb <= 32) {
+            nb = gblk_m;
+            kc_gram_mfma<1, 4><<<dim3(gblk_m, npair), 256, 0, s>>>(gp, ld, N, Rb, w.gpart);
+        } else {
+            nb = gblk_m;
+            kc_gram_mfma<4, 2><<<dim3(gblk_m, npair), 256, 0, s>>>(gp, ld, N, Rb, w.gpart);
+        }
+        kc_gram_sum1<<<dim3((RR + 63) / 64, kGramSegs, npair), 64, 0, s>>>(w.gpart, nb, RR, w.gseg);
         kc_gram_sum2<<<dim3((RR + 255) / 256, npair), 256, 0, s>>>(w.gseg, RR, w.gsum);
         HIP_CALL(hipGetLastError());
         HIP_CALL(hipMemcpyAsync(w.hgram, w.gsum, sizeof(double2) * RR * npair, hipMemcpyDeviceToHost, s));

@@ -168,11 +168,16 @@ def test_block_cg_size_policy_small_system_takes_lockstep(monkeypatch):
     assert np.array_equal(xb, xl)
 
 
-@pytest.mark.parametrize("switch", ["BEM_BLOCK_CG_TWO_PASS", "BEM_BLOCK_CG_EXPLICIT"])
-def test_block_cg_qr_variants_agree(monkeypatch, switch):
+@pytest.mark.parametrize("switch,nrhs", [("BEM_BLOCK_CG_TWO_PASS", 20), ("BEM_BLOCK_CG_EXPLICIT", 20),
+                                         ("BEM_GRAM_VALU", 20), ("BEM_GRAM_VALU", 40)])
+def test_block_cg_qr_variants_agree(monkeypatch, switch, nrhs):
+    """The second CholeskyQR pass at every step, the explicit QR at every step,
+    and the Gram products on the VALU kernel instead of MFMA (20 right-hand
+    sides: one 16 x 16 tile per wave; 40: four) give the default solve's
+    solution to 1e-9 after refinement."""
     m = _mesh(3, 8)
     k = 31.3
-    d = C.sphere_points(23, 20, 1.0)
+    d = C.sphere_points(23, nrhs, 1.0)
     s = bem.Solver(m)
     s.assemble(k, bem.PREC_FP32)
     s.set_rhs(bem.RHS_PLANE, d, 1.0)
