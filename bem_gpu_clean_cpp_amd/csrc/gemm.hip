@@ -84,11 +84,11 @@ __device__ __forceinline__ C2 keep2(bool k, C2 v) {
 // Several matrices at once (grid.z = matrix, astride apart): matrix z takes the
 // right-hand sides z rper .. z rper + rper - 1 (rg0 counts within them); a
 // single matrix has astride 0 and rper = R.
-template <typename T, int G, bool G3>
-__global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A, long lda, int M, int N,
-                                                   const typename MF<T>::c2 *__restrict__ X, long ldx, int R, int rg0,
-                                                   int kc, typename MF<T>::c2 *__restrict__ part, long ldp,
-                                                   long astride, int rper) {
+// (KU: MFMA steps per prefetched block; WV: waves per SIMD it is built for)
+template <typename T, int G, bool G3, int KU = kU, int WV = 1>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WV))) void gemm_n_mfma(
+    const float2 *__restrict__ A, long lda, int M, int N, const typename MF<T>::c2 *__restrict__ X, long ldx, int R,
+    int rg0, int kc, typename MF<T>::c2 *__restrict__ part, long ldp, long astride, int rper) {
     typedef typename MF<T>::v4 v4;
     typedef typename MF<T>::c2 c2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -114,14 +114,14 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
 #pragma unroll
     for (int g = 0; g < G; g++) Xg[g] = X + (size_t)min(rg0 + g * 16 + m, rend - 1) * ldx;
     struct Buf {
-        float4 a[kU];
-        c2 x[kU][G];
+        float4 a[KU];
+        c2 x[KU][G];
     };
     // full blocks: every column inside the chunk, no masking; the address is
     // clamped only for the prefetch past the last pair of blocks
     auto load = [&](int jb, Buf &B) {
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
+        for (int u = 0; u < KU; u++) {
             const int Jc = min(jb + 4 * u + kk, j1 - 1);
             B.a[u] = Ab[(size_t)Jc * lda4];
 #pragma unroll
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
     };
     auto load_masked = [&](int jb, Buf &B) {
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
+        for (int u = 0; u < KU; u++) {
             const int J = jb + 4 * u + kk;
             const bool in = J < j1;
             const int Jc = min(J, j1 - 1);
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
     };
     auto mul = [&](const Buf &B) {
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
+        for (int u = 0; u < KU; u++) {
             const T av[4] = {(T)B.a[u].x, (T)B.a[u].y, (T)B.a[u].z, (T)B.a[u].w};
 #pragma unroll
             for (int g = 0; g < G; g++) {
@@ -168,17 +168,17 @@ __global__ __launch_bounds__(256) void gemm_n_mfma(const float2 *__restrict__ A,
         }
     };
     Buf b0, b1;
-    const int jpair = j0 + (j1 - j0) / (8 * kU) * (8 * kU);  // end of the paired full blocks
+    const int jpair = j0 + (j1 - j0) / (8 * KU) * (8 * KU);  // end of the paired full blocks
     load(j0, b0);
-    for (int jb = j0; jb < jpair; jb += 8 * kU) {
-        load(jb + 4 * kU, b1);
+    for (int jb = j0; jb < jpair; jb += 8 * KU) {
+        load(jb + 4 * KU, b1);
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
         mul(b0);
-        load(jb + 8 * kU, b0);
+        load(jb + 8 * KU, b0);
         __builtin_amdgcn_sched_barrier(0);
         mul(b1);
     }
-    for (int jb = jpair; jb < j1; jb += 4 * kU) {
+    for (int jb = jpair; jb < j1; jb += 4 * KU) {
         load_masked(jb, b1);
         mul(b1);
     }
@@ -826,9 +826,13 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
         } else {
             dim3 grid(gx, nch, nmat);
             const bool two = rper - rg > 16;
+            // FP64, 32 right-hand sides: 2 MFMA steps per prefetched block
+            // leave 3 waves per SIMD (168 VGPRs) where 4 steps held one (210
+            // VGPRs + 96 AGPRs): M = N = 40962, 6.46-6.50 -> 5.70-5.74 ms
+            constexpr int ku2 = sizeof(T) == 8 ? 2 : kU, wv2 = sizeof(T) == 8 ? 2 : 1;
             if (use_3m() && two)
-                gemm_n_mfma<T, 2, true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp,
-                                                             astride, rper);
+                gemm_n_mfma<T, 2, true, ku2, wv2><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc,
+                                                                       part, ldp, astride, rper);
             else if (use_3m())
                 gemm_n_mfma<T, 1, true><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, X, ldx, R, rg, kc, part, ldp,
                                                              astride, rper);
