@@ -363,9 +363,10 @@ constexpr int kWavesH = 2;
 __device__ __forceinline__ int d32_row(int lane, int v) { return 8 * (v / 4) + 4 * (lane >> 5) + (v & 3); }
 
 // part[(chunk * R + n) * ldp + i] = sum_{j in chunk} A[i, j] X[j, n] for
-// n in [rg0, rg0 + 32): wave = 64 complex rows.  Lane (i = l % 32, k = l / 32)
-// loads complex rows row0 + 2i, row0 + 2i + 1 of column j + k (four real rows)
-// and X[j + k][rg0 + i] (the B operand).  Two forms of the complex product:
+// n in [rg0, rg0 + 32): wave = RT tiles of 64 complex rows.  Lane
+// (i = l % 32, k = l / 32) loads complex rows 2i, 2i + 1 of each tile in column
+// j + k (four real rows per tile) and X[j + k][rg0 + i] (the B operand, one
+// load for all RT tiles).  Two forms of the complex product:
 //
 // G3 = false, four real products per complex multiply-add (8 MFMAs per k
 // step): MFMA t accumulates real row 4 i' + t of the tile against Re X (P_t)
