@@ -1428,6 +1428,10 @@ bool tuned_shape(int M, int E, TileShape &out) {
 }
 }  // namespace
 
+// halo share (elements integrated per element) from which the element-once
+// plan is taken
+constexpr double kGhostMinDup = 1.05;
+
 // side columns the element-once plan may use: about the nodes on the tile
 // boundaries (4 sqrt(N P) + P; 4190 for N = 10242 in 102 tiles, which need
 // 3110), and never more than N / 2
@@ -1652,19 +1656,21 @@ int device_problem_build(const Mesh &m, DeviceProblem &dp, hipStream_t s) {
                                 hipMemcpyHostToDevice, s));
     dp.qg = qg;
     // element-once plan for the FP32 fast kernel: taken when the halo plan
-    // integrates at least 10 % of the elements twice and the side columns stay
-    // within ghost_slot_bound (device_problem_bytes counts that bound)
+    // integrates at least 5 % of the elements twice and the side columns stay
+    // within ghost_slot_bound (device_problem_bytes counts that bound).  It
+    // runs at twice the waves as well, so it pays below 10 %: the 40962-node
+    // ellipsoid (52 tiles, 7.9 % twice) assembles in 82.9 ms against 91.0 ms.
     dp.use_ghost = 0;
     // Too many side columns for the bound: fewer tiles (two thirds each try;
     // the 2562-node perturbed ellipsoid starts at 320 tiles of 8 nodes, and at
     // 64 tiles assembles in 0.48 ms against 0.70 ms on the halo plan).  A halo
-    // under 10 % is left as it is.
+    // under 5 % is left as it is.
     GhostPlan gp;
     int gP = choose_tiles(dp.M, dp.N, dp.E, 64, 2);
     bool take = false;
     for (int tries = 0; qg == 1 && !getenv("BEM_NO_GHOST_PLAN") && tries < 8 && gP >= 4; tries++) {
         build_ghost_plan(m, gP, gp);
-        if (gp.P != gP || gp.dup < 1.1 || gp.nslots == 0) break;
+        if (gp.P != gP || gp.dup < kGhostMinDup || gp.nslots == 0) break;
         // (ghost_reduce puts the ghost nodes on grid.y: at most 65535)
         if (gp.nslots <= ghost_slot_bound(dp.N, gP) && gp.gnode.size() <= 65535) {
             take = true;
