@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "bem_internal.hpp"
+#include "xcd.hpp"
 
 namespace bem {
 
@@ -93,14 +94,8 @@ struct AsmArgs {
     QTab<R> q;
 };
 
-// Workgroup id -> logical id, so that consecutive logical ids (row blocks of the
-// same tile, which share the tile's element records) land on the same XCD.
-// Dispatch places workgroup g on XCD g % 8.
-__device__ __forceinline__ int xcd_remap(int g, int n) {
-    const int per = n / 8, rem = n % 8;
-    const int xcd = g % 8, idx = g / 8;
-    return (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
-}
+// (xcd_remap, xcd.hpp: consecutive logical ids, here the row blocks of one
+// tile, which share the tile's element records, land on the same XCD)
 
 // --------------------------------------------------------- element prep ---
 // 6-point Gauss-Legendre nodes and weights on [-1, 1]

@@ -58,6 +58,7 @@
 
 #include "bem_internal.hpp"
 #include "small_dense.hpp"
+#include "xcd.hpp"
 
 namespace bem {
 
@@ -339,15 +340,21 @@ __global__ void kc_gram_sum2(const double2 *__restrict__ seg, int RR, double2 *_
 // out[:, b] = Z[:, b] (or 0) + sum_a U[:, a] C[a, b] for an Rb x Rb matrix C
 // (column-major); out32, when given, gets the FP32 copy.  out may be Z but
 // not U.
+// One-dimensional grid of (n / 256 row blocks) x (column groups of kMulCols):
+// the column groups of a row block are consecutive logical ids, which
+// xcd_remap puts on one XCD in dispatch order, so the groups after the first
+// read U from that XCD's L2.
 __global__ __launch_bounds__(256) void kc_block_mul(const double2 *__restrict__ U, const double2 *__restrict__ C,
                                                     int Rb, const double2 *Z, double2 *out, float2 *__restrict__ out32,
                                                     long ld, int n) {
     __shared__ double2 Cs[kMaxBlock * kMulCols];  // [column - b0][a]
-    const int b0 = blockIdx.y * kMulCols;
+    const int ncg = (Rb + kMulCols - 1) / kMulCols;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = L / ncg, b0 = (L - rb * ncg) * kMulCols;
     const int nb = min(kMulCols, Rb - b0);
     for (int e = threadIdx.x; e < Rb * nb; e += 256) Cs[e] = C[(size_t)(b0 + e / Rb) * Rb + e % Rb];
     __syncthreads();
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = rb * 256 + threadIdx.x;
     if (i >= n) return;
     double2 acc[kMulCols];
 #pragma unroll
@@ -657,6 +664,9 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         return grams(gp, 1, Rb, &G);
     };
     // out = Z (or 0) + U C for an Rb x Rb host matrix C
+    auto block_mul_grid = [&](int Rb) {
+        return (unsigned)(((N + 255) / 256) * ((Rb + kMulCols - 1) / kMulCols));
+    };
     int cslot = 0;
     auto bmul = [&](const double2 *U, const Small &Cm, int Rb, const double2 *Z, double2 *out,
                     float2 *out32) -> int {
@@ -665,8 +675,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         cslot = (cslot + 1) % kCSlots;
         for (int e = 0; e < Rb * Rb; e++) hc[e] = make_double2(Cm[e].real(), Cm[e].imag());
         HIP_CALL(hipMemcpyAsync(dc, hc, sizeof(double2) * Rb * Rb, hipMemcpyHostToDevice, s));
-        kc_block_mul<<<dim3((N + 255) / 256, (Rb + kMulCols - 1) / kMulCols), 256, 0, s>>>(U, dc, Rb, Z, out,
-                                                                                           out32, ld, N);
+        kc_block_mul<<<block_mul_grid(Rb), 256, 0, s>>>(U, dc, Rb, Z, out, out32, ld, N);
         HIP_CALL(hipGetLastError());
         return EXIT_SUCCESS;
     };
@@ -687,8 +696,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     };
     auto bmul_dev = [&](const double2 *U, const double2 *dc, int Rb, const double2 *Z, double2 *out,
                         float2 *out32) -> int {
-        kc_block_mul<<<dim3((N + 255) / 256, (Rb + kMulCols - 1) / kMulCols), 256, 0, s>>>(U, dc, Rb, Z, out,
-                                                                                           out32, ld, N);
+        kc_block_mul<<<block_mul_grid(Rb), 256, 0, s>>>(U, dc, Rb, Z, out, out32, ld, N);
         HIP_CALL(hipGetLastError());
         return EXIT_SUCCESS;
     };
