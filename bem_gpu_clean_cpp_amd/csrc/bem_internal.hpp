@@ -304,6 +304,15 @@ int gemm_hc_f32(const float2 *AH, long M, long N, const float2 *T, long ldt, flo
                 hipStream_t s);
 int gemm_hc_f32z(const float2 *AH, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R, void *ws,
                  hipStream_t s);
+// FP16-split products (A and X read as FP16 hi/lo pairs on FP16 MFMA, FP32
+// accumulation): amax bounds |Re A|, |Im A| (gemm_absmax); xs >= ws_split_x
+size_t ws_absmax();
+int gemm_absmax(const float2 *A, long lda, long M, long N, float *ws, float **out, hipStream_t s);
+size_t ws_split_x(long N, int R);
+int gemm_n_f16s(const float2 *A, long lda, long M, long N, float amax, const float2 *X, long ldx, float2 *Y, long ldy,
+                int R, void *ws, void *xs, hipStream_t s);
+int gemm_hc_f16s_z(const float2 *AH, long M, long N, float amax, const float2 *T, long ldt, double2 *Z, long ldz,
+                   int R, void *ws, void *xs, hipStream_t s);
 size_t ws_mres(long M, long N, int R);
 // nmat matrices astride elements apart, matrix z on right-hand sides
 // z R / nmat .. (z + 1) R / nmat - 1: the FP32 product with FP64 output and

@@ -1041,6 +1041,257 @@ int mh_f32a_hc(const float2 *AH, long M, long N, const double2 *T, long ldt, dou
                                "mh_f64acc", s, true);
 }
 
+// ------------------------------------------------- FP16-split products ---
+// Y = A X for up to 32 right-hand sides per launch group on
+// v_mfma_f32_32x32x16_f16, with A and X each read as a pair of FP16 values
+// (hi = fp16(v), lo = fp16(v - hi): about 22 bits) and
+//   A X ~ A_hi X_hi + A_hi X_lo + A_lo X_hi
+// accumulated in FP32 (the FP16 products are exact in FP32; the dropped
+// A_lo X_lo is 2^-22 of a term).  A stays the FP32 column-major matrix: a
+// lane splits its operands in registers.  X is split once per product into
+// FP16 planes.  Power-of-two scales keep the values in FP16's normal range:
+// |A| 2^sa <= 2^13 from amax (the caller's bound on |Re A|, |Im A|), and per
+// right-hand side |X| 2^sx <= 2^10 from the column's maximum; the result is
+// multiplied by 2^-(sa + sx).  Complex products in the four-product form
+// (its operands need no sums before the split).
+//
+// MFMA lane layout of v_mfma_f32_32x32x16_f16 (checked on the device with an
+// in-bounds one-hot probe): A[i][8 (l / 32) + h] from lane l = i + 32 (l / 32),
+// half h; B[8 (l / 32) + h][j] from lane j + 32 (l / 32); the result as for
+// v_mfma_f32_32x32x2f32.  Lane (i, hh = l / 32) loads complex rows 2i, 2i + 1
+// of columns jb + 8 hh .. jb + 8 hh + 7 (eight 16-byte loads of A per 16
+// columns, as many as the FP32 kernel) and the four planes of X for its 8
+// columns (16 bytes each).
+//
+// Measured at M = N = 40962, R = 32, 64 chunks: 2.93-2.98 ms with the X split
+// (standalone), 3.06 ms in the block CG solve, against 3.30-3.44 ms for the
+// FP32 kernel (three-product form, paired X, two row tiles).  The waves wait
+// on the A stream (4.5 TB/s); the MFMA and VALU work each take about a fifth
+// of the time.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+constexpr int kF16Cols = 16;     // columns per MFMA step (K)
+constexpr int kF16Chunk = 640;   // split-K chunk, kMaxAccum columns
+
+// cmax[n] = max over rows of max(|Re X[:, n]|, |Im X[:, n]|) (one workgroup
+// per right-hand side, fixed-order tree)
+__global__ __launch_bounds__(256) void k_colmax(const float2 *__restrict__ X, long ldx, int N, float *__restrict__ cmax) {
+    __shared__ float red[4];
+    const int n = blockIdx.x;
+    float m = 0.f;
+    for (int j = threadIdx.x; j < N; j += 256) {
+        const float2 x = X[(size_t)n * ldx + j];
+        m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) cmax[n] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// power of two p with v p <= 2^e (v > 0), else 1
+__device__ __forceinline__ float pow2_scale(float v, int e) {
+    if (!(v > 0.f) || !isfinite(v)) return 1.f;
+    return ldexpf(1.f, e - ilogbf(v) - 1);
+}
+
+// Xp[((n kb-major) * 4 + p) * 16 + t] for right-hand side n, 16-column block kb,
+// planes p = Re hi, Re lo, Im hi, Im lo; xinv[n] = 2^-(sa + sx_n)
+__global__ void k_split_x(const float2 *__restrict__ X, long ldx, int N, int R, const float *__restrict__ cmax,
+                          float sa, _Float16 *__restrict__ Xp, int nkb, float *__restrict__ xinv) {
+    const int n = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;  // e = kb * 16 + t
+    if (n >= R || e >= nkb * kF16Cols) return;
+    const float sx = pow2_scale(cmax[n], 10);
+    if (e == 0) xinv[n] = 1.f / (sa * sx);
+    float2 x = e < N ? X[(size_t)n * ldx + e] : make_float2(0.f, 0.f);
+    x.x *= sx;
+    x.y *= sx;
+    const _Float16 rh = (_Float16)x.x, ih = (_Float16)x.y;
+    _Float16 *o = Xp + ((size_t)n * nkb + (e >> 4)) * 64 + (e & 15);
+    o[0] = rh;
+    o[16] = (_Float16)(x.x - (float)rh);
+    o[32] = ih;
+    o[48] = (_Float16)(x.y - (float)ih);
+}
+
+// part[(chunk * R + n) * ldp + i] = 2^-(sa + sx_n) sum_{j in chunk} A~[i, j] X~[j, n]
+// for n in [rg0, rg0 + 32): wave = 64 complex rows (RING buffers of 16 columns)
+template <int RING>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_n_f16s_mfma(
+    const float2 *__restrict__ A, long lda, int M, int N, const _Float16 *__restrict__ Xp, int nkb, int R, int rg0,
+    float sa, const float *__restrict__ xinv, int kc, float2 *__restrict__ part, long ldp) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row0 = (blockIdx.x * 4 + wave) * 64;
+    if (row0 >= M) return;
+    const int chunk = blockIdx.y;
+    const int j0 = chunk * kc, j1 = min(N, j0 + kc);
+    const int i = lane & 31, hh = lane >> 5;
+    v16f acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[t] = (v16f){0};
+    const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * i);
+    const long lda4 = lda / 2;
+    // right-hand sides past R read the planes of R - 1 (their results are not written)
+    const half8 *Xb = reinterpret_cast<const half8 *>(Xp) + (size_t)min(rg0 + i, R - 1) * nkb * 8 + hh;
+    struct Buf {
+        float4 a[8];
+        half8 x[4];
+    };
+    auto load = [&](int jb, Buf &B) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const int c = jb + 8 * hh + t;
+            const float4 v = Ab[(size_t)min(c, j1 - 1) * lda4];
+            B.a[t] = c < j1 ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const int kb = min(jb / kF16Cols, nkb - 1);
+#pragma unroll
+        for (int p = 0; p < 4; p++) B.x[p] = Xb[(size_t)kb * 8 + 2 * p];
+    };
+    auto mul = [&](const Buf &B) {
+        // the lane's four real rows (Re, Im of rows 2i and 2i + 1) over its 8 columns
+        half8 hi[4], lo[4];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const float v[4] = {B.a[t].x * sa, B.a[t].y * sa, B.a[t].z * sa, B.a[t].w * sa};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const _Float16 h = (_Float16)v[q];
+                hi[q][t] = h;
+                lo[q][t] = (_Float16)(v[q] - (float)h);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            // real row q against Re X (planes 0, 1) and Im X (planes 2, 3)
+            acc[2 * q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi[q], B.x[0], acc[2 * q], 0, 0, 0);
+            acc[2 * q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi[q], B.x[1], acc[2 * q], 0, 0, 0);
+            acc[2 * q] = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo[q], B.x[0], acc[2 * q], 0, 0, 0);
+            acc[2 * q + 1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi[q], B.x[2], acc[2 * q + 1], 0, 0, 0);
+            acc[2 * q + 1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(hi[q], B.x[3], acc[2 * q + 1], 0, 0, 0);
+            acc[2 * q + 1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(lo[q], B.x[2], acc[2 * q + 1], 0, 0, 0);
+        }
+    };
+    Buf b[RING];
+    constexpr int W = kF16Cols;
+    const int jend = j0 + (j1 - j0) / (W * RING) * (W * RING);  // end of the full rounds of the ring
+#pragma unroll
+    for (int w = 0; w < RING - 1; w++) load(j0 + W * w, b[w]);
+    for (int jb = j0; jb < jend; jb += W * RING) {
+#pragma unroll
+        for (int w = 0; w < RING; w++) {
+            load(jb + W * (w + RING - 1), b[(w + RING - 1) % RING]);
+            __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
+            mul(b[w]);
+        }
+    }
+    for (int jb = jend; jb < j1; jb += W) {
+        load(jb, b[0]);
+        mul(b[0]);
+    }
+    const int n = rg0 + i;
+    if (n >= R) return;
+    const float u = xinv[n];
+    float2 *pc = part + ((size_t)chunk * R + n) * ldp;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int r = row0 + 2 * d32_row(lane, v);
+        if (r < M) pc[r] = make_float2((acc[0][v] - acc[3][v]) * u, (acc[1][v] + acc[2][v]) * u);
+        if (r + 1 < M) pc[r + 1] = make_float2((acc[4][v] - acc[7][v]) * u, (acc[5][v] + acc[6][v]) * u);
+    }
+}
+
+// bm[b] = max of max(|Re|, |Im|) over a column range of A; then out = max of bm
+// (fixed-order tree, one workgroup)
+__global__ __launch_bounds__(256) void k_absmax_part(const float2 *__restrict__ A, long lda, int M, int N,
+                                                     float *__restrict__ bm) {
+    __shared__ float red[4];
+    float m = 0.f;
+    for (int j = blockIdx.x; j < N; j += gridDim.x)
+        for (int i = threadIdx.x; i < M; i += 256) {
+            const float2 a = A[(size_t)j * lda + i];
+            m = fmaxf(m, fmaxf(fabsf(a.x), fabsf(a.y)));
+        }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) bm[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+__global__ __launch_bounds__(256) void k_absmax_final(const float *__restrict__ bm, int nb, float *__restrict__ out) {
+    __shared__ float red[4];
+    float m = 0.f;
+    for (int b = threadIdx.x; b < nb; b += 256) m = fmaxf(m, bm[b]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+constexpr int kAbsmaxBlocks = 2048;
+size_t ws_absmax() { return sizeof(float) * (kAbsmaxBlocks + 1); }
+
+// *out (device) = max |Re A|, |Im A| over the M x N matrix; ws >= ws_absmax()
+int gemm_absmax(const float2 *A, long lda, long M, long N, float *ws, float **out, hipStream_t s) {
+    if (M <= 0 || N <= 0) BEM_FAIL("absmax of an empty matrix");
+    const int nb = (int)std::min<long>(kAbsmaxBlocks, N);
+    k_absmax_part<<<nb, 256, 0, s>>>(A, lda, (int)M, (int)N, ws);
+    k_absmax_final<<<1, 256, 0, s>>>(ws, nb, ws + kAbsmaxBlocks);
+    HIP_CALL(hipGetLastError());
+    *out = ws + kAbsmaxBlocks;
+    return EXIT_SUCCESS;
+}
+
+// split workspace: planes of up to 32 right-hand sides, their maxima and scales
+size_t ws_split_x(long N, int R) {
+    const size_t nkb = (size_t)(N + kF16Cols - 1) / kF16Cols;
+    return sizeof(_Float16) * (size_t)std::max(1, R) * nkb * 64 + 2 * sizeof(float) * (size_t)std::max(1, R) + 256;
+}
+
+template <typename S, typename OC2>
+static int gemm_f16s_impl(const float2 *A, long lda, long M, long N, float amax, const float2 *X, long ldx, OC2 *Y,
+                          long ldy, int R, void *ws, void *xs, const char *name, hipStream_t s) {
+    if (M <= 0 || N <= 0 || R <= 0) return EXIT_SUCCESS;
+    if (lda % 2 || lda < (M + 63) / 64 * 64) BEM_FAIL("FP16-split product needs an even lda >= roundup(M, 64)");
+    if (!(amax >= 0.f) || !std::isfinite(amax)) BEM_FAIL("FP16-split product: bad bound %g on |A|", (double)amax);
+    // 2^sa with amax 2^sa <= 2^13
+    const float sa = amax > 0.f ? std::ldexp(1.0f, 13 - std::ilogb(amax) - 1) : 1.0f;
+    const int nkb = (int)((N + kF16Cols - 1) / kF16Cols);
+    const int nch = (int)((N + kF16Chunk - 1) / kF16Chunk);
+    const long ldp = (M + 63) / 64 * 64;
+    auto *part = (float2 *)ws;
+    auto *Xp = (_Float16 *)xs;
+    float *cmax = (float *)((char *)xs + sizeof(_Float16) * (size_t)R * nkb * 64);
+    float *xinv = cmax + R;
+    TimedRegion tr;
+    timing_begin(tr, name, s);
+    k_colmax<<<R, 256, 0, s>>>(X, ldx, (int)N, cmax);
+    k_split_x<<<dim3((unsigned)((nkb * kF16Cols + 255) / 256), R), 256, 0, s>>>(X, ldx, (int)N, R, cmax, sa, Xp, nkb,
+                                                                                 xinv);
+    for (int rg = 0; rg < R; rg += 32)
+        gemm_n_f16s_mfma<2><<<dim3((unsigned)((M + 255) / 256), nch), 256, 0, s>>>(
+            A, lda, (int)M, (int)N, Xp, nkb, R, rg, sa, xinv, kF16Chunk, part, ldp);
+    HIP_CALL(hipGetLastError());
+    timing_end(tr, s);
+    dim3 g2((unsigned)((M + 255) / 256), R);
+    gemm_reduce<float, S, OC2><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)M, (const OC2 *)nullptr, 0, Y, ldy);
+    HIP_CALL(hipGetLastError());
+    return EXIT_SUCCESS;
+}
+
+// Y (M x R) = A X; amax >= max |Re A|, |Im A|; ws >= ws_gemm(M, N, R), xs >= ws_split_x(N, R)
+int gemm_n_f16s(const float2 *A, long lda, long M, long N, float amax, const float2 *X, long ldx, float2 *Y, long ldy,
+                int R, void *ws, void *xs, hipStream_t s) {
+    return gemm_f16s_impl<float, float2>(A, lda, M, N, amax, X, ldx, Y, ldy, R, ws, xs, "gemm_n_f16s", s);
+}
+// Z (N x R) = A^H T through the conjugate-transposed copy, FP64 out
+int gemm_hc_f16s_z(const float2 *AH, long M, long N, float amax, const float2 *T, long ldt, double2 *Z, long ldz,
+                   int R, void *ws, void *xs, hipStream_t s) {
+    return gemm_f16s_impl<double, double2>(AH, adjoint_copy_ld(N), N, M, amax, T, ldt, Z, ldz, R, ws, xs,
+                                           "gemm_hc_f16s", s);
+}
+
 // ------------------------------------------------------ tuner driver ---
 // values in [-0.5, 0.5) from a counter hash: the tuner's operands
 __global__ void k_fill_tune(float2 *a, size_t n, unsigned long long seed) {

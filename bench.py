@@ -49,6 +49,7 @@ def executed_share():
     form (BEM_GEMM_4M) all 8."""
     return 1.0 if os.environ.get("BEM_GEMM_4M") else 0.75
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
 FLOP_PER_EVAL = 80.0       # nominal FP32 flops per (row, quadrature point) evaluation
 # measured VALU issue rate of the FP32 assembly kernel (wave-instructions per
 # SIMD-cycle, SQ counters of profiles/precision_s5_asm_valu_counters.txt)
@@ -424,34 +425,40 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
     sti = step()
     bem.sync()
     bem.timing_enable(False)
-    t = {name: bem.timing_get(name) for name in ("gemm_n_f32", "gemm_h_f32", "gemm_hc_f32", "adjoint_copy",
-                                                 "asm_tile_f32", "sing_elems", "asm_finalize", "ghost_reduce",
-                                                 "mres_f64acc", "mh_f64acc")}
+    t = {name: bem.timing_get(name) for name in ("gemm_n_f32", "gemm_h_f32", "gemm_hc_f32", "gemm_n_f16s",
+                                                 "gemm_hc_f16s", "adjoint_copy", "asm_tile_f32", "sing_elems",
+                                                 "asm_finalize", "ghost_reduce", "mres_f64acc", "mh_f64acc")}
     bem.timing_reset()
     sv.close()
     avg = {kk: (v[0] / v[1] if v[1] else None) for kk, v in t.items()}
     flops = 8.0 * M * N * R
     gbytes = 8.0 * M * N + 8.0 * (M + N) * R
-    # the adjoint products run on the Y = A X kernel over the conjugate-transposed
-    # copy when the solver made one ("gemm_hc_f32"), else on the adjoint kernel
-    hname = "gemm_hc_f32" if t["gemm_hc_f32"][1] else "gemm_h_f32"
-    g_avg, h_avg = avg["gemm_n_f32"], avg[hname]
+    # block CG runs its products on the FP16-split kernels ("gemm_n_f16s",
+    # "gemm_hc_f16s") unless BEM_GEMM_NO_F16 is set; the adjoint products run on
+    # the Y = A X kernel over the conjugate-transposed copy when the solver made
+    # one ("gemm_hc_*"), else on the adjoint kernel
+    f16s = bool(t["gemm_n_f16s"][1])
+    nname = "gemm_n_f16s" if f16s else "gemm_n_f32"
+    hname = "gemm_hc_f16s" if f16s else ("gemm_hc_f32" if t["gemm_hc_f32"][1] else "gemm_h_f32")
+    g_avg, h_avg = avg[nname], avg[hname]
     asm_ms = sum(v[0] for kk, v in t.items() if kk in ("asm_tile_f32", "sing_elems", "asm_finalize",
                                                         "ghost_reduce")) / max(t["asm_tile_f32"][1], 1)
     out = {
         "workload": "large: perturbed ellipsoid s6 (N=%d, E=%d, M=%d, A %.1f GB), rigid, 12 kHz, 32 CHIEF, "
-                    "32 point sources 1 cm outside; FP32 assembly + %s (FP32 MFMA products, R=32) with FP64 "
+                    "32 point sources 1 cm outside; FP32 assembly + %s (%s, R=32) with FP64 "
                     "refinement to %.0e" % (N, E, M, 8.0 * M * N / 1e9,
                                             {"gmres": "BA-GMRES(100) on the normal equations",
                                              "block": "block CG on the normal equations"}.get(
-                                                method, "CG on the normal equations"), refine_tol),
+                                                method, "CG on the normal equations"),
+                                            "FP32 matrix, FP16-split MFMA products with FP32 accumulation" if f16s
+                                            else "FP32 MFMA products", refine_tol),
         "N": N, "E": E, "M": M, "rhs": R,
         "value": R / (ms_per_step / 1e3), "ms_per_step": ms_per_step,
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2)},
         "assembly_ms": round(asm_ms, 2),
         "assembly_gbs": round((8.0 * M * N + 8.0 * M * R) / (asm_ms * 1e-3) / 1e9, 1),
         "solve": {kk: st[kk] for kk in ("iterations", "refinements", "converged", "residual")},
-        "instrumented_step": {"products_n": t["gemm_n_f32"][1], "products_h": t[hname][1],
+        "instrumented_step": {"products_n": t[nname][1], "products_h": t[hname][1],
                               "adjoint_copy_ms": round(t["adjoint_copy"][0], 3) if t["adjoint_copy"][1] else None,
                               "solve_ms": round(sti["solve_ms"], 1)},
         "roofline": {"bound": "mfma", "kernel": "gemm_n_f32", "achieved": round(flops / (g_avg * 1e-3) / 1e12, 2),
@@ -467,6 +474,24 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
                                  "frac": round(flops / (h_avg * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
                                  "traffic": measured_traffic(hname, "large")}},
     }
+    if f16s:
+        # the FP16-split product executes 24 FP16 MFMA flops per complex
+        # multiply-add (four real products, each as hi*hi + hi*lo + lo*hi) at
+        # 2.5 PFLOP/s: its time floor is the A stream, so its roofline is HBM
+        gbs = gbytes / (g_avg * 1e-3) / 1e9
+        hgbs = gbytes / (h_avg * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": nname, "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                           "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                           "traffic": measured_traffic(nname, "large"), "avg_ms": round(g_avg, 3),
+                           "launches": t[nname][1], "bytes_per_launch": gbytes, "flops_per_launch": flops,
+                           "complex_tflops": round(flops / (g_avg * 1e-3) / 1e12, 2),
+                           "mfma_fp16": {"executed_flops_per_launch": 3.0 * flops,
+                                         "achieved": round(3.0 * flops / (g_avg * 1e-3) / 1e12, 1),
+                                         "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                                         "frac": round(3.0 * flops / (g_avg * 1e-3) / 1e12 / PEAK_FP16_TFLOPS, 4)},
+                           "adjoint": {"kernel": hname, "avg_ms": round(h_avg, 3), "achieved": round(hgbs, 1),
+                                       "frac": round(hgbs / PEAK_HBM_GBS, 4),
+                                       "traffic": measured_traffic(hname, "large")}}
     if avg["mres_f64acc"]:
         out["roofline_fp64_residual"] = {"bound": "mfma", "kernel": "mres_f64acc (FP64 MFMA, FP32 A widened)",
                                          "avg_ms": round(avg["mres_f64acc"], 3),

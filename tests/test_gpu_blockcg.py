@@ -168,13 +168,63 @@ def test_block_cg_size_policy_small_system_takes_lockstep(monkeypatch):
     assert np.array_equal(xb, xl)
 
 
+@pytest.mark.parametrize("R", [24, 36])
+def test_block_cg_fp16_split_products(monkeypatch, R):
+    """Block CG's products on the FP16-split kernels (the default with an FP32
+    matrix) against the FP32 kernels (BEM_GEMM_NO_F16): per-launch timing
+    shows which kernels ran (one product pair per block step); point sources
+    at 0.12 m to 120 m from the centre of a 0.1 m sphere give right-hand sides
+    whose norms differ by about a thousand, which the per-column scales of the
+    split absorb; 36 right-hand sides take two launch groups, the second
+    ragged.  Both solutions match the oracle's least-squares solution at 1e-5
+    and each other at 1e-9 after FP64 refinement."""
+    m = _mesh(3, 7)
+    k = 27.1
+    u = C.sphere_points(41, R, 1.0)
+    src = (u * np.logspace(np.log10(0.12), np.log10(120.0), R)[:, None]).astype(np.float32)
+    s = bem.Solver(m)
+    s.assemble(k, bem.PREC_FP32)
+    s.set_rhs(bem.RHS_POINT, src, 1.0)
+    names = ("gemm_n_f16s", "gemm_hc_f16s", "gemm_n_f32", "gemm_hc_f32")
+    out = {}
+    try:
+        for mode in ("f16", "f32"):
+            if mode == "f32":
+                monkeypatch.setenv("BEM_GEMM_NO_F16", "1")
+            bem.timing_reset()
+            bem.timing_enable(True)
+            st = s.solve(bem.SOLVE_BLOCK_CG, **KW)
+            bem.sync()
+            bem.timing_enable(False)
+            out[mode] = (st, s.solution(), {n: bem.timing_get(n)[1] for n in names})
+    finally:
+        bem.timing_enable(False)
+        bem.timing_reset()
+    ref = _oracle(m, k, bem.RHS_POINT, src)
+    (s16, x16, c16), (s32, x32, c32) = out["f16"], out["f32"]
+    rel = _relcols(x16, x32).max()
+    e16, e32 = _relcols(x16, ref).max(), _relcols(x32, ref).max()
+    print("case: FP16-split products, R %d: %s %s / FP32 %s %s, difference %.3g, vs oracle %.3g / %.3g" %
+          (R, s16, c16, s32, c32, rel, e16, e32), flush=True)
+    assert s16["converged"] == 1 and s32["converged"] == 1
+    # (a refinement cycle left with one unconverged system runs lockstep CG on
+    # the FP32 kernels, so those may appear in the default solve too)
+    assert c16["gemm_n_f16s"] > 0 and c16["gemm_n_f16s"] == c16["gemm_hc_f16s"]
+    assert c32["gemm_n_f16s"] == 0 and c32["gemm_hc_f16s"] == 0 and c32["gemm_n_f32"] > 0
+    assert rel < 1e-9
+    assert e16 < 1e-5 and e32 < 1e-5
+
+
 @pytest.mark.parametrize("switch,nrhs", [("BEM_BLOCK_CG_TWO_PASS", 20), ("BEM_BLOCK_CG_EXPLICIT", 20),
-                                         ("BEM_GRAM_VALU", 20), ("BEM_GRAM_VALU", 40)])
+                                         ("BEM_GRAM_VALU", 20), ("BEM_GRAM_VALU", 40),
+                                         ("BEM_GEMM_NO_F16", 20), ("BEM_GEMM_NO_F16", 40)])
 def test_block_cg_qr_variants_agree(monkeypatch, switch, nrhs):
     """The second CholeskyQR pass at every step, the explicit QR at every step,
-    and the Gram products on the VALU kernel instead of MFMA (20 right-hand
-    sides: one 16 x 16 tile per wave; 40: four) give the default solve's
-    solution to 1e-9 after refinement."""
+    the Gram products on the VALU kernel instead of MFMA (20 right-hand sides:
+    one 16 x 16 tile per wave; 40: four), and the block products on the FP32
+    kernels instead of the FP16-split ones (20 right-hand sides: one launch
+    group; 40: two) give the default solve's solution to 1e-9 after
+    refinement."""
     m = _mesh(3, 8)
     k = 31.3
     d = C.sphere_points(23, nrhs, 1.0)

@@ -292,7 +292,9 @@ def test_three_and_four_product_forms_agree(monkeypatch, nrhs, method):
     kw = dict(restart=30, tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=50)
     monkeypatch.delenv("BEM_GEMM_4M", raising=False)
     if method == bem.SOLVE_BLOCK_CG:
+        # block CG on the FP32 product kernels (the FP16-split ones have one form)
         monkeypatch.setenv("BEM_BLOCK_CG_ALWAYS", "1")
+        monkeypatch.setenv("BEM_GEMM_NO_F16", "1")
     st3 = s.solve(method, **kw)
     x3 = s.solution()
     monkeypatch.setenv("BEM_GEMM_4M", "1")
@@ -309,8 +311,9 @@ def test_three_and_four_product_forms_agree(monkeypatch, nrhs, method):
 
 @pytest.mark.gpu
 def test_product_row_tiles_same_bits(monkeypatch):
-    """The 32-column product with two 64-row tiles per wave (taken for large
-    launches) against one tile per wave (BEM_GEMM_ROW_TILES forces either):
+    """The FP32 32-column product (BEM_GEMM_NO_F16) with two 64-row tiles per
+    wave (taken for large launches) against one tile per wave
+    (BEM_GEMM_ROW_TILES forces either):
     every element's sum runs in the same order, so a block CG solve with 24
     right-hand sides on an s3 sphere with 7 CHIEF rows (ragged row tiles)
     returns the same bits."""
@@ -321,6 +324,7 @@ def test_product_row_tiles_same_bits(monkeypatch):
     s.assemble(k, bem.PREC_FP32)
     s.set_rhs(bem.RHS_PLANE, d, 1.0)
     monkeypatch.setenv("BEM_BLOCK_CG_ALWAYS", "1")
+    monkeypatch.setenv("BEM_GEMM_NO_F16", "1")
     kw = dict(tol=1e-6, maxit=5000, refine_tol=1e-10, max_refine=50)
     out = {}
     for tiles in ("1", "2"):
