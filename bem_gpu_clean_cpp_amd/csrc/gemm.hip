@@ -1115,35 +1115,60 @@ __global__ void k_split_x(const float2 *__restrict__ X, long ldx, int N, int R, 
 }
 
 // part[(chunk * R + n) * ldp + i] = 2^-(sa + sx_n) sum_{j in chunk} A~[i, j] X~[j, n]
-// for n in [rg0, rg0 + 32): wave = 64 complex rows (RING buffers of 16 columns)
+// for n in [rg0, rg0 + 32).  A workgroup takes one 64-row tile over a chunk of
+// 4 kc columns: wave w the w-th kc columns (RING buffers of 16 columns), then
+// the four partials are summed in LDS in wave order and written as contiguous
+// 64-row columns.  Measured at M = N = 40962, R = 32: 2.58-2.65 ms against
+// 2.91-2.95 ms with one partial per wave written from registers (four times
+// the partial bytes: 672 MB, which cost about 0.5 ms of the A stream whatever
+// their write pattern) and 2.41 ms with no partials written at all.  Longer
+// per-wave sums (2 kc columns) were no faster and less accurate.
+constexpr int kF16Waves = 4;    // waves (column sub-chunks) per workgroup
+constexpr int kF16LdsPad = 65;  // float2 per right-hand side in the LDS partials (bank spread)
 template <int RING>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm_n_f16s_mfma(
     const float2 *__restrict__ A, long lda, int M, int N, const _Float16 *__restrict__ Xp, int nkb, int R, int rg0,
     float sa, const float *__restrict__ xinv, int kc, float2 *__restrict__ part, long ldp) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = (blockIdx.x * 4 + wave) * 64;
-    if (row0 >= M) return;
+    __shared__ float2 red[kF16Waves * 32 * kF16LdsPad];
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int row0 = blockIdx.x * 64;
     const int chunk = blockIdx.y;
-    const int j0 = chunk * kc, j1 = min(N, j0 + kc);
+    const int j0 = min(N, (chunk * kF16Waves + wave) * kc), j1 = min(N, j0 + kc);
     const int i = lane & 31, hh = lane >> 5;
     v16f acc[8];
 #pragma unroll
     for (int t = 0; t < 8; t++) acc[t] = (v16f){0};
-    const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * i);
-    const long lda4 = lda / 2;
     // right-hand sides past R read the planes of R - 1 (their results are not written)
-    const half8 *Xb = reinterpret_cast<const half8 *>(Xp) + (size_t)min(rg0 + i, R - 1) * nkb * 8 + hh;
+    const size_t xlane = (size_t)min(rg0 + i, R - 1) * nkb * 8 + hh;  // half8 index of the lane's planes
+    // a full 16-column step: wave-uniform base plus the lane's byte offset (< 2^32)
+    const unsigned aoff = (unsigned)(((size_t)(8 * hh) * lda + 2 * i) * sizeof(float2));
+    const unsigned xoff = (unsigned)(xlane * sizeof(half8));
+    const char *Aw = reinterpret_cast<const char *>(A + row0 + (size_t)j0 * lda);
+    const size_t cstep = (size_t)lda * sizeof(float2);  // bytes per column
+    const char *Xw = reinterpret_cast<const char *>(Xp) + (size_t)(j0 / kF16Cols) * 8 * sizeof(half8);
     struct Buf {
         float4 a[8];
         half8 x[4];
     };
-    auto load = [&](int jb, Buf &B) {
+    const int nfull = (j1 - j0) / kF16Cols;  // kc is a multiple of 16: only the last chunk has a partial step
+    auto load_full = [&](int st, Buf &B) {
+        const char *base = Aw + (size_t)(kF16Cols * st) * cstep;
+#pragma unroll
+        for (int t = 0; t < 8; t++) B.a[t] = *reinterpret_cast<const float4 *>(base + t * cstep + aoff);
+        const char *xb = Xw + (size_t)st * 8 * sizeof(half8);
+#pragma unroll
+        for (int p = 0; p < 4; p++) B.x[p] = *reinterpret_cast<const half8 *>(xb + 2 * p * sizeof(half8) + xoff);
+    };
+    auto load_tail = [&](int jb, Buf &B) {
+        const float4 *Ab = reinterpret_cast<const float4 *>(A + row0 + 2 * i);
+        const long lda4 = lda / 2;
 #pragma unroll
         for (int t = 0; t < 8; t++) {
             const int c = jb + 8 * hh + t;
             const float4 v = Ab[(size_t)min(c, j1 - 1) * lda4];
             B.a[t] = c < j1 ? v : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        const half8 *Xb = reinterpret_cast<const half8 *>(Xp) + xlane;
         const int kb = min(jb / kF16Cols, nkb - 1);
 #pragma unroll
         for (int p = 0; p < 4; p++) B.x[p] = Xb[(size_t)kb * 8 + 2 * p];
@@ -1173,31 +1198,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
     Buf b[RING];
-    constexpr int W = kF16Cols;
-    const int jend = j0 + (j1 - j0) / (W * RING) * (W * RING);  // end of the full rounds of the ring
 #pragma unroll
-    for (int w = 0; w < RING - 1; w++) load(j0 + W * w, b[w]);
-    for (int jb = j0; jb < jend; jb += W * RING) {
+    for (int w = 0; w < RING - 1; w++)
+        if (w < nfull) load_full(w, b[w]);
+    for (int s0 = 0; s0 < nfull; s0 += RING) {
 #pragma unroll
         for (int w = 0; w < RING; w++) {
-            load(jb + W * (w + RING - 1), b[(w + RING - 1) % RING]);
+            const int st = s0 + w;
+            if (st >= nfull) break;
+            if (st + RING - 1 < nfull) load_full(st + RING - 1, b[(w + RING - 1) % RING]);
             __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the products
             mul(b[w]);
         }
     }
-    for (int jb = jend; jb < j1; jb += W) {
-        load(jb, b[0]);
+    if (j0 + kF16Cols * nfull < j1) {
+        load_tail(j0 + kF16Cols * nfull, b[0]);
         mul(b[0]);
     }
-    const int n = rg0 + i;
-    if (n >= R) return;
-    const float u = xinv[n];
-    float2 *pc = part + ((size_t)chunk * R + n) * ldp;
+    // the wave's complex partial of right-hand side rg0 + i, rows 2 d and 2 d + 1
+    float2 *rw = red + (size_t)(wave * 32 + i) * kF16LdsPad;
 #pragma unroll
     for (int v = 0; v < 16; v++) {
-        const int r = row0 + 2 * d32_row(lane, v);
-        if (r < M) pc[r] = make_float2((acc[0][v] - acc[3][v]) * u, (acc[1][v] + acc[2][v]) * u);
-        if (r + 1 < M) pc[r + 1] = make_float2((acc[4][v] - acc[7][v]) * u, (acc[5][v] + acc[6][v]) * u);
+        const int d = d32_row(lane, v);
+        rw[2 * d] = make_float2(acc[0][v] - acc[3][v], acc[1][v] + acc[2][v]);
+        rw[2 * d + 1] = make_float2(acc[4][v] - acc[7][v], acc[5][v] + acc[6][v]);
+    }
+    __syncthreads();
+    const int r = threadIdx.x & 63;
+    if (row0 + r >= M) return;
+#pragma unroll
+    for (int k = 0; k < 32 / kF16Waves; k++) {
+        const int nl = (threadIdx.x >> 6) + kF16Waves * k, n = rg0 + nl;
+        if (n >= R) break;
+        float2 sum = red[(size_t)nl * kF16LdsPad + r];
+#pragma unroll
+        for (int w = 1; w < kF16Waves; w++) {
+            const float2 o = red[(size_t)(w * 32 + nl) * kF16LdsPad + r];
+            sum.x += o.x;
+            sum.y += o.y;
+        }
+        const float u = xinv[n];
+        part[((size_t)chunk * R + n) * ldp + row0 + r] = make_float2(sum.x * u, sum.y * u);
     }
 }
 
@@ -1258,7 +1299,10 @@ static int gemm_f16s_impl(const float2 *A, long lda, long M, long N, float amax,
     // 2^sa with amax 2^sa <= 2^13
     const float sa = amax > 0.f ? std::ldexp(1.0f, 13 - std::ilogb(amax) - 1) : 1.0f;
     const int nkb = (int)((N + kF16Cols - 1) / kF16Cols);
-    const int nch = (int)((N + kF16Chunk - 1) / kF16Chunk);
+    const int nch = (int)((N + kF16Waves * kF16Chunk - 1) / (kF16Waves * kF16Chunk));
+    // the kernel's lane offsets are 32-bit: 15 columns of A, and the planes of all R
+    if ((double)lda * 16 * sizeof(float2) >= 4294967296.0 || (double)R * nkb * 8 * sizeof(half8) >= 4294967296.0)
+        BEM_FAIL("FP16-split product: lda = %ld or %d x %ld right-hand-side planes too large", lda, R, N);
     const long ldp = (M + 63) / 64 * 64;
     auto *part = (float2 *)ws;
     auto *Xp = (_Float16 *)xs;
@@ -1270,7 +1314,7 @@ static int gemm_f16s_impl(const float2 *A, long lda, long M, long N, float amax,
     k_split_x<<<dim3((unsigned)((nkb * kF16Cols + 255) / 256), R), 256, 0, s>>>(X, ldx, (int)N, R, cmax, sa, Xp, nkb,
                                                                                  xinv);
     for (int rg = 0; rg < R; rg += 32)
-        gemm_n_f16s_mfma<2><<<dim3((unsigned)((M + 255) / 256), nch), 256, 0, s>>>(
+        gemm_n_f16s_mfma<2><<<dim3((unsigned)((M + 63) / 64), nch), 256, 0, s>>>(
             A, lda, (int)M, (int)N, Xp, nkb, R, rg, sa, xinv, kF16Chunk, part, ldp);
     HIP_CALL(hipGetLastError());
     timing_end(tr, s);
