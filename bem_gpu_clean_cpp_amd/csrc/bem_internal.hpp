@@ -298,16 +298,20 @@ long adjoint_copy_ld(long N);
 size_t adjoint_copy_bytes(long M, long N);
 // allocates *AH and fills it when the copy pays and the HBM is free (else *AH
 // stays null); adjoint_copy_fill refreshes an allocated copy
-int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s);
-int adjoint_copy_fill(const float2 *A, long lda, long M, long N, float2 *AH, hipStream_t s);
+// (with bmax, >= ws_adjoint_absmax(M, N) bytes, the copy also leaves per-tile
+// maxima of |Re A|, |Im A| there for adjoint_absmax_final)
+int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s,
+                      float *bmax = nullptr);
+int adjoint_copy_fill(const float2 *A, long lda, long M, long N, float2 *AH, hipStream_t s, float *bmax = nullptr);
 int gemm_hc_f32(const float2 *AH, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R, void *ws,
                 hipStream_t s);
 int gemm_hc_f32z(const float2 *AH, long M, long N, const float2 *T, long ldt, double2 *Z, long ldz, int R, void *ws,
                  hipStream_t s);
 // FP16-split products (A and X read as FP16 hi/lo pairs on FP16 MFMA, FP32
-// accumulation): amax bounds |Re A|, |Im A| (gemm_absmax); xs >= ws_split_x
-size_t ws_absmax();
-int gemm_absmax(const float2 *A, long lda, long M, long N, float *ws, float **out, hipStream_t s);
+// accumulation): amax bounds |Re A|, |Im A| (adjoint_absmax_final after a
+// copy made with bmax); xs >= ws_split_x
+size_t ws_adjoint_absmax(long M, long N);
+int adjoint_absmax_final(float *bmax, long M, long N, float **out, hipStream_t s);
 size_t ws_split_x(long N, int R);
 int gemm_n_f16s(const float2 *A, long lda, long M, long N, float amax, const float2 *X, long ldx, float2 *Y, long ldy,
                 int R, void *ws, void *xs, hipStream_t s);

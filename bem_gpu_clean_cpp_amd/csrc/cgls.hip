@@ -513,7 +513,6 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     HOST_CALL(salloc(&w.act, R, s));
     HOST_CALL(salloc(&w.done, R, s));
     HOST_CALL(salloc(&w.gws, std::max(std::max(ws_gemv_n(M, N, 1), ws_gemm(M, N, R)), ws_mres(M, N, R)), s));
-    if (f32 && R > 1) HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &w.AH, s));
     // Block steps pay three dozen small launches, a host wait and the R x R
     // algebra on the host (about 0.1 ms at R = 16, 0.3 ms at R = 32); below
     // about 4e8 of M N R a lockstep CG step is that cheap that the block's
@@ -522,6 +521,15 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     // takes the block at every size.
     bool block_on = block && R > 1 &&
                     ((double)M * (double)N * (double)R >= kBlockMinWork || getenv("BEM_BLOCK_CG_ALWAYS"));
+    // FP16-split block products: A and the block vectors read as FP16 hi/lo
+    // pairs on FP16 MFMA; the adjoint product reads the conjugate-transposed
+    // copy, which also leaves the bound amax on |Re A|, |Im A| (per-tile
+    // maxima).  At M = N = 40962 with 32 right-hand sides: 2.6 ms a product
+    // against 3.37 ms on the FP32 kernels, the same error against the FP64
+    // solve.  BEM_GEMM_NO_F16 keeps the FP32 products.
+    const bool want_f16s = block_on && f32 && R > 1 && getenv("BEM_GEMM_NO_F16") == nullptr;
+    if (want_f16s) HOST_CALL(salloc(&w.amaxws, ws_adjoint_absmax(M, N) / sizeof(float), s));
+    if (f32 && R > 1) HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &w.AH, s, w.amaxws));
     const int gblk = (N + kGramRows - 1) / kGramRows;
     if (block_on) {
         for (double2 **b : {&w.bQ, &w.bS, &w.bP, &w.bP2, &w.bW, &w.bX}) HOST_CALL(salloc(b, nv, s));
@@ -537,20 +545,12 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         HOST_CALL(PinnedPool::acquire(&w.hgram));
         w.hcm = w.hgram + kMaxPairs * mm;
     }
-    // FP16-split block products: A and the block vectors read as FP16 hi/lo
-    // pairs on FP16 MFMA; the adjoint product reads the conjugate-transposed
-    // copy.  amax bounds |Re A|, |Im A| (one pass over A).  At M = N = 40962
-    // with 32 right-hand sides: 3.06 ms a product against 3.37 ms on the FP32
-    // kernels, 43 block steps against 45, the same error against the FP64
-    // solve; the step 0.412 s against 0.451 s.  BEM_GEMM_NO_F16 keeps the
-    // FP32 products.
-    const bool f16s = block_on && f32 && w.AH && getenv("BEM_GEMM_NO_F16") == nullptr;
+    const bool f16s = want_f16s && w.AH;  // (the copy is skipped where memory is short)
     float amax = 0.f;
     if (f16s) {
         HOST_CALL(salloc(&w.xsplit, ws_split_x(std::max(M, N), R), s));
-        HOST_CALL(salloc(&w.amaxws, ws_absmax() / sizeof(float), s));
         float *dmax = nullptr;
-        HOST_CALL(gemm_absmax(A32, lda, M, N, w.amaxws, &dmax, s));
+        HOST_CALL(adjoint_absmax_final(w.amaxws, M, N, &dmax, s));
         HIP_CALL(hipMemcpyAsync(&amax, dmax, sizeof(float), hipMemcpyDeviceToHost, s));
         HIP_CALL(hipStreamSynchronize(s));
     }

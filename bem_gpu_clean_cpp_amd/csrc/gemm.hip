@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -929,11 +930,15 @@ int gemm_h_f32z(const float2 *A, long lda, long M, long N, const float2 *T, long
 // AH[j + i ldah] = conj(A[i + j lda]) for i < M, j < N; rows N .. ldah - 1 of
 // AH are zero.  64 x 64 tiles through LDS, so both the reads of A and the
 // writes of AH are 512-byte column segments (two complex values per lane).
+// With bmax, the tile's max(|Re|, |Im|) goes to bmax[tile] as well (the
+// bound of the FP16-split products, taken while A passes anyway).
 __global__ __launch_bounds__(256) void k_adjoint_copy(const float2 *__restrict__ A, long lda, int M, int N,
-                                                      float2 *__restrict__ AH, long ldah) {
+                                                      float2 *__restrict__ AH, long ldah, float *__restrict__ bmax) {
     __shared__ float2 tile[64][65];  // [column of A][row of A], one pad
+    __shared__ float red[4];
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
     const int rr = (threadIdx.x & 31) * 2, cc = threadIdx.x >> 5;
+    float m = 0.f;
     for (int c = cc; c < 64; c += 8) {
         const int j = j0 + c, i = i0 + rr;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -948,8 +953,16 @@ __global__ __launch_bounds__(256) void k_adjoint_copy(const float2 *__restrict__
         }
         tile[c][rr] = make_float2(v.x, v.y);
         tile[c][rr + 1] = make_float2(v.z, v.w);
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+    }
+    if (bmax) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     }
     __syncthreads();
+    if (bmax && threadIdx.x == 0)
+        bmax[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
     for (int r = cc; r < 64; r += 8) {
         const int i = i0 + r;
         if (i >= M) break;
@@ -967,20 +980,20 @@ bool adjoint_copy_pays(long M, long N, int R) {
 long adjoint_copy_ld(long N) { return (N + 63) / 64 * 64; }
 size_t adjoint_copy_bytes(long M, long N) { return sizeof(float2) * (size_t)adjoint_copy_ld(N) * (size_t)M; }
 
-int adjoint_copy_fill(const float2 *A, long lda, long M, long N, float2 *AH, hipStream_t s) {
+int adjoint_copy_fill(const float2 *A, long lda, long M, long N, float2 *AH, hipStream_t s, float *bmax) {
     if (M <= 0 || N <= 0) return EXIT_SUCCESS;
     if (lda % 2 || lda < M) BEM_FAIL("adjoint copy needs an even lda >= M");
     if ((N + 63) / 64 > 65535) BEM_FAIL("adjoint copy: N = %ld is too large", N);
     TimedRegion tr;
     timing_begin(tr, "adjoint_copy", s);
     k_adjoint_copy<<<dim3((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64)), 256, 0, s>>>(
-        A, lda, (int)M, (int)N, AH, adjoint_copy_ld(N));
+        A, lda, (int)M, (int)N, AH, adjoint_copy_ld(N), bmax);
     HIP_CALL(hipGetLastError());
     timing_end(tr, s);
     return EXIT_SUCCESS;
 }
 
-int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s) {
+int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s, float *bmax) {
     *AH = nullptr;
     if (!adjoint_copy_pays(M, N, R)) return EXIT_SUCCESS;
     const size_t bytes = adjoint_copy_bytes(M, N);
@@ -988,7 +1001,7 @@ int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 *
     HIP_CALL(hipMemGetInfo(&free_b, &total_b));
     if (free_b < bytes + ((size_t)2 << 30)) return EXIT_SUCCESS;  // keep 2 GiB for the rest of the solve
     HIP_CALL(hipMallocAsync((void **)AH, bytes, s));
-    const int rc = adjoint_copy_fill(A, lda, M, N, *AH, s);
+    const int rc = adjoint_copy_fill(A, lda, M, N, *AH, s, bmax);
     if (rc != EXIT_SUCCESS) {
         (void)hipFreeAsync(*AH, s);
         *AH = nullptr;
@@ -1072,13 +1085,16 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 constexpr int kF16Cols = 16;     // columns per MFMA step (K)
 constexpr int kF16Chunk = 640;   // split-K chunk, kMaxAccum columns
 
-// cmax[n] = max over rows of max(|Re X[:, n]|, |Im X[:, n]|) (one workgroup
-// per right-hand side, fixed-order tree)
-__global__ __launch_bounds__(256) void k_colmax(const float2 *__restrict__ X, long ldx, int N, float *__restrict__ cmax) {
+// cpart[n kColmaxBlocks + b] = max of max(|Re X[j, n]|, |Im X[j, n]|) over
+// the rows j = b 256 + t (mod kColmaxBlocks 256): workgroup (b, n); the split
+// takes the max of a column's partials.  (One workgroup per right-hand side
+// took 43 us at N = 40962, R = 32: 32 workgroups on 256 CUs.)
+constexpr int kColmaxBlocks = 16;
+__global__ __launch_bounds__(256) void k_colmax(const float2 *__restrict__ X, long ldx, int N, float *__restrict__ cpart) {
     __shared__ float red[4];
-    const int n = blockIdx.x;
+    const int n = blockIdx.y;
     float m = 0.f;
-    for (int j = threadIdx.x; j < N; j += 256) {
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < N; j += kColmaxBlocks * 256) {
         const float2 x = X[(size_t)n * ldx + j];
         m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
     }
@@ -1086,7 +1102,7 @@ __global__ __launch_bounds__(256) void k_colmax(const float2 *__restrict__ X, lo
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) cmax[n] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (threadIdx.x == 0) cpart[n * kColmaxBlocks + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
 // power of two p with v p <= 2^e (v > 0), else 1
@@ -1096,12 +1112,22 @@ __device__ __forceinline__ float pow2_scale(float v, int e) {
 }
 
 // Xp[((n kb-major) * 4 + p) * 16 + t] for right-hand side n, 16-column block kb,
-// planes p = Re hi, Re lo, Im hi, Im lo; xinv[n] = 2^-(sa + sx_n)
-__global__ void k_split_x(const float2 *__restrict__ X, long ldx, int N, int R, const float *__restrict__ cmax,
-                          float sa, _Float16 *__restrict__ Xp, int nkb, float *__restrict__ xinv) {
+// planes p = Re hi, Re lo, Im hi, Im lo; xinv[n] = 2^-(sa + sx_n); cpart holds
+// the kColmaxBlocks partial maxima of each column (k_colmax)
+__global__ __launch_bounds__(256) void k_split_x(const float2 *__restrict__ X, long ldx, int N, int R,
+                                                 const float *__restrict__ cpart, float sa, _Float16 *__restrict__ Xp,
+                                                 int nkb, float *__restrict__ xinv) {
+    __shared__ float cmax;
     const int n = blockIdx.y, e = blockIdx.x * 256 + threadIdx.x;  // e = kb * 16 + t
+    if (threadIdx.x < 64) {
+        float m = threadIdx.x < kColmaxBlocks ? cpart[n * kColmaxBlocks + threadIdx.x] : 0.f;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (threadIdx.x == 0) cmax = m;
+    }
+    __syncthreads();
     if (n >= R || e >= nkb * kF16Cols) return;
-    const float sx = pow2_scale(cmax[n], 10);
+    const float sx = pow2_scale(cmax, 10);
     if (e == 0) xinv[n] = 1.f / (sa * sx);
     float2 x = e < N ? X[(size_t)n * ldx + e] : make_float2(0.f, 0.f);
     x.x *= sx;
@@ -1242,23 +1268,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
-// bm[b] = max of max(|Re|, |Im|) over a column range of A; then out = max of bm
-// (fixed-order tree, one workgroup)
-__global__ __launch_bounds__(256) void k_absmax_part(const float2 *__restrict__ A, long lda, int M, int N,
-                                                     float *__restrict__ bm) {
-    __shared__ float red[4];
-    float m = 0.f;
-    for (int j = blockIdx.x; j < N; j += gridDim.x)
-        for (int i = threadIdx.x; i < M; i += 256) {
-            const float2 a = A[(size_t)j * lda + i];
-            m = fmaxf(m, fmaxf(fabsf(a.x), fabsf(a.y)));
-        }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) bm[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-}
+// out = max of bm[0 .. nb) (fixed-order tree, one workgroup)
 __global__ __launch_bounds__(256) void k_absmax_final(const float *__restrict__ bm, int nb, float *__restrict__ out) {
     __shared__ float red[4];
     float m = 0.f;
@@ -1270,24 +1280,26 @@ __global__ __launch_bounds__(256) void k_absmax_final(const float *__restrict__ 
     if (threadIdx.x == 0) out[0] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
-constexpr int kAbsmaxBlocks = 2048;
-size_t ws_absmax() { return sizeof(float) * (kAbsmaxBlocks + 1); }
-
-// *out (device) = max |Re A|, |Im A| over the M x N matrix; ws >= ws_absmax()
-int gemm_absmax(const float2 *A, long lda, long M, long N, float *ws, float **out, hipStream_t s) {
-    if (M <= 0 || N <= 0) BEM_FAIL("absmax of an empty matrix");
-    const int nb = (int)std::min<long>(kAbsmaxBlocks, N);
-    k_absmax_part<<<nb, 256, 0, s>>>(A, lda, (int)M, (int)N, ws);
-    k_absmax_final<<<1, 256, 0, s>>>(ws, nb, ws + kAbsmaxBlocks);
-    HIP_CALL(hipGetLastError());
-    *out = ws + kAbsmaxBlocks;
-    return EXIT_SUCCESS;
-}
-
 // split workspace: planes of up to 32 right-hand sides, their maxima and scales
 size_t ws_split_x(long N, int R) {
     const size_t nkb = (size_t)(N + kF16Cols - 1) / kF16Cols;
-    return sizeof(_Float16) * (size_t)std::max(1, R) * nkb * 64 + 2 * sizeof(float) * (size_t)std::max(1, R) + 256;
+    return sizeof(_Float16) * (size_t)std::max(1, R) * nkb * 64 +
+           (kColmaxBlocks + 1) * sizeof(float) * (size_t)std::max(1, R) + 256;
+}
+
+// the bound on |A| from the per-tile maxima k_adjoint_copy wrote: ws >=
+// ws_adjoint_absmax(M, N) floats' bytes, filled by adjoint_copy_make(.., ws);
+// *out (device) = the max
+size_t ws_adjoint_absmax(long M, long N) {
+    return sizeof(float) * ((size_t)((M + 63) / 64) * (size_t)((N + 63) / 64) + 1);
+}
+int adjoint_absmax_final(float *bmax, long M, long N, float **out, hipStream_t s) {
+    const long nb = ((M + 63) / 64) * ((N + 63) / 64);
+    if (nb <= 0 || nb > INT_MAX) BEM_FAIL("adjoint absmax: %ld tiles", nb);
+    k_absmax_final<<<1, 256, 0, s>>>(bmax, (int)nb, bmax + nb);
+    HIP_CALL(hipGetLastError());
+    *out = bmax + nb;
+    return EXIT_SUCCESS;
 }
 
 template <typename S, typename OC2>
@@ -1306,13 +1318,13 @@ static int gemm_f16s_impl(const float2 *A, long lda, long M, long N, float amax,
     const long ldp = (M + 63) / 64 * 64;
     auto *part = (float2 *)ws;
     auto *Xp = (_Float16 *)xs;
-    float *cmax = (float *)((char *)xs + sizeof(_Float16) * (size_t)R * nkb * 64);
-    float *xinv = cmax + R;
+    float *cpart = (float *)((char *)xs + sizeof(_Float16) * (size_t)R * nkb * 64);
+    float *xinv = cpart + (size_t)R * kColmaxBlocks;
     TimedRegion tr;
     timing_begin(tr, name, s);
-    k_colmax<<<R, 256, 0, s>>>(X, ldx, (int)N, cmax);
-    k_split_x<<<dim3((unsigned)((nkb * kF16Cols + 255) / 256), R), 256, 0, s>>>(X, ldx, (int)N, R, cmax, sa, Xp, nkb,
-                                                                                 xinv);
+    k_colmax<<<dim3(kColmaxBlocks, R), 256, 0, s>>>(X, ldx, (int)N, cpart);
+    k_split_x<<<dim3((unsigned)((nkb * kF16Cols + 255) / 256), R), 256, 0, s>>>(X, ldx, (int)N, R, cpart, sa, Xp,
+                                                                                 nkb, xinv);
     for (int rg = 0; rg < R; rg += 32)
         gemm_n_f16s_mfma<2><<<dim3((unsigned)((M + 63) / 64), nch), 256, 0, s>>>(
             A, lda, (int)M, (int)N, Xp, nkb, R, rg, sa, xinv, kF16Chunk, part, ldp);
