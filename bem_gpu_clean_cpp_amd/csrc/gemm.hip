@@ -1083,7 +1083,7 @@ int mh_f32a_hc(const float2 *AH, long M, long N, const double2 *T, long ldt, dou
 // of the time.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 constexpr int kF16Cols = 16;     // columns per MFMA step (K)
-constexpr int kF16Chunk = 640;   // split-K chunk, kMaxAccum columns
+constexpr int kF16Chunk = 640;   // columns a wave sums in FP32 (about kMaxAccum)
 
 // cpart[n kColmaxBlocks + b] = max of max(|Re X[j, n]|, |Im X[j, n]|) over
 // the rows j = b 256 + t (mod kColmaxBlocks 256): workgroup (b, n); the split
@@ -1311,7 +1311,13 @@ static int gemm_f16s_impl(const float2 *A, long lda, long M, long N, float amax,
     // 2^sa with amax 2^sa <= 2^13
     const float sa = amax > 0.f ? std::ldexp(1.0f, 13 - std::ilogb(amax) - 1) : 1.0f;
     const int nkb = (int)((N + kF16Cols - 1) / kF16Cols);
-    const int nch = (int)((N + kF16Waves * kF16Chunk - 1) / (kF16Waves * kF16Chunk));
+    // chunks of four wave sub-chunks of about kF16Chunk columns (a multiple of
+    // 16, up to 3 % longer): N = 40962 = 16 x 2560 + 2 takes 16 chunks of
+    // 4 x 656 columns rather than a 17th of 2 columns (641 workgroups that
+    // write a whole partial, and 21.3 rounds of 512 workgroups instead of 20.03)
+    const long nch0 = std::max(1L, (long)std::ceil((double)N / (kF16Waves * kF16Chunk * 1.03)));
+    const int kc = (int)(((N + kF16Waves * nch0 - 1) / (kF16Waves * nch0) + kF16Cols - 1) / kF16Cols * kF16Cols);
+    const int nch = (int)((N + (long)kF16Waves * kc - 1) / ((long)kF16Waves * kc));
     // the kernel's lane offsets are 32-bit: 15 columns of A, and the planes of all R
     if ((double)lda * 16 * sizeof(float2) >= 4294967296.0 || (double)R * nkb * 8 * sizeof(half8) >= 4294967296.0)
         BEM_FAIL("FP16-split product: lda = %ld or %d x %ld right-hand-side planes too large", lda, R, N);
@@ -1327,7 +1333,7 @@ static int gemm_f16s_impl(const float2 *A, long lda, long M, long N, float amax,
                                                                                  nkb, xinv);
     for (int rg = 0; rg < R; rg += 32)
         gemm_n_f16s_mfma<2><<<dim3((unsigned)((M + 63) / 64), nch), 256, 0, s>>>(
-            A, lda, (int)M, (int)N, Xp, nkb, R, rg, sa, xinv, kF16Chunk, part, ldp);
+            A, lda, (int)M, (int)N, Xp, nkb, R, rg, sa, xinv, kc, part, ldp);
     HIP_CALL(hipGetLastError());
     timing_end(tr, s);
     dim3 g2((unsigned)((M + 255) / 256), R);
