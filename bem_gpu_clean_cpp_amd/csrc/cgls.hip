@@ -147,6 +147,15 @@ __global__ void kc_start(int N, const double2 *__restrict__ s64, double2 *__rest
     d[o] = make_double2(0, 0);
 }
 
+// out (FP32) = in (FP64), n rows of each column r = blockIdx.y
+__global__ void kc_to_f32(int n, const double2 *__restrict__ in, long ldi, float2 *__restrict__ out, long ldo) {
+    const int r = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 v = in[(size_t)r * ldi + i];
+    out[(size_t)r * ldo + i] = make_float2((float)v.x, (float)v.y);
+}
+
 __global__ void kc_add(int N, const double2 *__restrict__ d, long ldd, double2 *__restrict__ X, long ldx) {
     const int r = blockIdx.y;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -565,6 +574,14 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
         if (x_zero) {
             HIP_CALL(hipMemcpy2DAsync(w.rr, sizeof(double2) * ld, B, sizeof(double2) * ldb, sizeof(double2) * M, R,
                                       hipMemcpyDeviceToDevice, s));
+            if (f16s) {
+                // A^H b from X = 0 only sets the first cycle's target and the
+                // reference norms (each later residual is FP64), so it runs on
+                // the FP16-split product: 2.6 ms instead of 5.7 at M = N = 40962
+                kc_to_f32<<<dim3((unsigned)((M + 255) / 256), R), 256, 0, s>>>((int)M, w.rr, ld, w.q32, ld);
+                HIP_CALL(hipGetLastError());
+                return gemm_hc_f16s_z(w.AH, M, N, amax, w.q32, ld, w.s64, ld, R, w.gws, w.xsplit, s);
+            }
         } else if (f32) {
             if (R == 1) HOST_CALL(resid_n_f32a(A32, lda, M, N, X, 0, B, 0, w.rr, 0, 1, w.gws, s));
             else HOST_CALL(mres_f32a(A32, lda, M, N, X, ldx, B, ldb, w.rr, ld, R, w.gws, s));
