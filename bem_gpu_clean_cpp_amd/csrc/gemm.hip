@@ -1268,16 +1268,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
-// out = max of bm[0 .. nb) (fixed-order tree, one workgroup)
+// out[g] = max of bm[b] over b = g 256 + t (mod gridDim 256)
 __global__ __launch_bounds__(256) void k_absmax_final(const float *__restrict__ bm, int nb, float *__restrict__ out) {
     __shared__ float red[4];
     float m = 0.f;
-    for (int b = threadIdx.x; b < nb; b += 256) m = fmaxf(m, bm[b]);
+    for (int b = blockIdx.x * 256 + threadIdx.x; b < nb; b += gridDim.x * 256) m = fmaxf(m, bm[b]);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
-    if (threadIdx.x == 0) out[0] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (threadIdx.x == 0) out[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
 // split workspace: planes of up to 32 right-hand sides, their maxima and scales
@@ -1290,15 +1290,19 @@ size_t ws_split_x(long N, int R) {
 // the bound on |A| from the per-tile maxima k_adjoint_copy wrote: ws >=
 // ws_adjoint_absmax(M, N) floats' bytes, filled by adjoint_copy_make(.., ws);
 // *out (device) = the max
+// (two levels: 256 workgroups, then one; one workgroup over the 410k tiles at
+// M = N = 40962 took 0.64 ms)
+constexpr int kAbsmaxGroups = 256;
 size_t ws_adjoint_absmax(long M, long N) {
-    return sizeof(float) * ((size_t)((M + 63) / 64) * (size_t)((N + 63) / 64) + 1);
+    return sizeof(float) * ((size_t)((M + 63) / 64) * (size_t)((N + 63) / 64) + kAbsmaxGroups + 1);
 }
 int adjoint_absmax_final(float *bmax, long M, long N, float **out, hipStream_t s) {
     const long nb = ((M + 63) / 64) * ((N + 63) / 64);
     if (nb <= 0 || nb > INT_MAX) BEM_FAIL("adjoint absmax: %ld tiles", nb);
-    k_absmax_final<<<1, 256, 0, s>>>(bmax, (int)nb, bmax + nb);
+    k_absmax_final<<<kAbsmaxGroups, 256, 0, s>>>(bmax, (int)nb, bmax + nb);
+    k_absmax_final<<<1, 256, 0, s>>>(bmax + nb, kAbsmaxGroups, bmax + nb + kAbsmaxGroups);
     HIP_CALL(hipGetLastError());
-    *out = bmax + nb;
+    *out = bmax + nb + kAbsmaxGroups;
     return EXIT_SUCCESS;
 }
 
