@@ -1076,11 +1076,11 @@ int mh_f32a_hc(const float2 *AH, long M, long N, const double2 *T, long ldt, dou
 // columns, as many as the FP32 kernel) and the four planes of X for its 8
 // columns (16 bytes each).
 //
-// Measured at M = N = 40962, R = 32, 64 chunks: 2.93-2.98 ms with the X split
-// (standalone), 3.06 ms in the block CG solve, against 3.30-3.44 ms for the
-// FP32 kernel (three-product form, paired X, two row tiles).  The waves wait
-// on the A stream (4.5 TB/s); the MFMA and VALU work each take about a fifth
-// of the time.
+// Measured at M = N = 40962, R = 32: 2.55-2.61 ms a product in the block CG
+// solve (5.2-5.3 TB/s of A; a read-only kernel with this access pattern
+// reaches 6.2 TB/s), against 3.30-3.44 ms for the FP32 kernel (three-product
+// form, paired X, two row tiles).  The waves wait on the A stream: MFMA busy
+// 24 % of the cycles at 1.97 GHz.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 constexpr int kF16Cols = 16;     // columns per MFMA step (K)
 constexpr int kF16Chunk = 640;   // columns a wave sums in FP32 (about kMaxAccum)
