@@ -372,8 +372,10 @@ __global__ __launch_bounds__(256) void kc_block_mul(const double2 *__restrict__ 
 #pragma unroll
     for (int bb = 0; bb < kMulCols; bb++)
         acc[bb] = (Z && bb < nb) ? Z[(size_t)(b0 + bb) * ld + i] : make_double2(0, 0);
-    for (int a = 0; a < Rb; a++) {
-        const double2 u = U[(size_t)a * ld + i];
+    // U's columns in batches of 8 loads in flight (22.7 us against 25.4 with
+    // one at a time at N = 40962, Rb = 32; the same sums in the same order)
+    constexpr int kMulBatch = 8;
+    auto madd = [&](const double2 u, int a) {
 #pragma unroll
         for (int bb = 0; bb < kMulCols; bb++) {
             if (bb < nb) {
@@ -382,7 +384,16 @@ __global__ __launch_bounds__(256) void kc_block_mul(const double2 *__restrict__ 
                 acc[bb].y = fma(u.x, c.y, fma(u.y, c.x, acc[bb].y));
             }
         }
+    };
+    int a0 = 0;
+    for (; a0 + kMulBatch <= Rb; a0 += kMulBatch) {
+        double2 uu[kMulBatch];
+#pragma unroll
+        for (int q = 0; q < kMulBatch; q++) uu[q] = U[(size_t)(a0 + q) * ld + i];
+#pragma unroll
+        for (int q = 0; q < kMulBatch; q++) madd(uu[q], a0 + q);
     }
+    for (int a = a0; a < Rb; a++) madd(U[(size_t)a * ld + i], a);
 #pragma unroll
     for (int bb = 0; bb < kMulCols; bb++) {
         if (bb < nb) {
