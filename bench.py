@@ -429,6 +429,28 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
                                                  "gemm_hc_f16s", "adjoint_copy", "asm_tile_f32", "sing_elems",
                                                  "asm_finalize", "ghost_reduce", "mres_f64acc", "mh_f64acc")}
     bem.timing_reset()
+    # the same step with block CG's products on the FP32 kernels (BEM_GEMM_NO_F16),
+    # reported beside the headline: one warm-up, then the median of 2 steps
+    f32p = None
+    if method == "block":
+        os.environ["BEM_GEMM_NO_F16"] = "1"
+        try:
+            step()
+            bem.sync()
+            lat32 = []
+            for i in range(2):
+                ts = time.perf_counter()
+                st32 = step()
+                bem.sync()
+                lat32.append((time.perf_counter() - ts) * 1e3)
+            x32f = sv.solution()
+        finally:
+            del os.environ["BEM_GEMM_NO_F16"]
+        ms32 = float(np.median(lat32))
+        f32p = {"note": "block CG products on the FP32 32x32x2 MFMA kernels (BEM_GEMM_NO_F16), same step",
+                "ms_per_step": round(ms32, 2), "value": round(R / (ms32 / 1e3), 3),
+                "solve": {kk: st32[kk] for kk in ("iterations", "refinements", "converged", "residual")}}
+        log("large, FP32 products: %.3f s, %s" % (ms32 / 1e3, st32))
     sv.close()
     avg = {kk: (v[0] / v[1] if v[1] else None) for kk, v in t.items()}
     flops = 8.0 * M * N * R
@@ -492,6 +514,8 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
                            "adjoint": {"kernel": hname, "avg_ms": round(h_avg, 3), "achieved": round(hgbs, 1),
                                        "frac": round(hgbs / PEAK_HBM_GBS, 4),
                                        "traffic": measured_traffic(hname, "large")}}
+    if f32p is not None:
+        out["fp32_products"] = f32p
     if avg["mres_f64acc"]:
         out["roofline_fp64_residual"] = {"bound": "mfma", "kernel": "mres_f64acc (FP64 MFMA, FP32 A widened)",
                                          "avg_ms": round(avg["mres_f64acc"], 3),
@@ -519,6 +543,8 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
         log("large fp64: %.2f s, stats %s, err %.3g" % (t64, st64, err))
         out.update({"rel_err_vs_fp64": err, "fp64_single_rhs_s": round(t64, 3),
                     "speedup_vs_fp64_single_rhs": round(out["value"] * t64, 2)})
+        if f32p is not None:
+            f32p["rel_err_vs_fp64"] = float(np.linalg.norm(x32f[:, 0] - x64[:, 0]) / np.linalg.norm(x64[:, 0]))
     if cpu:
         products = 2 * st["iterations"] + 2 * (st["refinements"] + 1)
         out["cpu_baseline"] = cpu_baseline_large(m, k, rho, c, R, products, host_cores())
