@@ -530,7 +530,7 @@ def test_pulsating_sphere_reference_golden():
         bem.set_medium(RHO, SPEED)
 
 
-def test_rigid_sphere_series_cfg1():
+def test_rigid_sphere_series_s3_171hz():
     """Plain small case: icosphere s3, r = 0.1 m, 171 Hz, 8 CHIEF, plane wave +z."""
     a = 0.1
     m = _icosphere(3, a, nchief=8)
