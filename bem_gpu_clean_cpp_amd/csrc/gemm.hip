@@ -356,6 +356,7 @@ __global__ __launch_bounds__(256) void gemm_h_mfma(const float2 *__restrict__ A,
 typedef float v16f __attribute__((ext_vector_type(16)));
 constexpr int kRingN = 4;   // Y = A X: buffers in the ring
 constexpr int kRingPX = 2;  // (buffers of two k steps, paired X loads)
+constexpr int kF32LdsPad = 65;  // float2 per right-hand side in the LDS partials (bank spread)
 constexpr int kWavesN = 3;  // waves per SIMD it is built for
 constexpr int kRingH = 2;   // Z = A^H T
 constexpr int kStepsH = 4;  // k steps per buffer of the adjoint product
@@ -385,16 +386,23 @@ __device__ __forceinline__ int d32_row(int lane, int v) { return 8 * (v / 4) + 4
 // form's, normwise (|A| |X| scale).
 //
 // Needs lda >= roundup(M, 64).
-template <bool G3, bool PX, int RING = kRingN, int WAVES = kWavesN, int RT = 1>
+//
+// LDSR: a workgroup is one block of 64 RT rows over four consecutive column
+// sub-chunks of kc columns, one per wave; the four partials of each row tile
+// are summed in LDS in wave order and written as contiguous 64-row columns
+// (partial chunk blockIdx.y).  Without it, a workgroup is four row blocks
+// over one chunk, each wave writing its own partial.
+template <bool G3, bool PX, int RING = kRingN, int WAVES = kWavesN, int RT = 1, bool LDSR = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES))) void gemm_n_mfma32(
     const float2 *__restrict__ A, long lda, int M, int N, const float2 *__restrict__ X, long ldx, int R, int rg0,
     int kc, float2 *__restrict__ part, long ldp) {
     // RT row tiles of 64 complex rows per wave share every X load
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int row0 = (blockIdx.x * 4 + wave) * 64 * RT;
-    if (row0 >= M) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = LDSR ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : threadIdx.x >> 6;
+    const int row0 = LDSR ? blockIdx.x * 64 * RT : (blockIdx.x * 4 + wave) * 64 * RT;
+    if (row0 >= M) return;  // (LDSR: never, the grid covers M exactly)
     const int chunk = blockIdx.y;
-    const int j0 = chunk * kc, j1 = min(N, j0 + kc);
+    const int j0 = LDSR ? min(N, (chunk * 4 + wave) * kc) : chunk * kc, j1 = min(N, j0 + kc);
     const int i = lane & 31, k = lane >> 5;
     constexpr int NA = G3 ? 6 : 8;  // accumulators per row tile
     v16f acc[RT][NA];
@@ -504,6 +512,52 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES, WAVE
     for (int jb = jend; jb < j1; jb += W) {
         load_masked(jb, b[0]);
         mul(b[0]);
+    }
+    if constexpr (LDSR) {
+        // 16 right-hand sides at a time through 33 KB of LDS (three workgroups
+        // per CU still fit at one row tile a wave)
+        __shared__ float2 red[4 * 16 * kF32LdsPad];
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+#pragma unroll
+            for (int hf = 0; hf < 2; hf++) {
+                if (rt + hf > 0) __syncthreads();  // the previous sums have read red
+                if ((i >> 4) == hf) {
+                    float2 *rw = red + (size_t)(wave * 16 + (i & 15)) * kF32LdsPad;
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        const int d = 2 * d32_row(lane, v);
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            if constexpr (G3)
+                                rw[d + h] = make_float2(acc[rt][3 * h][v] - acc[rt][3 * h + 1][v],
+                                                        acc[rt][3 * h][v] + acc[rt][3 * h + 2][v]);
+                            else
+                                rw[d + h] = make_float2(acc[rt][4 * h][v] - acc[rt][4 * h + 3][v],
+                                                        acc[rt][4 * h + 1][v] + acc[rt][4 * h + 2][v]);
+                        }
+                    }
+                }
+                __syncthreads();
+                const int r = threadIdx.x & 63, rowr = row0 + 64 * rt + r;
+                if (rowr < M) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int nl = (threadIdx.x >> 6) + 4 * q, n = rg0 + 16 * hf + nl;
+                        if (n >= R) break;
+                        float2 sum = red[(size_t)nl * kF32LdsPad + r];
+#pragma unroll
+                        for (int w = 1; w < 4; w++) {
+                            const float2 o = red[(size_t)(w * 16 + nl) * kF32LdsPad + r];
+                            sum.x += o.x;
+                            sum.y += o.y;
+                        }
+                        part[((size_t)chunk * R + n) * ldp + rowr] = sum;
+                    }
+                }
+            }
+        }
+        return;
     }
     const int n = rg0 + i;
     if (n >= R) return;
@@ -803,27 +857,39 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
     }
     const long ldp = (M + 63) / 64 * 64;
     auto *part = (typename MF<T>::c2 *)ws;
+    // paired X loads read X[(n) ldx + j .. j + 1] as one 16-byte load (an X
+    // that cannot be read in pairs takes the four-product form)
+    const bool px = w32 && ldx % 2 == 0 && ((uintptr_t)X & 15) == 0 && (N % 2 == 0 || ldx > N);
+    // The three-product kernel with paired X loads sums the partials of four
+    // column sub-chunks per workgroup in LDS, so it writes a quarter of the
+    // partials: chunks of four sub-chunks of about kc columns (an even count,
+    // up to 3 % longer, so that no near-empty last chunk is left).  At M =
+    // N = 40962, R = 32: 16 chunks of 4 x 672 columns instead of 65 of 640.
+    const bool lds = px && use_3m();
+    int nred = nch, kc4 = kc;
+    if (lds) {
+        const long n4 = std::max(1L, (long)std::ceil((double)N / (4.0 * kc * 1.03)));
+        kc4 = (int)(((N + 4 * n4 - 1) / (4 * n4) + 1) / 2 * 2);
+        nred = (int)((N + 4L * kc4 - 1) / (4L * kc4));
+    }
+    const bool rt2 = lds && two_row_tiles(M, nch);
     TimedRegion tr;
     timing_begin(tr, name, s);
     const unsigned gx = (unsigned)((M + 127) / 128);
     for (int rg = 0; rg < rper; rg += 32) {
         if (w32) {
-            const dim3 grid((unsigned)((M + 255) / 256), nch);
-            // paired X loads read X[(n) ldx + j .. j + 1] as one 16-byte load
-            const bool px = ldx % 2 == 0 && ((uintptr_t)X & 15) == 0 && (N % 2 == 0 || ldx > N);
-            auto go = [&](auto kern) {
-                kern<<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc, (float2 *)part,
-                                          ldp);
-            };
-            // (an X that cannot be read in pairs takes the four-product form)
-            if (use_3m() && px && two_row_tiles(M, nch)) {
-                const dim3 g2((unsigned)((M + 511) / 512), nch);
-                gemm_n_mfma32<true, true, kRingPX, 2, 2><<<g2, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X,
-                                                                           ldx, R, rg, kc, (float2 *)part, ldp);
-            } else if (use_3m() && px) {
-                go(gemm_n_mfma32<true, true, kRingPX>);
+            if (rt2) {
+                const dim3 g2((unsigned)((M + 127) / 128), nred);
+                gemm_n_mfma32<true, true, kRingPX, 2, 2, true><<<g2, 256, 0, s>>>(
+                    A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc4, (float2 *)part, ldp);
+            } else if (lds) {
+                const dim3 g1((unsigned)((M + 63) / 64), nred);
+                gemm_n_mfma32<true, true, kRingPX, kWavesN, 1, true><<<g1, 256, 0, s>>>(
+                    A, lda, (int)M, (int)N, (const float2 *)X, ldx, R, rg, kc4, (float2 *)part, ldp);
             } else {
-                go(gemm_n_mfma32<false, false>);
+                const dim3 grid((unsigned)((M + 255) / 256), nch);
+                gemm_n_mfma32<false, false><<<grid, 256, 0, s>>>(A, lda, (int)M, (int)N, (const float2 *)X, ldx, R,
+                                                                 rg, kc, (float2 *)part, ldp);
             }
         } else {
             dim3 grid(gx, nch, nmat);
@@ -849,7 +915,7 @@ static int gemm_n_impl(const float2 *A, long lda, long M, long N, const typename
     }
     timing_end(tr, s);
     dim3 g2((unsigned)((M + 255) / 256), R);
-    gemm_reduce<T, S, OC2><<<g2, 256, 0, s>>>(part, ldp, nch, R, (int)M, b, ldb, Y, ldy);
+    gemm_reduce<T, S, OC2><<<g2, 256, 0, s>>>(part, ldp, w32 ? nred : nch, R, (int)M, b, ldb, Y, ldy);
     HIP_CALL(hipGetLastError());
     return EXIT_SUCCESS;
 }
