@@ -299,9 +299,10 @@ size_t adjoint_copy_bytes(long M, long N);
 // allocates *AH and fills it when the copy pays and the HBM is free (else *AH
 // stays null); adjoint_copy_fill refreshes an allocated copy
 // (with bmax, >= ws_adjoint_absmax(M, N) bytes, the copy also leaves per-tile
-// maxima of |Re A|, |Im A| there for adjoint_absmax_final)
+// maxima of |Re A|, |Im A| there for adjoint_absmax_final; force makes it for
+// 16 or fewer right-hand sides too, where the FP16-split products need it)
 int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s,
-                      float *bmax = nullptr);
+                      float *bmax = nullptr, bool force = false);
 int adjoint_copy_fill(const float2 *A, long lda, long M, long N, float2 *AH, hipStream_t s, float *bmax = nullptr);
 int gemm_hc_f32(const float2 *AH, long M, long N, const float2 *T, long ldt, float2 *Z, long ldz, int R, void *ws,
                 hipStream_t s);

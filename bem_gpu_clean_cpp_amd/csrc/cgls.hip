@@ -552,7 +552,7 @@ int cgls_device(const float2 *A32, const double2 *A64, long lda, int M, int N, c
     // solve.  BEM_GEMM_NO_F16 keeps the FP32 products.
     const bool want_f16s = block_on && f32 && R > 1 && getenv("BEM_GEMM_NO_F16") == nullptr;
     if (want_f16s) HOST_CALL(salloc(&w.amaxws, ws_adjoint_absmax(M, N) / sizeof(float), s));
-    if (f32 && R > 1) HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &w.AH, s, w.amaxws));
+    if (f32 && R > 1) HOST_CALL(adjoint_copy_make(A32, lda, M, N, R, &w.AH, s, w.amaxws, want_f16s));
     const int gblk = (N + kGramRows - 1) / kGramRows;
     if (block_on) {
         for (double2 **b : {&w.bQ, &w.bS, &w.bP, &w.bP2, &w.bW, &w.bX}) HOST_CALL(salloc(b, nv, s));

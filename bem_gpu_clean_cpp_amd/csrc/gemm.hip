@@ -1059,9 +1059,11 @@ int adjoint_copy_fill(const float2 *A, long lda, long M, long N, float2 *AH, hip
     return EXIT_SUCCESS;
 }
 
-int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s, float *bmax) {
+int adjoint_copy_make(const float2 *A, long lda, long M, long N, int R, float2 **AH, hipStream_t s, float *bmax,
+                      bool force) {
     *AH = nullptr;
-    if (!adjoint_copy_pays(M, N, R)) return EXIT_SUCCESS;
+    if (M <= 0 || N <= 0 || getenv("BEM_NO_ADJOINT_COPY") || (!force && !adjoint_copy_pays(M, N, R)))
+        return EXIT_SUCCESS;
     const size_t bytes = adjoint_copy_bytes(M, N);
     size_t free_b = 0, total_b = 0;
     HIP_CALL(hipMemGetInfo(&free_b, &total_b));

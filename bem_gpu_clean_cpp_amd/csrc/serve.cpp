@@ -190,10 +190,14 @@ double bem_request_footprint(int M, int N, int E, int nrhs, int precision, int m
         b += sizeof(double2) * (K + 2) * (2 + ((size_t)std::max(M, N) + 1023) / 1024) * R;
         if (method == BEM_SOLVE_GMRES && precision != BEM_PREC_FP64 && nrhs == 1 && M > N && ne_fused_fits(M, N))
             b += sizeof(float2) * (size_t)M * ne_ldr(N) + std::max(ws_ne_resid(M, N), ws_ne_rhs(M, N));  // row-major copy of A
-        if (precision != BEM_PREC_FP64 && nrhs > 1 && M > N && adjoint_copy_pays(M, N, nrhs))
-            b += adjoint_copy_bytes(M, N);  // conjugate-transposed copy of A
-        if (method == BEM_SOLVE_BLOCK_CG)  // block vectors, Gram partials
-            b += (6 * sizeof(double2) + sizeof(float2)) * ldv * R + sizeof(double2) * 4096 * ((N + 63) / 64 + 21);
+        // conjugate-transposed copy of A (block CG makes it for its FP16-split
+        // products at any count above one)
+        if (precision != BEM_PREC_FP64 && nrhs > 1 &&
+            ((M > N && adjoint_copy_pays(M, N, nrhs)) || method == BEM_SOLVE_BLOCK_CG))
+            b += adjoint_copy_bytes(M, N);
+        if (method == BEM_SOLVE_BLOCK_CG)  // block vectors, Gram partials, the split's planes and bound
+            b += (6 * sizeof(double2) + sizeof(float2)) * ldv * R + sizeof(double2) * 4096 * ((N + 63) / 64 + 21) +
+                 ws_split_x(std::max(M, N), nrhs) + ws_adjoint_absmax(M, N);
     }
     b += sizeof(float2) * R * ld;  // right-hand-side partials
     return (double)b * 1.05 + (1 << 20);

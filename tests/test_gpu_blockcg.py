@@ -168,16 +168,17 @@ def test_block_cg_size_policy_small_system_takes_lockstep(monkeypatch):
     assert np.array_equal(xb, xl)
 
 
-@pytest.mark.parametrize("R", [24, 36])
+@pytest.mark.parametrize("R", [8, 24, 36])
 def test_block_cg_fp16_split_products(monkeypatch, R):
     """Block CG's products on the FP16-split kernels (the default with an FP32
     matrix) against the FP32 kernels (BEM_GEMM_NO_F16): per-launch timing
     shows which kernels ran (one product pair per block step); point sources
     at 0.12 m to 120 m from the centre of a 0.1 m sphere give right-hand sides
     whose norms differ by about a thousand, which the per-column scales of the
-    split absorb; 36 right-hand sides take two launch groups, the second
-    ragged.  Both solutions match the oracle's least-squares solution at 1e-5
-    and each other at 1e-9 after FP64 refinement."""
+    split absorb; 8 right-hand sides fill a quarter of a launch group (the
+    conjugate-transposed copy is made for them too), 36 take two groups, the
+    second ragged.  Both solutions match the oracle's least-squares solution
+    at 1e-5 and each other at 1e-9 after FP64 refinement."""
     m = _mesh(3, 7)
     k = 27.1
     u = C.sphere_points(41, R, 1.0)
