@@ -50,6 +50,9 @@ def executed_share():
     return 1.0 if os.environ.get("BEM_GEMM_4M") else 0.75
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 PEAK_FP16_TFLOPS = 2500.0  # MI355X dense FP16 MFMA
+# measured: a read-only kernel (64-row x 640-column wave tiles, 16-byte loads,
+# 2 workgroups per CU) streams 13.4 GB at 6.15-6.24 TB/s on MI355X
+READ_CEILING_GBS = 6200.0
 FLOP_PER_EVAL = 80.0       # nominal FP32 flops per (row, quadrature point) evaluation
 # measured VALU issue rate of the FP32 assembly kernel (wave-instructions per
 # SIMD-cycle, SQ counters of profiles/precision_s5_asm_valu_counters.txt)
@@ -504,6 +507,9 @@ def bench_large(bem, C, steps, warmup, fp64=True, cpu=True, method="block", refi
         hgbs = gbytes / (h_avg * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "kernel": nname, "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
                            "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                           "read_ceiling": {"gbs": READ_CEILING_GBS, "frac": round(gbs / READ_CEILING_GBS, 4),
+                                            "note": "a read-only kernel with the product's access pattern over "
+                                                    "the same 13.4 GB on MI355X (measured once)"},
                            "traffic": measured_traffic(nname, "large"), "avg_ms": round(g_avg, 3),
                            "launches": t[nname][1], "bytes_per_launch": gbytes, "flops_per_launch": flops,
                            "complex_tflops": round(flops / (g_avg * 1e-3) / 1e12, 2),
