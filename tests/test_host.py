@@ -291,6 +291,11 @@ def test_request_footprint_bounds_the_allocations():
     assert bem.request_footprint(M, N, E, 1, method=bem.SOLVE_QR) >= 16.0 * ld * N
     assert bem.request_footprint(M, N, E, 1, precision=bem.PREC_FP64) >= 16.0 * ld * N
     assert bem.request_footprint(N - 1, N, E, 1) < 0  # underdetermined: refused
+    # block CG keeps the conjugate-transposed copy (for its FP16-split
+    # products) at any count above one; lockstep CG only above 16
+    blk8 = bem.request_footprint(M, N, E, 8, method=bem.SOLVE_BLOCK_CG)
+    cg8 = bem.request_footprint(M, N, E, 8, method=bem.SOLVE_CGLS)
+    assert blk8 >= cg8 + 8.0 * M * ((N + 63) // 64 * 64)
 
 
 def test_tune_table_roundtrip(tmp_path):
